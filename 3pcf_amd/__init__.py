@@ -1,0 +1,282 @@
+"""Host-side Python binding of libtpcf, the MI355X triplet counter.
+
+The package directory is ``3pcf_amd`` (not a Python identifier); import it
+with ``importlib.import_module("3pcf_amd")``.
+
+The library is a C ABI (``include/tpcf.h``); this module wraps it with
+ctypes.  Nothing here computes histograms: every count comes from the HIP
+kernels in ``libtpcf.so``.  If the library is missing or no gfx950 device is
+present, the calls raise instead of falling back to anything else.
+
+Reference interface being replaced: the command-line programs
+``src/3pcf_C_version_triangle_parametrization.cc`` (main at :245-571) and
+``src/3pcf_C_version.cc`` (main at :147-466): three catalogs in, one histogram
+out.  :func:`count_files` mirrors that call; :class:`Context` exposes the
+library's catalogs, jobs, batches and the serving queue.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Iterable, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtpcf.so")
+BIN_DIR = os.path.join(HERE, "bin")
+
+R123, SQT, JBJ = 0, 1, 2
+LIN, LOG = 0, 1
+MODE_FAST, MODE_PLAIN, MODE_FAST64 = 0, 1, 2
+PARAMS = {"r123": R123, "sqt": SQT, "jbj": JBJ}
+MODES = {"fast": MODE_FAST, "plain": MODE_PLAIN, "fast64": MODE_FAST64}
+
+# DDD, DDR, DRR, RRR over segment 0 = data, segment 1 = randoms.
+FOUR_TERMS = ((0, 0, 0), (0, 0, 1), (0, 1, 1), (1, 1, 1))
+TERM_NAMES = ("DDD", "DDR", "DRR", "RRR")
+
+
+class TpcfError(RuntimeError):
+    """A libtpcf call returned an error status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libtpcf error {code}: {msg}")
+        self.code = code
+
+
+class Axis(C.Structure):
+    _fields_ = [("n", C.c_int), ("lo", C.c_double), ("hi", C.c_double), ("scale", C.c_int)]
+
+
+class BinSpec(C.Structure):
+    _fields_ = [("param", C.c_int), ("ax", Axis * 3), ("recheck_rel", C.c_double)]
+
+
+class Job(C.Structure):
+    _fields_ = [
+        ("nseg", C.c_int),
+        ("seg", C.c_void_p * 3),
+        ("seg_lo", C.c_int64 * 3),
+        ("seg_hi", C.c_int64 * 3),
+        ("nterms", C.c_int),
+        ("term", (C.c_int * 3) * 4),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("triplets", C.c_uint64),
+        ("pairs", C.c_uint64),
+        ("pairs_rechecked", C.c_uint64),
+        ("device_ms", C.c_double),
+        ("wall_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# Every symbol include/tpcf.h declares, with its ctypes signature.
+_SIGNATURES = {
+    "tpcf_version": (C.c_char_p, []),
+    "tpcf_last_error": (C.c_char_p, []),
+    "tpcf_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "tpcf_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "tpcf_ctx_set_memory_budget": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "tpcf_catalog_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),
+    "tpcf_catalog_destroy": (C.c_int, [C.c_void_p]),
+    "tpcf_catalog_size": (C.c_int64, [C.c_void_p]),
+    "tpcf_read_catalog": (C.c_int64, [C.c_char_p, C.c_int, C.c_double, C.POINTER(C.POINTER(C.c_double))]),
+    "tpcf_free": (None, [C.c_void_p]),
+    "tpcf_hist_cells": (C.c_int64, [C.POINTER(BinSpec)]),
+    "tpcf_axis_edges": (C.c_int, [C.POINTER(Axis), C.POINTER(C.c_double)]),
+    "tpcf_binspec_check": (C.c_int, [C.POINTER(BinSpec)]),
+    "tpcf_count": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(BinSpec), C.c_int, C.c_void_p, C.POINTER(Stats)]),
+    "tpcf_count_batch": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(Job), C.POINTER(BinSpec), C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(Stats)],
+    ),
+    "tpcf_submit": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(Job), C.POINTER(BinSpec), C.c_int, C.c_void_p, C.POINTER(C.c_int64)],
+    ),
+    "tpcf_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Stats)]),
+    "tpcf_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Stats)]),
+    "tpcf_job_footprint": (C.c_int64, [C.POINTER(Job), C.POINTER(BinSpec), C.c_int]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libtpcf.so (built in-tree by ``__graft_entry__.build()``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise TpcfError(rc, load_library().tpcf_last_error().decode())
+    return rc
+
+
+def axis(n: int, lo: float, hi: float, scale: str | int = "lin") -> Axis:
+    s = scale if isinstance(scale, int) else (LOG if scale == "log" else LIN)
+    return Axis(int(n), float(lo), float(hi), s)
+
+
+def binspec(param: str | int, axes: Sequence[Axis], recheck_rel: float = 1e-5) -> BinSpec:
+    p = param if isinstance(param, int) else PARAMS[param]
+    spec = BinSpec()
+    spec.param = p
+    for i, a in enumerate(axes):
+        spec.ax[i] = a
+    spec.recheck_rel = recheck_rel
+    return spec
+
+
+def r123_spec(n: int, lo: float, hi: float, scale: str = "lin", recheck_rel: float = 1e-5) -> BinSpec:
+    return binspec(R123, [axis(n, lo, hi, scale)], recheck_rel)
+
+
+def hist_cells(spec: BinSpec) -> int:
+    return _check(load_library().tpcf_hist_cells(C.byref(spec)))
+
+
+def axis_edges(ax: Axis) -> np.ndarray:
+    e = np.zeros(ax.n + 1, dtype=np.float64)
+    _check(load_library().tpcf_axis_edges(C.byref(ax), e.ctypes.data_as(C.POINTER(C.c_double))))
+    return e
+
+
+def read_catalog(path: str, fmt: int = 7, scale: float = 1.0) -> np.ndarray:
+    """Read a catalog file into an (n, 3) float64 array (library reader)."""
+    lib = load_library()
+    ptr = C.POINTER(C.c_double)()
+    n = _check(lib.tpcf_read_catalog(path.encode(), fmt, scale, C.byref(ptr)))
+    try:
+        soa = np.ctypeslib.as_array(ptr, shape=(3 * n,)).copy() if n else np.zeros(0)
+    finally:
+        lib.tpcf_free(ptr)
+    return soa.reshape(3, n).T.copy() if n else np.zeros((0, 3))
+
+
+class Catalog:
+    """A catalog resident in device memory."""
+
+    def __init__(self, ctx: "Context", xyz: np.ndarray):
+        xyz = np.asarray(xyz, dtype=np.float64)
+        if xyz.ndim != 2 or xyz.shape[1] != 3:
+            raise ValueError("catalog must be an (n, 3) array")
+        soa = np.ascontiguousarray(xyz.T)
+        self.ctx = ctx
+        self.n = xyz.shape[0]
+        self.handle = C.c_void_p()
+        _check(ctx.lib.tpcf_catalog_create(ctx.handle, soa.ctypes.data, self.n, C.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.tpcf_catalog_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_job(segments: Sequence[tuple], terms: Iterable[tuple]) -> Job:
+    """segments: (Catalog, lo, hi) triples; terms: sorted label triples."""
+    job = Job()
+    job.nseg = len(segments)
+    for s, (cat, lo, hi) in enumerate(segments):
+        job.seg[s] = cat.handle.value
+        job.seg_lo[s] = lo
+        job.seg_hi[s] = hi
+    terms = list(terms)
+    job.nterms = len(terms)
+    for t, tr in enumerate(terms):
+        for k in range(3):
+            job.term[t][k] = tr[k]
+    return job
+
+
+class Context:
+    """A libtpcf context bound to one device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        self.handle = C.c_void_p()
+        _check(self.lib.tpcf_ctx_create(device, C.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            self.lib.tpcf_ctx_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def catalog(self, xyz: np.ndarray) -> Catalog:
+        return Catalog(self, xyz)
+
+    def set_memory_budget(self, nbytes: int):
+        _check(self.lib.tpcf_ctx_set_memory_budget(self.handle, int(nbytes)))
+
+    def count(self, job: Job, spec: BinSpec, mode: int = MODE_FAST):
+        """Histograms of every term of job: array (nterms, cells) uint64, Stats."""
+        cells = hist_cells(spec)
+        out = np.zeros((job.nterms, cells), dtype=np.uint64)
+        st = Stats()
+        _check(self.lib.tpcf_count(self.handle, C.byref(job), C.byref(spec), mode, out.ctypes.data, C.byref(st)))
+        return out, st
+
+    def count_batch(self, jobs: Sequence[Job], specs: Sequence[BinSpec], mode: int = MODE_FAST):
+        n = len(jobs)
+        JA = (Job * n)(*jobs)
+        SA = (BinSpec * n)(*specs)
+        outs = [np.zeros((j.nterms, hist_cells(s)), dtype=np.uint64) for j, s in zip(jobs, specs)]
+        ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
+        st = Stats()
+        _check(self.lib.tpcf_count_batch(self.handle, JA, SA, n, mode, ptrs, C.byref(st)))
+        return outs, st
+
+    def footprint(self, job: Job, spec: BinSpec, mode: int = MODE_FAST) -> int:
+        return _check(self.lib.tpcf_job_footprint(C.byref(job), C.byref(spec), mode))
+
+    def submit(self, job: Job, spec: BinSpec, mode: int = MODE_FAST):
+        """Enqueue a job; returns (ticket, output array).  Keep the array alive."""
+        out = np.zeros((job.nterms, hist_cells(spec)), dtype=np.uint64)
+        t = C.c_int64()
+        _check(self.lib.tpcf_submit(self.handle, C.byref(job), C.byref(spec), mode, out.ctypes.data, C.byref(t)))
+        return t.value, out
+
+    def poll(self, ticket: int):
+        st = Stats()
+        r = _check(self.lib.tpcf_poll(self.handle, ticket, C.byref(st)))
+        return (r == 1), st
+
+    def wait(self, ticket: int) -> Stats:
+        st = Stats()
+        _check(self.lib.tpcf_wait(self.handle, ticket, C.byref(st)))
+        return st
+
+
+def four_term_job(data: Catalog, randoms: Catalog) -> Job:
+    """DDD, DDR, DRR, RRR of one data and one random catalog."""
+    return make_job([(data, 0, data.n), (randoms, 0, randoms.n)], FOUR_TERMS)

@@ -1,0 +1,335 @@
+// cli.cpp -- drop-in command-line programs over libtpcf.
+//
+// One source, three programs (selected at build time with TPCF_PROGRAM):
+//   0  3pcf_C_version_triangle_parametrization      (s,q,theta), TRI format
+//        argv of src/3pcf_C_version_triangle_parametrization.cc:269-313
+//   1  3pcf_C_version_triangle_parametrization_JBJ  JBJ variant, TRI format
+//   2  3pcf_C_version                               sorted (r1,r2,r3), cube format
+//        argv of src/3pcf_C_version.cc:181-217
+// Usage is the reference's: three catalog paths plus getopt flags, one
+// histogram file out (default default_out.dat).  Long options add runtime
+// bin specs and the device path:
+//   --param r123|sqt|jbj    parametrization
+//   --axis I,N,LO,HI[,log]  bin spec of axis I (0..2); r123 uses axis 0
+//   --bins N                r123: N bins per side (with --lo/--hi)
+//   --lo X --hi X --log     r123 range and log scale
+//   --mode fast|plain|fast64
+//   --format tri|plain      output layout (default per program)
+//   --recheck-rel X         FP64 recheck window (default 1e-5)
+//   --device N, --quiet, --stats
+// Which triplets are counted follows the catalogs given, compared as files
+// (realpath), not as strings: "D D R" counts C(N_D,2)*N_R triplets.
+#include <getopt.h>
+#include <algorithm>
+#include <limits.h>
+#include <stdlib.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tpcf.h"
+
+#ifndef TPCF_PROGRAM
+#define TPCF_PROGRAM 0
+#endif
+
+namespace {
+
+struct Catalog {
+    std::string path, real;
+    double *xyz = nullptr;
+    int64_t n = 0;
+};
+
+int die(int code, const char *msg) {
+    std::fprintf(stderr, "error: %s%s%s\n", msg, tpcf_last_error()[0] ? ": " : "", tpcf_last_error());
+    return code;
+}
+
+bool parse_axis(const char *arg, int &idx, tpcf_axis &ax) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s", arg);
+    std::vector<char *> tok;
+    for (char *p = std::strtok(buf, ","); p; p = std::strtok(nullptr, ",")) tok.push_back(p);
+    if (tok.size() < 4 || tok.size() > 5) return false;
+    idx = std::atoi(tok[0]);
+    ax.n = std::atoi(tok[1]);
+    ax.lo = std::atof(tok[2]);
+    ax.hi = std::atof(tok[3]);
+    ax.scale = (tok.size() == 5 && std::strcmp(tok[4], "log") == 0) ? TPCF_LOG : TPCF_LIN;
+    return idx >= 0 && idx < 3;
+}
+
+// Reference voxel chunk: src/3pcf_C_version_triangle_parametrization.cc:52-80.
+bool vox_chunk(int div, int idx, int64_t n, int64_t &lo, int64_t &hi) {
+    const int64_t per = n / div;
+    if (per == 0) return false;
+    lo = per * idx;
+    hi = per * (idx + 1);
+    if (idx == div - 1) hi += n % div;
+    return true;
+}
+
+// TRI-format writer (src/3pcf_C_version_triangle_parametrization.cc:537-569).
+bool write_tri(const char *path, const int64_t ng[3], const int nb[3], const double lo[3], const double hi[3],
+               const double w[3], const uint64_t *h) {
+    FILE *f = std::fopen(path, "w");
+    if (!f) return false;
+    for (int c = 0; c < 3; ++c) std::fprintf(f, "%lld\n", (long long)ng[c]);
+    std::fprintf(f, "%d %d %d\n", nb[0], nb[1], nb[2]);
+    for (int a = 0; a < 3; ++a) std::fprintf(f, "%-6.3f %-6.3f %-6.3f\n", lo[a], hi[a], w[a]);
+    for (int i = 0; i < nb[0]; ++i)
+        for (int j = 0; j < nb[1]; ++j) {
+            for (int k = 0; k < nb[2]; ++k)
+                std::fprintf(f, "%llu ", (unsigned long long)h[((int64_t)i * nb[1] + j) * nb[2] + k]);
+            std::fputc('\n', f);
+        }
+    return std::fclose(f) == 0;
+}
+
+// Cube-format writer (src/3pcf_C_version.cc:440-460): sorted cell
+// (r1,r2,r3) lives at n^2*r1 + n*r2 + r3.
+bool write_plain(const char *path, int n, const uint64_t *h) {
+    FILE *f = std::fopen(path, "w");
+    if (!f) return false;
+    std::fprintf(f, "%d %d %d\n", n, n, n);
+    for (int i = 0; i < n; ++i) {
+        std::fprintf(f, "%d\n", i);
+        for (int j = 0; j < n; ++j) {
+            for (int k = 0; k < n; ++k)
+                std::fprintf(f, "%llu ", (unsigned long long)h[(int64_t)n * n * k + (int64_t)n * j + i]);
+            std::fputc('\n', f);
+        }
+    }
+    return std::fclose(f) == 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    const int program = TPCF_PROGRAM;
+    const bool triprog = program != 2;
+    const char *outfile = "default_out.dat";
+    int vdiv = -999, vtemp = -999, device = 0, quiet = 0, want_stats = 0;
+    double scale = 1.0;
+    tpcf_binspec spec{};
+    spec.recheck_rel = 1e-5;
+    if (program == 2) {
+        // src/3pcf_C_version.cc:169-174: 64 bins on [0, sqrt(3*24*24)).
+        spec.param = TPCF_R123;
+        spec.ax[0] = {64, 0.0, std::sqrt(3.0 * 24 * 24), TPCF_LIN};
+    } else {
+        // src/3pcf_C_version_triangle_parametrization.cc:31-42.
+        spec.param = program == 1 ? TPCF_JBJ : TPCF_SQT;
+        spec.ax[0] = {50, 2.0, 12.0, TPCF_LIN};
+        spec.ax[1] = {16, 0.9, 4.1, TPCF_LIN};
+        spec.ax[2] = {32, 0.0, 1.0, TPCF_LIN};
+    }
+    int mode = TPCF_MODE_FAST;
+    int format = triprog ? 0 : 1;
+    bool have_r123_range = false;
+    double r_lo = 0, r_hi = 0;
+    int r_n = 0, r_log = 0;
+
+    enum { O_PARAM = 1000, O_AXIS, O_BINS, O_LO, O_HI, O_LOG, O_MODE, O_FORMAT, O_RECHECK, O_DEVICE, O_QUIET, O_STATS };
+    static option longopts[] = {{"param", required_argument, nullptr, O_PARAM},
+                                {"axis", required_argument, nullptr, O_AXIS},
+                                {"bins", required_argument, nullptr, O_BINS},
+                                {"lo", required_argument, nullptr, O_LO},
+                                {"hi", required_argument, nullptr, O_HI},
+                                {"log", no_argument, nullptr, O_LOG},
+                                {"mode", required_argument, nullptr, O_MODE},
+                                {"format", required_argument, nullptr, O_FORMAT},
+                                {"recheck-rel", required_argument, nullptr, O_RECHECK},
+                                {"device", required_argument, nullptr, O_DEVICE},
+                                {"quiet", no_argument, nullptr, O_QUIET},
+                                {"stats", no_argument, nullptr, O_STATS},
+                                {nullptr, 0, nullptr, 0}};
+    const char *shortopts = triprog ? "ao:l:x:X:" : "ao:L:l:w:sm";
+    int c;
+    while ((c = getopt_long(argc, argv, shortopts, longopts, nullptr)) != -1) {
+        switch (c) {
+        case 'o': outfile = optarg; break;
+        case 'X': vdiv = std::atoi(optarg); break;
+        case 'x': vtemp = std::atoi(optarg); break;
+        case 's': scale = 206264.0; break;     // src/3pcf_C_version.cc:195-201
+        case 'm': scale = 3437.74677; break;   // src/3pcf_C_version.cc:202-208
+        case 'a': case 'l': case 'L': case 'w': break;  // parsed by the reference, no effect on counts
+        case O_PARAM:
+            if (!std::strcmp(optarg, "r123")) spec.param = TPCF_R123;
+            else if (!std::strcmp(optarg, "sqt")) spec.param = TPCF_SQT;
+            else if (!std::strcmp(optarg, "jbj")) spec.param = TPCF_JBJ;
+            else { std::fprintf(stderr, "error: --param r123|sqt|jbj\n"); return 2; }
+            if (spec.param == TPCF_R123 && program != 2) spec.ax[0] = {16, 0.0, 40.0, TPCF_LIN};
+            break;
+        case O_AXIS: {
+            int idx;
+            tpcf_axis ax;
+            if (!parse_axis(optarg, idx, ax)) { std::fprintf(stderr, "error: --axis I,N,LO,HI[,log]\n"); return 2; }
+            spec.ax[idx] = ax;
+            break;
+        }
+        case O_BINS: r_n = std::atoi(optarg); have_r123_range = true; break;
+        case O_LO: r_lo = std::atof(optarg); have_r123_range = true; break;
+        case O_HI: r_hi = std::atof(optarg); have_r123_range = true; break;
+        case O_LOG: r_log = 1; have_r123_range = true; break;
+        case O_MODE:
+            if (!std::strcmp(optarg, "fast")) mode = TPCF_MODE_FAST;
+            else if (!std::strcmp(optarg, "plain")) mode = TPCF_MODE_PLAIN;
+            else if (!std::strcmp(optarg, "fast64")) mode = TPCF_MODE_FAST64;
+            else { std::fprintf(stderr, "error: --mode fast|plain|fast64\n"); return 2; }
+            break;
+        case O_FORMAT: format = std::strcmp(optarg, "plain") == 0 ? 1 : 0; break;
+        case O_RECHECK: spec.recheck_rel = std::atof(optarg); break;
+        case O_DEVICE: device = std::atoi(optarg); break;
+        case O_QUIET: quiet = 1; break;
+        case O_STATS: want_stats = 1; break;
+        case '?': std::printf("unknown arg %c\n", optopt); break;  // the reference continues
+        default: break;
+        }
+    }
+    if (have_r123_range) {
+        if (r_n > 0) spec.ax[0].n = r_n;
+        if (r_hi > r_lo) {
+            spec.ax[0].lo = r_lo;
+            spec.ax[0].hi = r_hi;
+        }
+        spec.ax[0].scale = r_log ? TPCF_LOG : TPCF_LIN;
+    }
+    if (argc - optind < 3) {
+        std::printf("\nMust pass in at least three input files on command line!\n");
+        return 1;
+    }
+    if (format == 1 && spec.param != TPCF_R123) {
+        std::fprintf(stderr, "error: the cube format holds (r1,r2,r3) histograms only\n");
+        return 2;
+    }
+    if (tpcf_binspec_check(&spec) != TPCF_OK) return die(2, "bad bin spec");
+
+    // Catalogs, deduplicated by file identity.
+    Catalog cat[3];
+    for (int i = 0; i < 3; ++i) {
+        cat[i].path = argv[optind + i];
+        char rp[PATH_MAX];
+        if (!realpath(cat[i].path.c_str(), rp)) {
+            std::fprintf(stderr, "error: cannot open %s\n", cat[i].path.c_str());
+            return 1;
+        }
+        cat[i].real = rp;
+        int same = -1;
+        for (int j = 0; j < i; ++j)
+            if (cat[j].real == cat[i].real) same = j;
+        if (same >= 0) {
+            cat[i].xyz = cat[same].xyz;
+            cat[i].n = cat[same].n;
+            continue;
+        }
+        cat[i].n = tpcf_read_catalog(cat[i].path.c_str(), triprog ? 7 : 3, scale, &cat[i].xyz);
+        if (cat[i].n < 0) return die(1, "cannot read catalog");
+        if (!quiet) std::printf("Opening input file %d: %s (%lld points)\n", i, cat[i].path.c_str(), (long long)cat[i].n);
+    }
+
+    // Index ranges (whole catalogs, or the reference's voxel chunks).
+    int64_t lo[3], hi[3];
+    int vidx[3] = {0, 0, 0};
+    if (vdiv > 0) {
+        vidx[0] = vtemp / 1000000;
+        vidx[1] = (vtemp / 1000) % 1000;
+        vidx[2] = vtemp % 1000;
+    }
+    for (int i = 0; i < 3; ++i) {
+        lo[i] = 0;
+        hi[i] = cat[i].n;
+        if (vdiv > 0 && vtemp >= 0 && !vox_chunk(vdiv, vidx[i], cat[i].n, lo[i], hi[i])) {
+            std::printf("Voxel chunks are bigger than the number of galaxies!");
+            return 255;
+        }
+    }
+    // Roles of the same file must come in chunk order; otherwise the voxel
+    // selects no triplet.
+    bool empty = false;
+    for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b)
+            if (cat[a].real == cat[b].real && lo[a] > lo[b]) empty = true;
+
+    tpcf_ctx *ctx = nullptr;
+    if (tpcf_ctx_create(device, &ctx) != TPCF_OK) return die(3, "no usable GPU");
+    tpcf_catalog *dev[3] = {nullptr, nullptr, nullptr};
+    tpcf_job job{};
+    int label[3] = {-1, -1, -1};
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < i; ++j)
+            if (cat[j].real == cat[i].real && lo[j] == lo[i]) {
+                label[i] = label[j];
+                break;
+            }
+        if (label[i] >= 0) continue;
+        int same_cat = -1;
+        for (int j = 0; j < i; ++j)
+            if (cat[j].real == cat[i].real) same_cat = j;
+        if (same_cat >= 0) dev[i] = dev[same_cat];
+        else if (tpcf_catalog_create(ctx, cat[i].xyz, cat[i].n, &dev[i]) != TPCF_OK)
+            return die(3, "catalog upload failed");
+        label[i] = job.nseg;
+        job.seg[job.nseg] = dev[i];
+        job.seg_lo[job.nseg] = lo[i];
+        job.seg_hi[job.nseg] = hi[i];
+        job.nseg++;
+    }
+    int t3[3] = {label[0], label[1], label[2]};
+    std::sort(t3, t3 + 3);
+    job.nterms = 1;
+    std::memcpy(job.term[0], t3, sizeof t3);
+
+    const int64_t cells = tpcf_hist_cells(&spec);
+    std::vector<uint64_t> h(cells, 0);
+    tpcf_stats st{};
+    if (!empty && tpcf_count(ctx, &job, &spec, mode, h.data(), &st) != TPCF_OK) return die(4, "counting failed");
+
+    uint64_t total = 0;
+    for (auto v : h) total += v;
+    bool ok;
+    if (format == 1) {
+        ok = write_plain(outfile, spec.ax[0].n, h.data());
+    } else {
+        const int64_t ng[3] = {cat[0].n, cat[1].n, cat[2].n};
+        int nb[3];
+        double alo[3], ahi[3], w[3];
+        for (int a = 0; a < 3; ++a) {
+            const tpcf_axis &ax = spec.param == TPCF_R123 ? spec.ax[0] : spec.ax[a];
+            nb[a] = ax.n;
+            alo[a] = ax.lo;
+            ahi[a] = ax.hi;
+            w[a] = ax.scale == TPCF_LOG ? std::log(ax.hi / ax.lo) / ax.n : (ax.hi - ax.lo) / ax.n;
+        }
+        ok = write_tri(outfile, ng, nb, alo, ahi, w, h.data());
+    }
+    if (!ok) return die(1, "cannot write the histogram file");
+    if (!quiet) {
+        std::printf("Output filename is %s\n", outfile);
+        std::printf("Total: %llu\n", (unsigned long long)total);
+        std::printf("numcalc: %llu\n", (unsigned long long)st.triplets);
+    }
+    if (want_stats)
+        std::printf("{\"triplets\": %llu, \"pairs\": %llu, \"pairs_rechecked\": %llu, \"device_ms\": %.3f, "
+                    "\"wall_ms\": %.3f}\n",
+                    (unsigned long long)st.triplets, (unsigned long long)st.pairs,
+                    (unsigned long long)st.pairs_rechecked, st.device_ms, st.wall_ms);
+    for (int i = 0; i < 3; ++i) {
+        bool first = true;
+        for (int j = 0; j < i; ++j)
+            if (dev[j] == dev[i]) first = false;
+        if (first && dev[i]) tpcf_catalog_destroy(dev[i]);
+    }
+    for (int i = 0; i < 3; ++i) {
+        bool first = true;
+        for (int j = 0; j < i; ++j)
+            if (cat[j].xyz == cat[i].xyz) first = false;
+        if (first) tpcf_free(cat[i].xyz);
+    }
+    tpcf_ctx_destroy(ctx);
+    return 0;
+}

@@ -1,0 +1,449 @@
+// kernels_r123.hip -- gfx950 kernels of the sorted-side (r1,r2,r3) path.
+//
+//   pair_tiles_kernel     pair stage: every pair of a job's union layout gets
+//                         a u8 bin (or the sentinel) in 64x64 tiles.  FP32
+//                         squared distances against squared edges, with every
+//                         pair whose FP32 value lies within a rigorous error
+//                         bound (or the requested relative window) of an edge
+//                         recomputed in FP64; or FP64 throughout.
+//   r123_triplet_kernel   persistent triplet stage over a work queue of
+//                         (term, A, B, C-range) block triples: per triplet one
+//                         v_add3 and one LDS atomic into a privatized
+//                         unsorted-cube histogram, folded into sorted uint64
+//                         cells on flush.
+//   r123_plain_kernel     the FP64 baseline: all three distances of every
+//                         triplet recomputed in FP64 and binned, like the
+//                         reference's distance() (src/3pcf_C_version.cc:54-142).
+//
+// Compile with -ffp-contract=off: the FP64 arithmetic must be the unfused
+// IEEE sequence ((dx*dx + dy*dy) + dz*dz) that the CPU definition uses.
+#include "tpcf_internal.h"
+
+namespace tpcf {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+// Largest m in [0, n] with T[m] <= s, or -1 when s < T[0]; m == n means the
+// value is past the last edge.
+template <typename F>
+__device__ inline int search_le(F s, const F *T, int n) {
+    if (!(s >= T[0])) return -1;
+    int lo = 0, hi = n + 1;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (T[mid] <= s) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Stored pair value: bin << shift, or the sentinel outside [0, n).
+__device__ inline uint32_t bin_of(int m, int n, int shift) {
+    return (m >= 0 && m < n) ? ((uint32_t)m << shift) : kSentinel;
+}
+
+__device__ inline double sq64(const double *u, int64_t npad, int64_t a, int64_t b) {
+    double dx = u[a] - u[b];
+    double dy = u[npad + a] - u[npad + b];
+    double dz = u[2 * npad + a] - u[2 * npad + b];
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+// ---------------------------------------------------------------------------
+// pair stage
+// ---------------------------------------------------------------------------
+
+// One 64x64 tile (R, C), R <= C, per 256-thread workgroup; thread t owns row
+// t/4 and 16 consecutive columns, written as one 16-byte store.
+template <bool F32>
+__global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
+    const int64_t R = blockIdx.y, C = blockIdx.x;
+    if (C < R) return;
+    __shared__ float4 sR[kBlock], sC[kBlock];
+    __shared__ double sT64[kMaxBins + 1];
+    __shared__ float sT32[kMaxBins + 1];
+    const int t = threadIdx.x;
+    const int n = J.n;
+    for (int m = t; m <= n; m += 256) {
+        sT64[m] = J.T64[m];
+        sT32[m] = J.T32[m];
+    }
+    if (F32) {
+        if (t < kBlock) sR[t] = J.u32[R * kBlock + t];
+        else if (t < 2 * kBlock) sC[t - kBlock] = J.u32[C * kBlock + t - kBlock];
+    }
+    __syncthreads();
+    const int cntR = J.blk_count[R], cntC = J.blk_count[C];
+    const int i = t >> 2;
+    const int jb = (t & 3) * 16;
+    const int64_t gi = R * kBlock + i;
+    uint32_t packed[4] = {0u, 0u, 0u, 0u};
+    unsigned long long nrecheck = 0;
+    const float u = 5.9604644775390625e-8f;  // 2^-24
+    const float base = 2.002f * u * J.rmax;
+    float4 pi = F32 ? sR[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 4
+    for (int q = 0; q < 16; ++q) {
+        const int j = jb + q;
+        uint32_t b = kSentinel;
+        const bool valid = i < cntR && j < cntC && (R != C || j > i);
+        if (valid) {
+            const int64_t gj = C * kBlock + j;
+            if (F32) {
+                const float4 pj = sC[j];
+                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                const float s = (dx * dx + dy * dy) + dz * dz;
+                const int m = search_le(s, sT32, n);
+                // |s - s64| <= E: coordinate rounding (2u*rmax per point),
+                // the FP32 difference, and three FP32 roundings of the sum.
+                const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+                const float ex = base + 1.002f * u * ax;
+                const float ey = base + 1.002f * u * ay;
+                const float ez = base + 1.002f * u * az;
+                const float E = 1.01f * (ex * (2.f * ax + ex) + ey * (2.f * ay + ey) + ez * (2.f * az + ez)) +
+                                3.01f * u * s + 1e-30f;
+                const double sd = (double)s;
+                const double Ed = (double)E + 1e-15 * sd;
+                bool nearb = false;
+                if (m >= 0) {
+                    const double T = sT64[m];
+                    nearb |= fabs(sd - T) <= fmax(Ed, 2.0 * (double)J.rel * T);
+                }
+                if (m < n) {
+                    const double T = sT64[m + 1];
+                    nearb |= fabs(sd - T) <= fmax(Ed, 2.0 * (double)J.rel * T);
+                }
+                if (nearb) {
+                    ++nrecheck;
+                    b = bin_of(search_le(sq64(J.u64, J.npad, gi, gj), sT64, n), n, J.shift);
+                } else {
+                    b = bin_of(m, n, J.shift);
+                }
+            } else {
+                b = bin_of(search_le(sq64(J.u64, J.npad, gi, gj), sT64, n), n, J.shift);
+            }
+        }
+        packed[q >> 2] |= (b & 0xFFu) << (8 * (q & 3));
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(J.tiles + tile_index(R, C, J.nblk) * kTileBytes + i * kBlock + jb);
+    *dst = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+    if (F32 && nrecheck) atomicAdd(J.rechecked, nrecheck);
+}
+
+// ---------------------------------------------------------------------------
+// triplet stage (pair-bin form)
+// ---------------------------------------------------------------------------
+
+// Pair bins are stored pre-scaled as 4*bin (a byte offset into a row of
+// 32-bit counters); 0xFF is the sentinel, and every valid value is < 128.
+//
+// LDS histogram of one term with n bins: unsorted cube word
+// z*n^2 + y*n + x for x = bin(i,j), y = bin(i,k), z = bin(j,k).  A lane whose
+// z is out of range adds into a dump word n^3 + lane + y*n + x instead of
+// branching, so the dump needs n^2 + 64 extra words.
+__host__ __device__ inline int r123_lds_words(int n) { return n * n * n + n * n + 64; }
+
+// Fold the privatized unsorted cube into the term's sorted uint64 histogram
+// and clear it (dump words included).
+__device__ inline void r123_flush(uint32_t *hist, int n, unsigned long long *g) {
+    __syncthreads();
+    const int n2 = n * n, n3 = n2 * n;
+    const int words = r123_lds_words(n);
+    for (int c = threadIdx.x; c < words; c += blockDim.x) {
+        const uint32_t v = hist[c];
+        if (c < n3 && v) {
+            int z = c / n2, y = (c / n) % n, x = c % n, t;
+            if (x > y) { t = x; x = y; y = t; }
+            if (y > z) { t = y; y = z; z = t; }
+            if (x > y) { t = x; x = y; y = t; }
+            atomicAdd(g + (x * n + y) * n + z, (unsigned long long)v);
+        }
+        hist[c] = 0u;
+    }
+    __syncthreads();
+}
+
+typedef const uint32_t __attribute__((address_space(4))) *scalar_u32_ptr;
+
+__device__ inline void lds_add1(uint32_t *hist, uint32_t byte_addr) {
+    __hip_atomic_fetch_add(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + byte_addr), 1u,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Persistent: workgroups pull items (term, A, B, C-range) from a global
+// counter until the queue is drained; the waves of a workgroup split the C
+// blocks of an item (wave w takes C = c0+w, c0+w+WAVES, ...) and share one LDS
+// histogram.  Lane k owns point k of block C: z = bin(j,k) is preloaded for
+// the 64 columns j of block B, y = bin(i,k) is per lane and fixed for a row
+// i of block A, and x = bin(i,j) is uniform, read with scalar loads.  A row
+// whose 64 x are all valid takes the branch-free path (one s_bfe, one
+// v_add3, one ds_add per 64 triplets); a row with none valid is skipped;
+// mixed rows test x per column.
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletItem *__restrict__ items,
+                                                                  int nitems, int *__restrict__ counter,
+                                                                  const TripletTerm *__restrict__ terms,
+                                                                  int lds_words) {
+    extern __shared__ uint32_t hist[];
+    __shared__ int s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // Clear the whole allocation once; every flush clears what its term used,
+    // so terms of any bin count start from zero.
+    for (int c = threadIdx.x; c < lds_words; c += blockDim.x) hist[c] = 0u;
+    int cur = -1, cur_n = 0;
+    unsigned long long since_flush = 0;
+    unsigned long long *cur_hist = nullptr;
+    // One C block adds at most 64^3 counts to a cell; flush the 32-bit LDS
+    // counters well before they could wrap.
+    const unsigned long long flush_limit = 0xC0000000ull;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= nitems) break;
+        const TripletItem I = items[it];
+        const int term = __builtin_amdgcn_readfirstlane(I.term);
+        const TripletTerm T = terms[term];
+        const int n = __builtin_amdgcn_readfirstlane(T.n);
+        const int A = __builtin_amdgcn_readfirstlane(I.A);
+        const int B = __builtin_amdgcn_readfirstlane(I.B);
+        const int c0 = __builtin_amdgcn_readfirstlane(I.c0);
+        const int c1 = __builtin_amdgcn_readfirstlane(I.c1);
+        const unsigned long long add = (unsigned long long)(c1 - c0) * (kBlock * kBlock * kBlock);
+        if (term != cur || since_flush + add > flush_limit) {
+            if (cur >= 0) r123_flush(hist, cur_n, cur_hist);
+            cur = term;
+            cur_n = n;
+            cur_hist = T.hist;
+            since_flush = 0;
+        }
+        since_flush += add;
+        const int64_t nblk = T.nblk;
+        const uint32_t n1 = (uint32_t)n, n2 = n1 * n1;
+        const uint32_t dump = (n2 * n1 + (uint32_t)lane) * 4u;
+        const uint8_t *tAB = T.tiles + tile_index(A, B, nblk) * kTileBytes;
+        for (int C = c0 + wave; C < c1; C += WAVES) {
+            const uint8_t *tAC = T.tiles + tile_index(A, C, nblk) * kTileBytes;
+            const uint8_t *tBC = T.tiles + tile_index(B, C, nblk) * kTileBytes;
+            // z of the 64 columns as byte offsets of the z slab, or the
+            // lane's dump slot for the sentinel.
+            uint32_t Zs[kBlock];
+#pragma unroll
+            for (int j = 0; j < kBlock; ++j) Zs[j] = tBC[j * kBlock + lane];
+#pragma unroll
+            for (int j = 0; j < kBlock; ++j) Zs[j] = Zs[j] < 128u ? __umul24(Zs[j], n2) : dump;
+            for (int i = 0; i < kBlock; ++i) {
+                const uint32_t y4 = tAC[i * kBlock + lane];
+                const uint32_t xv = tAB[i * kBlock + lane];
+                const unsigned long long rowmask = __ballot(xv < 128u);
+                if (rowmask == 0ull) continue;
+                if (y4 >= 128u) continue;
+                const uint32_t Ys = __umul24(y4, n1);
+                scalar_u32_ptr xrow = (scalar_u32_ptr)(tAB + i * kBlock);
+                uint32_t xw[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) xw[q] = xrow[q];
+                if (rowmask == ~0ull) {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const uint32_t x4 = (xw[q] >> (8 * r)) & 0xFFu;
+                            lds_add1(hist, Zs[q * 4 + r] + Ys + x4);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const uint32_t x4 = (xw[q] >> (8 * r)) & 0xFFu;
+                            if (x4 < 128u) lds_add1(hist, Zs[q * 4 + r] + Ys + x4);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (cur >= 0) r123_flush(hist, cur_n, cur_hist);
+}
+
+// ---------------------------------------------------------------------------
+// plain FP64 baseline
+// ---------------------------------------------------------------------------
+
+// Static round-robin over items (one job per launch).  Lane k owns point k of
+// block C; every triplet recomputes its three FP64 squared distances and
+// bins each against the squared thresholds, then sorts the three bins.
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void r123_plain_kernel(PlainArgs P) {
+    extern __shared__ uint32_t hist[];
+    __shared__ double sT[kMaxBins + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = P.n, n3 = n * n * n;
+    const bool lds = n <= kMaxLdsR123Bins;
+    for (int m = threadIdx.x; m <= n; m += blockDim.x) sT[m] = P.T64[m];
+    if (lds)
+        for (int c = threadIdx.x; c < n3; c += blockDim.x) hist[c] = 0u;
+    __syncthreads();
+    const int64_t np = P.npad;
+    for (int it = blockIdx.x; it < P.nitems; it += gridDim.x) {
+        const TripletItem I = P.items[it];
+        const int A = I.A, B = I.B;
+        const int cntA = P.blk_count[A], cntB = P.blk_count[B];
+        for (int C = I.c0; C < I.c1; ++C) {
+            const int cntC = P.blk_count[C];
+            const int64_t gk = (int64_t)C * kBlock + lane;
+            const bool kval = lane < cntC;
+            for (int i = wave; i < cntA; i += WAVES) {
+                const int64_t gi = (int64_t)A * kBlock + i;
+                if (A == C && lane <= i) continue;
+                for (int j = (A == B ? i + 1 : 0); j < cntB; ++j) {
+                    if (!kval || (B == C && lane <= j)) continue;
+                    const int64_t gj = (int64_t)B * kBlock + j;
+                    const double sij = sq64(P.u64, np, gi, gj);
+                    const double sik = sq64(P.u64, np, gi, gk);
+                    const double sjk = sq64(P.u64, np, gj, gk);
+                    int x = search_le(sij, sT, n), y = search_le(sik, sT, n), z = search_le(sjk, sT, n);
+                    if (x < 0 || x >= n || y < 0 || y >= n || z < 0 || z >= n) continue;
+                    int t;
+                    if (x > y) { t = x; x = y; y = t; }
+                    if (y > z) { t = y; y = z; z = t; }
+                    if (x > y) { t = x; x = y; y = t; }
+                    if (lds)
+                        __hip_atomic_fetch_add(hist + (x * n + y) * n + z, 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    else
+                        atomicAdd(P.hist + (x * n + y) * n + z, 1ull);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (int c = threadIdx.x; c < n3; c += blockDim.x)
+            if (hist[c]) atomicAdd(P.hist + c, (unsigned long long)hist[c]);
+}
+
+// ---------------------------------------------------------------------------
+// wide bin specs (n > 32): pair bins stored unscaled, counts go straight to
+// the sorted uint64 histogram in device memory.  Static round-robin.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void r123_triplet_wide_kernel(const TripletItem *__restrict__ items, int nitems,
+                                                                const TripletTerm *__restrict__ terms) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const TripletItem I = items[it];
+        const TripletTerm T = terms[I.term];
+        const int n = T.n;
+        const uint8_t *tAB = T.tiles + tile_index(I.A, I.B, T.nblk) * kTileBytes;
+        for (int C = I.c0 + wave; C < I.c1; C += 4) {
+            const uint8_t *tAC = T.tiles + tile_index(I.A, C, T.nblk) * kTileBytes;
+            const uint8_t *tBC = T.tiles + tile_index(I.B, C, T.nblk) * kTileBytes;
+            for (int i = 0; i < kBlock; ++i) {
+                const int y = tAC[i * kBlock + lane];
+                if (y == kSentinel) continue;
+                for (int j = 0; j < kBlock; ++j) {
+                    const int x = tAB[i * kBlock + j];
+                    const int z = tBC[j * kBlock + lane];
+                    if (x == kSentinel || z == kSentinel) continue;
+                    int a = x, b = y, c = z, t;
+                    if (a > b) { t = a; a = b; b = t; }
+                    if (b > c) { t = b; b = c; c = t; }
+                    if (a > b) { t = a; a = b; b = t; }
+                    atomicAdd(T.hist + ((int64_t)a * n + b) * n + c, 1ull);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// small utilities
+// ---------------------------------------------------------------------------
+
+__global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// FP32 copy of the union layout relative to the origin o.
+__global__ void to_f32_kernel(const double *u64, int64_t npad, double ox, double oy, double oz, float4 *u32) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npad; i += (int64_t)gridDim.x * blockDim.x)
+        u32[i] = make_float4((float)(u64[i] - ox), (float)(u64[npad + i] - oy), (float)(u64[2 * npad + i] - oz), 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (C++ linkage, used by tpcf_api.cpp)
+// ---------------------------------------------------------------------------
+
+hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st) {
+    dim3 grid((unsigned)J.nblk, (unsigned)J.nblk);
+    if (f32) hipLaunchKernelGGL(pair_tiles_kernel<true>, grid, dim3(256), 0, st, J);
+    else hipLaunchKernelGGL(pair_tiles_kernel<false>, grid, dim3(256), 0, st, J);
+    return hipGetLastError();
+}
+
+int r123_lds_bytes(int n) { return r123_lds_words(n) * 4; }
+
+const void *r123_triplet_kernel_ptr(int waves) {
+    switch (waves) {
+    case 4: return reinterpret_cast<const void *>(&r123_triplet_kernel<4>);
+    case 8: return reinterpret_cast<const void *>(&r123_triplet_kernel<8>);
+    case 16: return reinterpret_cast<const void *>(&r123_triplet_kernel<16>);
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_r123_triplets(const TripletItem *items, int nitems, int *counter, const TripletTerm *terms,
+                                int max_n, int waves, int grid, hipStream_t st) {
+    if (max_n < 1 || max_n > kMaxLdsR123Bins) return hipErrorInvalidValue;
+    const int words = r123_lds_words(max_n);
+    const size_t lds = (size_t)words * 4;
+    switch (waves) {
+    case 4:
+        hipLaunchKernelGGL(r123_triplet_kernel<4>, dim3(grid), dim3(256), lds, st, items, nitems, counter, terms,
+                           words);
+        break;
+    case 8:
+        hipLaunchKernelGGL(r123_triplet_kernel<8>, dim3(grid), dim3(512), lds, st, items, nitems, counter, terms,
+                           words);
+        break;
+    case 16:
+        hipLaunchKernelGGL(r123_triplet_kernel<16>, dim3(grid), dim3(1024), lds, st, items, nitems, counter, terms,
+                           words);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_r123_triplets_wide(const TripletItem *items, int nitems, const TripletTerm *terms, int grid,
+                                     hipStream_t st) {
+    hipLaunchKernelGGL(r123_triplet_wide_kernel, dim3(grid), dim3(256), 0, st, items, nitems, terms);
+    return hipGetLastError();
+}
+
+hipError_t launch_r123_plain(const PlainArgs &P, int grid, hipStream_t st) {
+    const size_t lds = P.n <= kMaxLdsR123Bins ? (size_t)P.n * P.n * P.n * 4 : 0;
+    hipLaunchKernelGGL(r123_plain_kernel<4>, dim3(grid), dim3(256), lds, st, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(64), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f32(const double *u64, int64_t npad, const double o[3], float4 *u32, hipStream_t st) {
+    hipLaunchKernelGGL(to_f32_kernel, dim3(256), dim3(256), 0, st, u64, npad, o[0], o[1], o[2], u32);
+    return hipGetLastError();
+}
+
+}  // namespace tpcf

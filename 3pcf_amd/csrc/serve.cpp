@@ -1,0 +1,185 @@
+// serve.cpp -- asynchronous job interface (tpcf_submit / tpcf_poll /
+// tpcf_wait): a request queue with admission by device-memory footprint and
+// continuous batching.
+//
+// The reference has no counterpart; it runs one OS process per (term, voxel)
+// job (examples/build_voxel_submission_scripts.py:9-41).  Here a context owns
+// a small pool of workers, each with its own HIP stream and workspace.  An
+// idle worker takes every queued job that fits the memory left in the budget
+// (first fit, in arrival order; a job larger than the whole budget fails with
+// TPCF_EBUSY) and runs them as one batch: pair stages back to back, then one
+// persistent triplet launch over all their work items.  Jobs that arrive
+// while a batch is on the GPU are picked up by another worker as soon as
+// memory allows, so their kernels overlap with the running batch on the
+// device.  PLAIN-mode jobs run one per launch (the FP64 baseline).
+#include <chrono>
+#include <cstring>
+
+#include "tpcf_host.h"
+
+namespace tpcf {
+
+struct Ticket {
+    tpcf_job job{};
+    tpcf_binspec spec{};
+    int mode = TPCF_MODE_FAST;
+    uint64_t *hist = nullptr;
+    int64_t footprint = 0;
+    int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+    int rc = TPCF_OK;
+    std::string err;
+    tpcf_stats stats{};
+    std::chrono::steady_clock::time_point t_submit;
+};
+
+static constexpr int kWorkers = 3;
+static constexpr int kMaxBatch = 64;
+
+static void worker_main(tpcf_ctx *ctx) {
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = nullptr;
+    (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    std::unique_ptr<Workspace, WorkspaceDeleter> ws(new_workspace());
+    for (;;) {
+        std::vector<std::shared_ptr<Ticket>> batch;
+        std::vector<int64_t> ids;
+        int64_t reserved = 0;
+        {
+            std::unique_lock<std::mutex> lk(ctx->mu);
+            for (;;) {
+                if (ctx->stopping && ctx->pending.empty()) break;
+                // First fit in arrival order; PLAIN jobs run alone.
+                for (auto it = ctx->pending.begin(); it != ctx->pending.end() && (int)batch.size() < kMaxBatch;) {
+                    auto t = ctx->tickets[*it];
+                    const bool plain = t->mode == TPCF_MODE_PLAIN;
+                    if (plain && !batch.empty()) { ++it; continue; }
+                    if (!batch.empty() && batch[0]->mode == TPCF_MODE_PLAIN) break;
+                    if (!batch.empty() && t->mode != batch[0]->mode) { ++it; continue; }
+                    if ((uint64_t)t->footprint > ctx->budget) {
+                        t->state = 3;
+                        t->rc = TPCF_EBUSY;
+                        t->err = "job workspace exceeds the device memory budget";
+                        it = ctx->pending.erase(it);
+                        ctx->cv.notify_all();
+                        continue;
+                    }
+                    if (ctx->in_use + reserved + (uint64_t)t->footprint <= ctx->budget) {
+                        reserved += t->footprint;
+                        t->state = 1;
+                        batch.push_back(t);
+                        ids.push_back(*it);
+                        it = ctx->pending.erase(it);
+                        if (plain) break;
+                    } else {
+                        ++it;
+                    }
+                }
+                if (!batch.empty()) break;
+                ctx->cv.wait(lk);
+            }
+            if (batch.empty()) break;
+            ctx->in_use += reserved;
+        }
+        std::vector<int> rcs(batch.size(), TPCF_OK);
+        std::vector<std::string> errs(batch.size());
+        if (batch[0]->mode == TPCF_MODE_PLAIN) {
+            rcs[0] = run_single(ctx, *ws, batch[0]->job, batch[0]->spec, batch[0]->mode, batch[0]->hist,
+                                &batch[0]->stats, st);
+            if (rcs[0]) errs[0] = tpcf_last_error();
+        } else {
+            std::vector<tpcf_job> jobs;
+            std::vector<tpcf_binspec> specs;
+            std::vector<uint64_t *> hists;
+            std::vector<tpcf_stats> per(batch.size());
+            for (auto &t : batch) {
+                jobs.push_back(t->job);
+                specs.push_back(t->spec);
+                hists.push_back(t->hist);
+            }
+            const int rc = run_batch(ctx, *ws, jobs.data(), specs.data(), (int)jobs.size(), batch[0]->mode,
+                                     hists.data(), per.data(), nullptr, st);
+            for (size_t k = 0; k < batch.size(); ++k) {
+                rcs[k] = rc;
+                if (rc) errs[k] = tpcf_last_error();
+                batch[k]->stats = per[k];
+            }
+        }
+        {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            const auto now = std::chrono::steady_clock::now();
+            for (size_t k = 0; k < batch.size(); ++k) {
+                batch[k]->rc = rcs[k];
+                batch[k]->err = errs[k];
+                batch[k]->state = rcs[k] ? 3 : 2;
+                batch[k]->stats.wall_ms =
+                    std::chrono::duration<double, std::milli>(now - batch[k]->t_submit).count();
+            }
+            ctx->in_use -= reserved;
+        }
+        ctx->cv.notify_all();
+    }
+    ws.reset();
+    if (st) (void)hipStreamDestroy(st);
+}
+
+}  // namespace tpcf
+
+using namespace tpcf;
+
+extern "C" {
+
+int tpcf_submit(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int mode, uint64_t *hist,
+                int64_t *ticket) {
+    clear_error();
+    if (!ctx || !job || !spec || !hist || !ticket) return set_error(TPCF_EINVAL, "NULL argument");
+    const int64_t fp = plan_bytes(*job, *spec, mode);
+    if (fp < 0) return (int)fp;
+    auto t = std::make_shared<Ticket>();
+    t->job = *job;
+    t->spec = *spec;
+    t->mode = mode;
+    t->hist = hist;
+    t->footprint = fp;
+    t->t_submit = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (ctx->workers.empty())
+            for (int w = 0; w < kWorkers; ++w) ctx->workers.emplace_back(worker_main, ctx);
+        const int64_t id = ctx->next_ticket++;
+        ctx->tickets[id] = t;
+        ctx->pending.push_back(id);
+        *ticket = id;
+    }
+    ctx->cv.notify_all();
+    return TPCF_OK;
+}
+
+static int finish(tpcf_ctx *ctx, int64_t ticket, tpcf_stats *stats, bool block) {
+    std::unique_lock<std::mutex> lk(ctx->mu);
+    auto it = ctx->tickets.find(ticket);
+    if (it == ctx->tickets.end()) return set_error(TPCF_ENOTFOUND, "unknown ticket %lld", (long long)ticket);
+    auto t = it->second;
+    if (block) ctx->cv.wait(lk, [&] { return t->state >= 2; });
+    if (t->state < 2) return 0;
+    if (stats) *stats = t->stats;
+    const int rc = t->rc;
+    const std::string err = t->err;
+    ctx->tickets.erase(it);
+    if (rc) return set_error(rc, "%s", err.c_str());
+    return 1;
+}
+
+int tpcf_poll(tpcf_ctx *ctx, int64_t ticket, tpcf_stats *stats) {
+    clear_error();
+    if (!ctx) return set_error(TPCF_EINVAL, "ctx is NULL");
+    return finish(ctx, ticket, stats, false);
+}
+
+int tpcf_wait(tpcf_ctx *ctx, int64_t ticket, tpcf_stats *stats) {
+    clear_error();
+    if (!ctx) return set_error(TPCF_EINVAL, "ctx is NULL");
+    const int r = finish(ctx, ticket, stats, true);
+    return r == 1 ? TPCF_OK : r;
+}
+
+}  // extern "C"

@@ -1,0 +1,637 @@
+// tpcf_api.cpp -- the C ABI of libtpcf: contexts, device catalogs, job
+// planning and the launch sequences of the FAST (FP32 pair stage with FP64
+// recheck + integer triplet kernel) and PLAIN (FP64 per-triplet) paths.
+//
+// The reference's equivalent is main() of each C program:
+// src/3pcf_C_version_triangle_parametrization.cc:245-571 (read catalogs,
+// pick loop bounds from the file names, run the triple loop, write the
+// histogram) and src/3pcf_C_version.cc:147-466.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "tpcf_host.h"
+
+namespace tpcf {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+void clear_error() { g_err.clear(); }
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return set_error(e_ == hipErrorOutOfMemory ? TPCF_ENOMEM : TPCF_EDEVICE, "%s: %s (%s:%d)", \
+                             #expr, hipGetErrorString(e_), __FILE__, __LINE__);                         \
+    } while (0)
+
+DevBuf::~DevBuf() {
+    if (p) (void)hipFree(p);
+}
+
+int DevBuf::reserve(size_t need) {
+    if (need <= bytes) return TPCF_OK;
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return set_error(TPCF_ENOMEM, "device allocation of %zu bytes failed: %s", need, hipGetErrorString(e));
+    }
+    bytes = need;
+    return TPCF_OK;
+}
+
+static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// ---------------------------------------------------------------------------
+// job planning
+// ---------------------------------------------------------------------------
+
+static int check_job(const tpcf_job &job) {
+    if (job.nseg < 1 || job.nseg > 3) return set_error(TPCF_EINVAL, "a job has 1..3 segments, got %d", job.nseg);
+    if (job.nterms < 1 || job.nterms > 4) return set_error(TPCF_EINVAL, "a job has 1..4 terms, got %d", job.nterms);
+    for (int s = 0; s < job.nseg; ++s) {
+        const tpcf_catalog *c = job.seg[s];
+        if (!c) return set_error(TPCF_EINVAL, "segment %d has no catalog", s);
+        if (job.seg_lo[s] < 0 || job.seg_hi[s] < job.seg_lo[s] || job.seg_hi[s] > c->n)
+            return set_error(TPCF_EINVAL, "segment %d range [%lld, %lld) outside catalog of %lld points", s,
+                             (long long)job.seg_lo[s], (long long)job.seg_hi[s], (long long)c->n);
+        if (c->ctx != job.seg[0]->ctx) return set_error(TPCF_EINVAL, "segments belong to different contexts");
+    }
+    for (int t = 0; t < job.nterms; ++t) {
+        const int *tr = job.term[t];
+        if (tr[0] < 0 || tr[2] >= job.nseg || tr[0] > tr[1] || tr[1] > tr[2])
+            return set_error(TPCF_EINVAL, "term %d (%d,%d,%d) must be sorted segment labels < %d", t, tr[0], tr[1],
+                             tr[2], job.nseg);
+    }
+    return TPCF_OK;
+}
+
+static Layout make_layout(const tpcf_job &job) {
+    Layout L;
+    L.nseg = job.nseg;
+    int64_t off = 0;
+    for (int s = 0; s < job.nseg; ++s) {
+        L.seg_n[s] = job.seg_hi[s] - job.seg_lo[s];
+        L.seg_off[s] = off;
+        L.seg_blk0[s] = off / kBlock;
+        L.seg_nblk[s] = (L.seg_n[s] + kBlock - 1) / kBlock;
+        off += L.seg_nblk[s] * kBlock;
+    }
+    L.npad = off;
+    L.nblk = off / kBlock;
+    return L;
+}
+
+// Exact number of unique triplets of a term.
+static uint64_t term_triplets(const Layout &L, const int *tr) {
+    auto c2 = [](uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; };
+    auto c3 = [](uint64_t n) { return n < 3 ? 0 : n * (n - 1) / 2 * (n - 2) / 3; };
+    const uint64_t a = L.seg_n[tr[0]], b = L.seg_n[tr[1]], c = L.seg_n[tr[2]];
+    if (tr[0] == tr[1] && tr[1] == tr[2]) return c3(a);
+    if (tr[0] == tr[1]) return c2(a) * c;
+    if (tr[1] == tr[2]) return a * c2(b);
+    return a * b * c;
+}
+
+// Work items of one term: blocks A <= B <= C of the term's segments.
+static void term_items(const Layout &L, const int *tr, int term_slot, std::vector<TripletItem> &out) {
+    const int64_t a0 = L.seg_blk0[tr[0]], a1 = a0 + L.seg_nblk[tr[0]];
+    const int64_t b0 = L.seg_blk0[tr[1]], b1 = b0 + L.seg_nblk[tr[1]];
+    const int64_t cb0 = L.seg_blk0[tr[2]], cb1 = cb0 + L.seg_nblk[tr[2]];
+    constexpr int64_t kMaxItemBlocks = 512;
+    for (int64_t A = a0; A < a1; ++A)
+        for (int64_t B = (tr[0] == tr[1] ? A : b0); B < b1; ++B) {
+            const int64_t cs = (tr[1] == tr[2]) ? B : cb0;
+            for (int64_t c = cs; c < cb1; c += kMaxItemBlocks) {
+                TripletItem it{};
+                it.term = term_slot;
+                it.A = (int)A;
+                it.B = (int)B;
+                it.c0 = (int)c;
+                it.c1 = (int)std::min(cb1, c + kMaxItemBlocks);
+                out.push_back(it);
+            }
+        }
+}
+
+// Everything one job needs on the device, laid out in one region.
+struct JobPlan {
+    Layout L;
+    int param = TPCF_R123;
+    int n = 0;
+    int nterms = 0;
+    int64_t cells = 0;
+    double origin[3] = {0, 0, 0};
+    float rmax = 0.f;
+    float rel = 0.f;
+    std::vector<double> T64;
+    std::vector<float> T32;
+    std::vector<uint8_t> blk_count;
+    uint64_t triplets = 0;
+    // byte offsets inside the job's device region
+    size_t o_u64 = 0, o_u32 = 0, o_blk = 0, o_T64 = 0, o_T32 = 0, o_cnt = 0, o_hist = 0, o_tiles = 0;
+    size_t bytes = 0;
+    bool need_tiles = true;
+};
+
+static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, JobPlan &P) {
+    if (int e = check_job(job)) return e;
+    if (int e = binspec_check(spec)) return e;
+    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_PLAIN && mode != TPCF_MODE_FAST64)
+        return set_error(TPCF_EINVAL, "unknown mode %d", mode);
+    if (spec.param != TPCF_R123)
+        return set_error(TPCF_EINVAL, "parametrization %d is not available on the device yet", spec.param);
+    P.L = make_layout(job);
+    P.param = spec.param;
+    P.n = spec.ax[0].n;
+    P.nterms = job.nterms;
+    P.cells = hist_cells(spec);
+    P.rel = (float)(spec.recheck_rel > 0 ? spec.recheck_rel : 0.0);
+    double e[kMaxBins + 1];
+    axis_edges(spec.ax[0], e);
+    P.T64.resize(P.n + 1);
+    sq_thresholds(e, P.n, P.T64.data());
+    P.T32.resize(P.n + 1);
+    for (int m = 0; m <= P.n; ++m) P.T32[m] = (float)P.T64[m];
+    P.blk_count.assign(P.L.nblk, 0);
+    for (int s = 0; s < P.L.nseg; ++s)
+        for (int64_t b = 0; b < P.L.seg_nblk[s]; ++b)
+            P.blk_count[P.L.seg_blk0[s] + b] = (uint8_t)std::min<int64_t>(kBlock, P.L.seg_n[s] - b * kBlock);
+    // Origin and extent of the union for the FP32 copy.
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int s = 0; s < job.nseg; ++s) {
+        const tpcf_catalog *c = job.seg[s];
+        for (int d = 0; d < 3; ++d)
+            for (int64_t i = job.seg_lo[s]; i < job.seg_hi[s]; ++i) {
+                const double v = c->h[d * c->n + i];
+                lo[d] = std::min(lo[d], v);
+                hi[d] = std::max(hi[d], v);
+            }
+    }
+    double r = 0;
+    for (int d = 0; d < 3; ++d) {
+        if (!(hi[d] >= lo[d])) continue;  // empty job
+        if (!std::isfinite(lo[d]) || !std::isfinite(hi[d]))
+            return set_error(TPCF_EINVAL, "catalog coordinates must be finite");
+        P.origin[d] = 0.5 * (lo[d] + hi[d]);
+        r = std::max(r, std::max(hi[d] - P.origin[d], P.origin[d] - lo[d]));
+    }
+    P.rmax = (float)(r * (1.0 + 1e-6)) + 1e-30f;
+    P.triplets = 0;
+    for (int t = 0; t < job.nterms; ++t) P.triplets += term_triplets(P.L, job.term[t]);
+    P.need_tiles = mode != TPCF_MODE_PLAIN;
+    size_t o = 0;
+    P.o_u64 = o; o = align256(o + sizeof(double) * 3 * P.L.npad);
+    P.o_u32 = o; o = align256(o + (P.need_tiles ? sizeof(float4) * P.L.npad : 0));
+    P.o_blk = o; o = align256(o + P.L.nblk + 1);
+    P.o_T64 = o; o = align256(o + sizeof(double) * (P.n + 1));
+    P.o_T32 = o; o = align256(o + sizeof(float) * (P.n + 1));
+    P.o_cnt = o; o = align256(o + 64);
+    P.o_hist = o; o = align256(o + sizeof(uint64_t) * P.cells * P.nterms);
+    P.o_tiles = o; o = align256(o + (P.need_tiles ? (size_t)tile_count(P.L.nblk) * kTileBytes : 0));
+    P.bytes = o;
+    return TPCF_OK;
+}
+
+// Upload the small tables, gather the segments into the union layout and
+// clear histograms/counters.  All on `st`, asynchronously.
+static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStream_t st) {
+    double *u64 = reinterpret_cast<double *>(base + P.o_u64);
+    HIP_TRY(hipMemsetAsync(u64, 0, sizeof(double) * 3 * P.L.npad, st));
+    for (int s = 0; s < job.nseg; ++s) {
+        const tpcf_catalog *c = job.seg[s];
+        const int64_t m = P.L.seg_n[s];
+        if (!m) continue;
+        for (int d = 0; d < 3; ++d)
+            HIP_TRY(hipMemcpyAsync(u64 + d * P.L.npad + P.L.seg_off[s], c->d + d * c->n + job.seg_lo[s],
+                                   sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+    }
+    HIP_TRY(hipMemcpyAsync(base + P.o_blk, P.blk_count.data(), P.blk_count.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(base + P.o_T64, P.T64.data(), sizeof(double) * P.T64.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(base + P.o_T32, P.T32.data(), sizeof(float) * P.T32.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(base + P.o_cnt), 16, 0u, st));
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(base + P.o_hist), 2 * P.cells * P.nterms, 0u, st));
+    if (P.need_tiles && P.L.npad) HIP_TRY(launch_to_f32(u64, P.L.npad, P.origin, reinterpret_cast<float4 *>(base + P.o_u32), st));
+    return TPCF_OK;
+}
+
+static PairJob pair_job(const JobPlan &P, char *base, int shift) {
+    PairJob J{};
+    J.u64 = reinterpret_cast<const double *>(base + P.o_u64);
+    J.u32 = reinterpret_cast<const float4 *>(base + P.o_u32);
+    J.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
+    J.tiles = reinterpret_cast<uint8_t *>(base + P.o_tiles);
+    J.T64 = reinterpret_cast<const double *>(base + P.o_T64);
+    J.T32 = reinterpret_cast<const float *>(base + P.o_T32);
+    J.rechecked = reinterpret_cast<unsigned long long *>(base + P.o_cnt);
+    J.npad = P.L.npad;
+    J.nblk = P.L.nblk;
+    J.n = P.n;
+    J.rmax = P.rmax;
+    J.rel = P.rel;
+    J.shift = shift;
+    return J;
+}
+
+// ---------------------------------------------------------------------------
+// workspaces
+// ---------------------------------------------------------------------------
+
+struct Workspace {
+    DevBuf region;   // job regions
+    DevBuf queue;    // items + terms + counter
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~Workspace() {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    int init() {
+        for (auto &e : ev)
+            if (!e) HIP_TRY(hipEventCreate(&e));
+        return TPCF_OK;
+    }
+};
+
+static int grid_for(tpcf_ctx *ctx, int waves, int max_n) {
+    const void *k = r123_triplet_kernel_ptr(waves);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, waves * 64, r123_lds_bytes(max_n)) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    if (ctx->shape.per_cu > 0) per_cu = std::min(per_cu, ctx->shape.per_cu);
+    return ctx->num_cu * per_cu;
+}
+
+// Launch the triplet stage of several jobs as one persistent kernel.
+static int launch_triplets(tpcf_ctx *ctx, Workspace &ws, const std::vector<const JobPlan *> &plans,
+                           const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases,
+                           hipStream_t st) {
+    std::vector<TripletTerm> terms;
+    std::vector<TripletItem> items;
+    int max_n = 1;
+    for (size_t j = 0; j < plans.size(); ++j) {
+        const JobPlan &P = *plans[j];
+        for (int t = 0; t < P.nterms; ++t) {
+            TripletTerm T{};
+            T.tiles = reinterpret_cast<const uint8_t *>(bases[j] + P.o_tiles);
+            T.nblk = P.L.nblk;
+            T.hist = reinterpret_cast<unsigned long long *>(bases[j] + P.o_hist) + (size_t)t * P.cells;
+            T.n = P.n;
+            const size_t before = items.size();
+            term_items(P.L, jobs[j]->term[t], (int)terms.size(), items);
+            // Longest C ranges first within a term (dynamic LPT order).
+            std::stable_sort(items.begin() + before, items.end(), [](const TripletItem &a, const TripletItem &b) {
+                return (a.c1 - a.c0) > (b.c1 - b.c0);
+            });
+            terms.push_back(T);
+            max_n = std::max(max_n, P.n);
+        }
+    }
+    if (items.empty()) return TPCF_OK;
+    const bool wide = max_n > kMaxLdsR123Bins;
+    const size_t o_items = 0;
+    const size_t o_terms = align256(sizeof(TripletItem) * items.size());
+    const size_t o_ctr = align256(o_terms + sizeof(TripletTerm) * terms.size());
+    if (int e = ws.queue.reserve(o_ctr + 256)) return e;
+    char *q = reinterpret_cast<char *>(ws.queue.p);
+    HIP_TRY(hipMemcpyAsync(q + o_items, items.data(), sizeof(TripletItem) * items.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(q + o_terms, terms.data(), sizeof(TripletTerm) * terms.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(q + o_ctr), 1, 0u, st));
+    if (wide) {
+        // A batch mixing wide and narrow specs runs everything on the wide
+        // kernel, whose pair values are unscaled.
+        HIP_TRY(launch_r123_triplets_wide(reinterpret_cast<const TripletItem *>(q + o_items), (int)items.size(),
+                                          reinterpret_cast<const TripletTerm *>(q + o_terms), ctx->num_cu * 8, st));
+        return TPCF_OK;
+    }
+    const int waves = ctx->shape.waves;
+    HIP_TRY(launch_r123_triplets(reinterpret_cast<const TripletItem *>(q + o_items), (int)items.size(),
+                                 reinterpret_cast<int *>(q + o_ctr), reinterpret_cast<const TripletTerm *>(q + o_terms),
+                                 max_n, waves, grid_for(ctx, waves, max_n), st));
+    return TPCF_OK;
+}
+
+static int launch_plain(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tpcf_job &job, char *base,
+                        hipStream_t st) {
+    for (int t = 0; t < P.nterms; ++t) {
+        std::vector<TripletItem> items;
+        term_items(P.L, job.term[t], 0, items);
+        if (items.empty()) continue;
+        if (int e = ws.queue.reserve(sizeof(TripletItem) * items.size())) return e;
+        HIP_TRY(hipMemcpyAsync(ws.queue.p, items.data(), sizeof(TripletItem) * items.size(), hipMemcpyHostToDevice, st));
+        PlainArgs A{};
+        A.u64 = reinterpret_cast<const double *>(base + P.o_u64);
+        A.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
+        A.T64 = reinterpret_cast<const double *>(base + P.o_T64);
+        A.items = reinterpret_cast<const TripletItem *>(ws.queue.p);
+        A.nitems = (int)items.size();
+        A.n = P.n;
+        A.npad = P.L.npad;
+        A.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist) + (size_t)t * P.cells;
+        const int grid = std::min<int64_t>((int64_t)items.size(), (int64_t)ctx->num_cu * 4);
+        HIP_TRY(launch_r123_plain(A, grid, st));
+        // The item buffer is reused by the next term: finish this one first.
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return TPCF_OK;
+}
+
+// Run one job synchronously on stream st with workspace ws.
+static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf_binspec &spec, int mode,
+                   uint64_t *hist, tpcf_stats *stats, hipStream_t st) {
+    const auto t0 = std::chrono::steady_clock::now();
+    JobPlan P;
+    if (int e = plan_job(job, spec, mode, P)) return e;
+    if (int e = ws.init()) return e;
+    if (int e = ws.region.reserve(P.bytes)) return e;
+    char *base = reinterpret_cast<char *>(ws.region.p);
+    if (int e = stage_job(job, P, base, st)) return e;
+    HIP_TRY(hipEventRecord(ws.ev[0], st));
+    if (mode == TPCF_MODE_PLAIN) {
+        if (P.L.nblk) {
+            if (int e = launch_plain(ctx, ws, P, job, base, st)) return e;
+        }
+    } else if (P.L.nblk) {
+        HIP_TRY(launch_pair_tiles(pair_job(P, base, P.n <= kMaxLdsR123Bins ? 2 : 0), mode == TPCF_MODE_FAST, st));
+        HIP_TRY(hipEventRecord(ws.ev[1], st));
+        std::vector<const JobPlan *> plans{&P};
+        std::vector<const tpcf_job *> jobs{&job};
+        std::vector<char *> bases{base};
+        if (int e = launch_triplets(ctx, ws, plans, jobs, bases, st)) return e;
+    }
+    HIP_TRY(hipEventRecord(ws.ev[2], st));
+    HIP_TRY(hipMemcpyAsync(hist, base + P.o_hist, sizeof(uint64_t) * P.cells * P.nterms, hipMemcpyDeviceToHost, st));
+    uint64_t rechecked = 0;
+    HIP_TRY(hipMemcpyAsync(&rechecked, base + P.o_cnt, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (stats) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ws.ev[0], ws.ev[2]));
+        int64_t nu = 0;
+        for (int s = 0; s < P.L.nseg; ++s) nu += P.L.seg_n[s];
+        stats->triplets = P.triplets;
+        stats->pairs = mode == TPCF_MODE_PLAIN ? 0 : (uint64_t)(nu * (nu - 1) / 2);
+        stats->pairs_rechecked = mode == TPCF_MODE_FAST ? rechecked : 0;
+        stats->device_ms = ms;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return TPCF_OK;
+}
+
+// Run several jobs as one batch on stream st: per-job staging and pair
+// stage, then one persistent triplet launch over every job's work items.
+// per[j] receives job j's counters; total the batch figures (device_ms is
+// the batch's kernel time).
+int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode,
+              uint64_t *const *hist, tpcf_stats *per, tpcf_stats *total, hipStream_t st) {
+    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_FAST64)
+        return set_error(TPCF_EINVAL, "batched counting runs the FAST paths only");
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<JobPlan> plans(njobs);
+    size_t bytes = 0;
+    std::vector<size_t> off(njobs);
+    for (int j = 0; j < njobs; ++j) {
+        if (int e = plan_job(jobs[j], specs[j], mode, plans[j])) return e;
+        off[j] = bytes;
+        bytes = align256(bytes + plans[j].bytes);
+    }
+    if (int e = ws.init()) return e;
+    if (int e = ws.region.reserve(std::max<size_t>(bytes, 256))) return e;
+    char *base = reinterpret_cast<char *>(ws.region.p);
+    for (int j = 0; j < njobs; ++j)
+        if (int e = stage_job(jobs[j], plans[j], base + off[j], st)) return e;
+    HIP_TRY(hipEventRecord(ws.ev[0], st));
+    int max_n = 1;
+    for (int j = 0; j < njobs; ++j) max_n = std::max(max_n, plans[j].n);
+    const int shift = max_n <= kMaxLdsR123Bins ? 2 : 0;
+    std::vector<const JobPlan *> pp;
+    std::vector<const tpcf_job *> jj;
+    std::vector<char *> bb;
+    for (int j = 0; j < njobs; ++j) {
+        if (!plans[j].L.nblk) continue;
+        HIP_TRY(launch_pair_tiles(pair_job(plans[j], base + off[j], shift), mode == TPCF_MODE_FAST, st));
+        pp.push_back(&plans[j]);
+        jj.push_back(&jobs[j]);
+        bb.push_back(base + off[j]);
+    }
+    HIP_TRY(hipEventRecord(ws.ev[1], st));
+    if (int e = launch_triplets(ctx, ws, pp, jj, bb, st)) return e;
+    HIP_TRY(hipEventRecord(ws.ev[2], st));
+    std::vector<uint64_t> rech(njobs, 0);
+    for (int j = 0; j < njobs; ++j) {
+        HIP_TRY(hipMemcpyAsync(hist[j], base + off[j] + plans[j].o_hist,
+                               sizeof(uint64_t) * plans[j].cells * plans[j].nterms, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(&rech[j], base + off[j] + plans[j].o_cnt, 8, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ws.ev[0], ws.ev[2]));
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    tpcf_stats tot{};
+    for (int j = 0; j < njobs; ++j) {
+        tpcf_stats s{};
+        s.triplets = plans[j].triplets;
+        int64_t nu = 0;
+        for (int q = 0; q < plans[j].L.nseg; ++q) nu += plans[j].L.seg_n[q];
+        s.pairs = (uint64_t)(nu * (nu - 1) / 2);
+        s.pairs_rechecked = mode == TPCF_MODE_FAST ? rech[j] : 0;
+        s.device_ms = ms;
+        s.wall_ms = wall;
+        if (per) per[j] = s;
+        tot.triplets += s.triplets;
+        tot.pairs += s.pairs;
+        tot.pairs_rechecked += s.pairs_rechecked;
+    }
+    tot.device_ms = ms;
+    tot.wall_ms = wall;
+    if (total) *total = tot;
+    return TPCF_OK;
+}
+
+int run_single(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf_binspec &spec, int mode, uint64_t *hist,
+               tpcf_stats *stats, hipStream_t st) {
+    return run_job(ctx, ws, job, spec, mode, hist, stats, st);
+}
+
+int64_t plan_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode) {
+    JobPlan P;
+    if (int e = plan_job(job, spec, mode, P)) return e;
+    return (int64_t)P.bytes;
+}
+
+Workspace *new_workspace() { return new Workspace(); }
+void delete_workspace(Workspace *w) { delete w; }
+
+}  // namespace tpcf
+
+using namespace tpcf;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char *tpcf_version(void) { return "libtpcf 0.1 (gfx950)"; }
+
+const char *tpcf_last_error(void) { return g_err.c_str(); }
+
+int tpcf_ctx_create(int device, tpcf_ctx **out) {
+    clear_error();
+    if (!out) return set_error(TPCF_EINVAL, "out is NULL");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return set_error(TPCF_EINVAL, "device %d of %d", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(TPCF_EDEVICE, "libtpcf is built for gfx950, device %d is %s", device, prop.gcnArchName);
+    auto *c = new tpcf_ctx();
+    c->device = device;
+    c->num_cu = prop.multiProcessorCount;
+    c->lds_per_block = prop.sharedMemPerBlock;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return set_error(TPCF_EDEVICE, "stream creation failed");
+    }
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) c->budget = (uint64_t)(0.9 * (double)fr);
+    c->ws.reset(new Workspace());
+    // r123 triplet kernels raise their dynamic LDS limit once.
+    for (int w : {4, 8, 16}) {
+        const void *k = r123_triplet_kernel_ptr(w);
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+    }
+    *out = c;
+    return TPCF_OK;
+}
+
+int tpcf_ctx_destroy(tpcf_ctx *ctx) {
+    if (!ctx) return TPCF_OK;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->stopping = true;
+    }
+    ctx->cv.notify_all();
+    for (auto &t : ctx->workers) t.join();
+    (void)hipSetDevice(ctx->device);
+    ctx->ws.reset();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TPCF_OK;
+}
+
+int tpcf_ctx_set_memory_budget(tpcf_ctx *ctx, uint64_t bytes) {
+    if (!ctx) return set_error(TPCF_EINVAL, "ctx is NULL");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (bytes) ctx->budget = bytes;
+    ctx->cv.notify_all();
+    return TPCF_OK;
+}
+
+int tpcf_catalog_create(tpcf_ctx *ctx, const double *xyz, int64_t n, tpcf_catalog **out) {
+    clear_error();
+    if (!ctx || !out || n < 0 || (n > 0 && !xyz)) return set_error(TPCF_EINVAL, "bad catalog arguments");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto *c = new tpcf_catalog();
+    c->ctx = ctx;
+    c->n = n;
+    c->h.assign(xyz, xyz + 3 * n);
+    if (n > 0) {
+        if (hipMalloc(&c->d, sizeof(double) * 3 * n) != hipSuccess) {
+            delete c;
+            return set_error(TPCF_ENOMEM, "device allocation for %lld points failed", (long long)n);
+        }
+        if (hipMemcpy(c->d, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(c->d);
+            delete c;
+            return set_error(TPCF_EDEVICE, "catalog upload failed");
+        }
+    }
+    *out = c;
+    return TPCF_OK;
+}
+
+int tpcf_catalog_destroy(tpcf_catalog *cat) {
+    if (!cat) return TPCF_OK;
+    if (cat->d) (void)hipFree(cat->d);
+    delete cat;
+    return TPCF_OK;
+}
+
+int64_t tpcf_catalog_size(const tpcf_catalog *cat) { return cat ? cat->n : -1; }
+
+int64_t tpcf_hist_cells(const tpcf_binspec *spec) {
+    if (!spec || binspec_check(*spec)) return -1;
+    return hist_cells(*spec);
+}
+
+int tpcf_axis_edges(const tpcf_axis *ax, double *edges) {
+    clear_error();
+    if (!ax || !edges) return set_error(TPCF_EINVAL, "NULL argument");
+    return axis_edges(*ax, edges);
+}
+
+int tpcf_binspec_check(const tpcf_binspec *spec) {
+    clear_error();
+    if (!spec) return set_error(TPCF_EINVAL, "spec is NULL");
+    return binspec_check(*spec);
+}
+
+int64_t tpcf_job_footprint(const tpcf_job *job, const tpcf_binspec *spec, int mode) {
+    if (!job || !spec) return set_error(TPCF_EINVAL, "NULL argument");
+    JobPlan P;
+    if (int e = plan_job(*job, *spec, mode, P)) return e;
+    return (int64_t)P.bytes;
+}
+
+int tpcf_count(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int mode, uint64_t *hist,
+               tpcf_stats *stats) {
+    clear_error();
+    if (!ctx || !job || !spec || !hist) return set_error(TPCF_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> g(ctx->call_mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    return run_job(ctx, *ctx->ws, *job, *spec, mode, hist, stats, ctx->stream);
+}
+
+int tpcf_count_batch(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode,
+                     uint64_t *const *hist, tpcf_stats *stats) {
+    clear_error();
+    if (!ctx || !jobs || !specs || !hist || njobs < 0) return set_error(TPCF_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> g(ctx->call_mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<tpcf_stats> per(njobs);
+    tpcf_stats total{};
+    if (int e = run_batch(ctx, *ctx->ws, jobs, specs, njobs, mode, hist, per.data(), &total, ctx->stream)) return e;
+    if (stats) *stats = total;
+    return TPCF_OK;
+}
+
+void tpcf_free(void *p) { free(p); }
+
+}  // extern "C"

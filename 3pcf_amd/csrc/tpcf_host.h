@@ -1,0 +1,97 @@
+// tpcf_host.h -- host-side internals of libtpcf (not part of the C ABI).
+#pragma once
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/tpcf.h"
+#include "tpcf_internal.h"
+
+namespace tpcf {
+
+constexpr int kMaxSqtCells = 36 * 1024;  // LDS histogram limit of the (s,q,theta) kernels
+
+// Record a message for tpcf_last_error() and return code.
+int set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+int axis_edges(const tpcf_axis &ax, double *e);
+void sq_thresholds(const double *e, int n, double *T);
+int binspec_check(const tpcf_binspec &s);
+int64_t hist_cells(const tpcf_binspec &s);
+
+// Device allocation that remembers its size.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf();
+    int reserve(size_t need);  // grow-only
+};
+
+// Union layout of a job: each segment padded to whole 64-point blocks.
+struct Layout {
+    int nseg = 0;
+    int64_t seg_n[3] = {0, 0, 0};
+    int64_t seg_off[3] = {0, 0, 0};
+    int64_t seg_blk0[3] = {0, 0, 0};
+    int64_t seg_nblk[3] = {0, 0, 0};
+    int64_t npad = 0, nblk = 0;
+};
+
+// Tuned launch shape of the triplet kernels.
+struct LaunchShape {
+    int waves = 4;     // waves per workgroup
+    int per_cu = 0;    // workgroups per CU (0 = occupancy limit)
+};
+
+}  // namespace tpcf
+
+struct tpcf_catalog {
+    tpcf_ctx *ctx = nullptr;
+    double *d = nullptr;   // device SoA [x|y|z]
+    int64_t n = 0;
+    std::vector<double> h; // host copy (origin / extent computations)
+};
+
+namespace tpcf {
+struct Ticket;
+struct Workspace;
+}  // namespace tpcf
+
+namespace tpcf {
+int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode,
+              uint64_t *const *hist, tpcf_stats *per, tpcf_stats *total, hipStream_t st);
+int run_single(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf_binspec &spec, int mode, uint64_t *hist,
+               tpcf_stats *stats, hipStream_t st);
+int64_t plan_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode);
+Workspace *new_workspace();
+void delete_workspace(Workspace *w);
+struct WorkspaceDeleter {
+    void operator()(Workspace *w) const { delete_workspace(w); }
+};
+}  // namespace tpcf
+
+struct tpcf_ctx {
+    int device = 0;
+    int num_cu = 0;
+    size_t lds_per_block = 0;
+    hipStream_t stream = nullptr;
+    std::mutex call_mu;                    // serializes synchronous calls
+    std::unique_ptr<tpcf::Workspace, tpcf::WorkspaceDeleter> ws;  // workspace of synchronous calls
+    tpcf::LaunchShape shape;
+    // serving
+    uint64_t budget = 0, in_use = 0;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int64_t> pending;
+    std::map<int64_t, std::shared_ptr<tpcf::Ticket>> tickets;
+    std::vector<std::thread> workers;
+    int64_t next_ticket = 1;
+    bool stopping = false;
+};

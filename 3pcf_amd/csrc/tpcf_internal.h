@@ -1,0 +1,80 @@
+// tpcf_internal.h -- structures shared by the host runtime (tpcf_api.cpp) and
+// the gfx950 kernels (kernels_*.hip).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tpcf {
+
+constexpr int kBlock = 64;          // points per block = one wavefront of lanes
+constexpr int kTileBytes = kBlock * kBlock;
+constexpr uint8_t kSentinel = 0xFF; // pair bin "out of range / not a pair"
+constexpr int kMaxBins = 255;
+constexpr int kMaxLdsR123Bins = 32; // LDS-histogram triplet kernel; wider specs use global atomics
+
+// Tile (r, c), r <= c, of a job's pair-bin matrix; tiles are stored
+// upper-triangle, row-major over r, 4 KiB each, bytes row-major [i][j].
+__host__ __device__ inline int64_t tile_index(int64_t r, int64_t c, int64_t nblk) {
+    return r * nblk - (r * (r - 1)) / 2 + (c - r);
+}
+__host__ __device__ inline int64_t tile_count(int64_t nblk) { return nblk * (nblk + 1) / 2; }
+
+// One (job, term) of the triplet stage.
+struct TripletTerm {
+    const uint8_t *tiles;  // pair-bin tiles of the job
+    int64_t nblk;          // blocks in the job's union layout
+    unsigned long long *hist;  // nterm histogram, n^3 uint64 (sorted-side cells)
+    int n;                 // bins per side
+    int pad;
+};
+
+// A unit of triplet work: blocks A <= B <= C, C in [c0, c1).
+struct TripletItem {
+    int term;  // index into the TripletTerm table
+    int A, B, c0, c1;
+    int pad[3];
+};
+
+// Pair stage of one job.
+struct PairJob {
+    const double *u64;     // union layout SoA: x[npad] y[npad] z[npad]
+    const float4 *u32;     // (x-o, y-o, z-o, 0) in FP32
+    const uint8_t *blk_count;  // real points in each block (0..64)
+    uint8_t *tiles;
+    const double *T64;     // squared thresholds [0..n]
+    const float *T32;      // T64 rounded to float
+    unsigned long long *rechecked;  // FP64 recomputations
+    int64_t npad;
+    int64_t nblk;
+    int n;
+    float rmax;            // max |coordinate - origin| over the union (FP32, rounded up)
+    float rel;             // extra relative window around each threshold (squared units)
+    int shift;             // stored pair value = bin << shift (2 for the LDS kernel, 0 otherwise)
+};
+
+// Plain FP64 baseline of one job.
+struct PlainArgs {
+    const double *u64;
+    const uint8_t *blk_count;
+    const double *T64;
+    const TripletItem *items;
+    int nitems;
+    int n;
+    int64_t npad;
+    unsigned long long *hist;  // sorted cells, n^3
+};
+
+// Launchers (kernels_r123.hip).
+hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st);
+hipError_t launch_r123_triplets(const TripletItem *items, int nitems, int *counter, const TripletTerm *terms,
+                                int max_n, int waves, int grid, hipStream_t st);
+hipError_t launch_r123_plain(const PlainArgs &P, int grid, hipStream_t st);
+hipError_t launch_r123_triplets_wide(const TripletItem *items, int nitems, const TripletTerm *terms, int grid,
+                                     hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_to_f32(const double *u64, int64_t npad, const double o[3], float4 *u32, hipStream_t st);
+int r123_lds_bytes(int n);
+const void *r123_triplet_kernel_ptr(int waves);
+
+}  // namespace tpcf

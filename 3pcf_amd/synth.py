@@ -1,0 +1,78 @@
+"""Seeded synthetic catalogs in the shapes of the benchmark workloads.
+
+There is no network and the reference's large catalogs are not shipped
+(.MISSING_LARGE_BLOBS), so every workload is generated here from a seed:
+
+* ``uniform_cube(n, size, seed)``            uniform points in [0, size)^3;
+* ``neyman_scott(n, nclust, sigma, size, seed)``  clustered data: centres
+  uniform in the cube, each point a centre plus N(0, sigma^2) per axis,
+  wrapped into the cube;
+* ``wedge(n, ra_deg, dec_deg, r_lo, r_hi, seed, clustered=...)``  an RA/Dec
+  wedge with comoving distance uniform in [r_lo, r_hi], converted to
+  cartesian (x = r cos(dec) cos(ra), y = r cos(dec) sin(ra), z = r sin(dec)),
+  the same conversion examples' catalog generator applies
+  (sample_data/gen_flat_MICE_data.py:53-99).
+
+All generators use numpy's PCG64 with the given seed, so a seed names a
+catalog on any machine.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def uniform_cube(n: int, size: float = 100.0, seed: int = 0) -> np.ndarray:
+    return _rng(seed).random((n, 3)) * size
+
+
+def neyman_scott(n: int, nclust: int = 200, sigma: float = 2.0, size: float = 200.0, seed: int = 0) -> np.ndarray:
+    g = _rng(seed)
+    centres = g.random((nclust, 3)) * size
+    pick = g.integers(0, nclust, n)
+    pts = centres[pick] + g.normal(0.0, sigma, (n, 3))
+    return np.mod(pts, size)
+
+
+def wedge(n: int, ra_deg: float = 10.0, dec_deg: float = 10.0, r_lo: float = 500.0, r_hi: float = 800.0,
+          seed: int = 0, clustered: bool = False, nclust: int = 200, sigma: float = 2.0) -> np.ndarray:
+    """Points in an RA/Dec wedge; with clustered=True, Neyman-Scott clusters
+    whose centres are drawn uniformly in the wedge."""
+    g = _rng(seed)
+
+    def draw(m):
+        ra = np.deg2rad(g.random(m) * ra_deg)
+        sdec = g.random(m) * np.sin(np.deg2rad(dec_deg))
+        dec = np.arcsin(sdec)
+        r = r_lo + (r_hi - r_lo) * g.random(m)
+        return np.stack([r * np.cos(dec) * np.cos(ra), r * np.cos(dec) * np.sin(ra), r * np.sin(dec)], axis=1)
+
+    if not clustered:
+        return draw(n)
+    centres = draw(nclust)
+    pick = g.integers(0, nclust, n)
+    return centres[pick] + g.normal(0.0, sigma, (n, 3))
+
+
+def lognormal_voxel_sizes(nvox: int, total: int, lo: int = 500, hi: int = 8000, seed: int = 0) -> np.ndarray:
+    """Uneven voxel occupancies: lognormal, clipped to [lo, hi], rescaled to
+    sum to total (clipping again after rescaling, remainder spread)."""
+    g = _rng(seed)
+    raw = g.lognormal(mean=0.0, sigma=0.8, size=nvox)
+    sizes = np.clip(raw / raw.sum() * total, lo, hi)
+    for _ in range(50):
+        sizes = np.clip(sizes * (total / sizes.sum()), lo, hi)
+    sizes = np.floor(sizes).astype(np.int64)
+    i = 0
+    while sizes.sum() < total:
+        if sizes[i % nvox] < hi:
+            sizes[i % nvox] += 1
+        i += 1
+    while sizes.sum() > total:
+        if sizes[i % nvox] > lo:
+            sizes[i % nvox] -= 1
+        i += 1
+    return sizes

@@ -1,0 +1,189 @@
+/*
+ * tpcf.h -- C ABI of libtpcf, the MI355X (gfx950) brute-force triplet counter
+ * behind the DDD / DDR / DRR / RRR triangle histograms of the three-point
+ * correlation function.
+ *
+ * The reference (mattbellis/3pcf) has no library interface: its boundary is
+ * the command line of its C programs plus text histogram files.  Each entry
+ * point below names the reference code it replaces.  Everything is plain C:
+ * pointers, sizes and status codes; no C++ or torch types cross this line.
+ *
+ * Conventions
+ *   - Every call returns TPCF_OK (0) or a negative TPCF_E* code, and never
+ *     exits the process.  tpcf_last_error() gives a thread-local message.
+ *   - Coordinates are FP64 structure-of-arrays [x(n) | y(n) | z(n)] owned by
+ *     the caller; catalogs are copied to device memory owned by the library
+ *     until tpcf_catalog_destroy().
+ *   - Histograms are uint64 arrays owned by the caller.
+ *   - A context is bound to one device.  tpcf_count() and tpcf_count_batch()
+ *     are synchronous; tpcf_submit()/tpcf_poll()/tpcf_wait() are the
+ *     asynchronous serving interface and are thread-safe.
+ *
+ * Semantics ("clean64"): the histograms equal, bit for bit, a CPU evaluation
+ * in IEEE FP64 of the following definitions.
+ *   - squared side s = ((dx*dx + dy*dy) + dz*dz), d = sqrt(s) (round to
+ *     nearest), dx = x_a - x_b in FP64;
+ *   - an axis with n bins on [lo, hi) has edges e[m] = lo + (hi-lo)*m/n
+ *     (linear) or exp(log lo + (log hi - log lo)*m/n) (log), e[0]=lo,
+ *     e[n]=hi; a value v is in bin m iff e[m] <= v < e[m+1], otherwise it is
+ *     out of range and its entry is dropped;
+ *   - TPCF_R123: sides sorted r1<=r2<=r3, one entry at cell (b1*n+b2)*n+b3,
+ *     all three sides use axis 0;
+ *   - TPCF_SQT: three entries per triangle (the vertex parametrizations of
+ *     src/3pcf_C_version_triangle_parametrization.cc:192-210):
+ *       (s=r1, q=r2/r1, angle between r1 and r2),
+ *       (s=r2, q=r3/r2, angle between r2 and r3),
+ *       (s=r1, q=r3/r1, angle between r1 and r3);
+ *     the angle axis is theta/pi in [0,1], binned by comparing the cosine
+ *     rule value ((sa+sb)-sc)/((2*da)*db) against cos(pi*edge); triangles
+ *     with a zero side are dropped; cell (i_s*nq + i_q)*nt + i_t;
+ *   - TPCF_JBJ: one entry (s=r1, q=r1/r2, v=(r3-r2)/r1)
+ *     (src/3pcf_C_version_triangle_parametrization_JBJ.cc:192-198);
+ *   - triplets are unique: a term is a sorted triple (a<=b<=c) of segment
+ *     labels; (a,a,a) counts i<j<k, (a,a,b) i<j in a and k in b, (a,b,b) i in
+ *     a and j<k in b, (a,b,c) the full product.  DDD/DDR/DRR/RRR of data
+ *     segment 0 and random segment 1 are (0,0,0), (0,0,1), (0,1,1), (1,1,1).
+ */
+#ifndef TPCF_H
+#define TPCF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TPCF_VERSION_MAJOR 0
+#define TPCF_VERSION_MINOR 1
+
+enum {
+    TPCF_OK = 0,
+    TPCF_EINVAL = -1,   /* bad argument or bin spec */
+    TPCF_EDEVICE = -2,  /* HIP runtime error */
+    TPCF_ENOMEM = -3,   /* device or host allocation failed */
+    TPCF_EIO = -4,      /* catalog or histogram file error */
+    TPCF_EBUSY = -5,    /* admission refused: not enough device memory now */
+    TPCF_ENOTFOUND = -6 /* unknown ticket */
+};
+
+/* Parametrizations. */
+enum { TPCF_R123 = 0, TPCF_SQT = 1, TPCF_JBJ = 2 };
+/* Axis scales. */
+enum { TPCF_LIN = 0, TPCF_LOG = 1 };
+/* Device implementations (identical histograms). */
+enum {
+    TPCF_MODE_FAST = 0,   /* FP32 pair stage + FP64 recheck, integer triplet kernel */
+    TPCF_MODE_PLAIN = 1,  /* FP64 distances recomputed per triplet, one job per launch */
+    TPCF_MODE_FAST64 = 2  /* as FAST with an all-FP64 pair stage */
+};
+
+typedef struct tpcf_ctx tpcf_ctx;
+typedef struct tpcf_catalog tpcf_catalog;
+
+typedef struct {
+    int n;        /* bins, 1..255 (R123 kernels: 1..32) */
+    double lo, hi;
+    int scale;    /* TPCF_LIN / TPCF_LOG */
+} tpcf_axis;
+
+typedef struct {
+    int param;          /* TPCF_R123 / TPCF_SQT / TPCF_JBJ */
+    tpcf_axis ax[3];    /* R123 reads ax[0] only */
+    double recheck_rel; /* FAST mode: extra relative window around each edge
+                           inside which pairs are recomputed in FP64 (0 = the
+                           rigorous FP32 error bound alone); default 1e-5 */
+} tpcf_binspec;
+
+/* One counting job: up to three segments, each an index range [lo, hi) of a
+ * device catalog, and 1..4 terms over them. */
+typedef struct {
+    int nseg;
+    tpcf_catalog *seg[3];
+    int64_t seg_lo[3], seg_hi[3];
+    int nterms;
+    int term[4][3];
+} tpcf_job;
+
+typedef struct {
+    uint64_t triplets;         /* unique triplets classified (all terms) */
+    uint64_t pairs;            /* pairs binned in the pair stage */
+    uint64_t pairs_rechecked;  /* pairs recomputed in FP64 (FAST) */
+    double device_ms;          /* kernel time measured with HIP events */
+    double wall_ms;            /* host wall time of the call */
+} tpcf_stats;
+
+/* ---- library / context ------------------------------------------------- */
+
+const char *tpcf_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char *tpcf_last_error(void);
+
+/* Bind a context to a HIP device (replaces the implicit CUDA device setup of
+ * src/3pcf_CUDA_version_triangle_parametrization.cu:705-731). */
+int tpcf_ctx_create(int device, tpcf_ctx **out);
+int tpcf_ctx_destroy(tpcf_ctx *ctx);
+/* Cap on device memory the context may hold for job workspaces (bytes);
+ * 0 = 90% of free memory at creation.  Admission control of tpcf_submit(). */
+int tpcf_ctx_set_memory_budget(tpcf_ctx *ctx, uint64_t bytes);
+
+/* ---- catalogs ------------------------------------------------------------ */
+
+/* Upload n points (FP64 SoA) to the device.  Replaces the float host arrays
+ * filled at src/3pcf_C_version_triangle_parametrization.cc:366-439. */
+int tpcf_catalog_create(tpcf_ctx *ctx, const double *xyz_soa, int64_t n, tpcf_catalog **out);
+int tpcf_catalog_destroy(tpcf_catalog *cat);
+int64_t tpcf_catalog_size(const tpcf_catalog *cat);
+
+/* Read a catalog file: format 7 = "%d %f %f %f %f %f %f" with x y z in
+ * columns 5-7 (src/3pcf_C_version_triangle_parametrization.cc:366-394);
+ * format 3 = "%d" count header then "x y z" lines, each divided by scale
+ * (src/3pcf_C_version.cc:311-332).  Values are parsed as FP64.  *out is
+ * malloc'ed SoA; free with tpcf_free().  Returns the point count (>= 0) or a
+ * negative error. */
+int64_t tpcf_read_catalog(const char *path, int format, double scale, double **out);
+void tpcf_free(void *p);
+
+/* ---- bin specs --------------------------------------------------------- */
+
+/* Cells of one term histogram: n^3 (R123) or n0*n1*n2. */
+int64_t tpcf_hist_cells(const tpcf_binspec *spec);
+/* Edge table e[0..n] of an axis (the host table every kernel uses). */
+int tpcf_axis_edges(const tpcf_axis *ax, double *edges);
+/* Validate a spec (returns TPCF_OK or TPCF_EINVAL with a message). */
+int tpcf_binspec_check(const tpcf_binspec *spec);
+
+/* ---- counting ---------------------------------------------------------- */
+
+/* Count all terms of one job into hist (nterms * tpcf_hist_cells() uint64,
+ * term-major).  Replaces the triple loop + distance() + distance_to_bin()
+ * of src/3pcf_C_version_triangle_parametrization.cc:477-533, :116-240,
+ * :84-113 and of src/3pcf_C_version.cc:363-432, :54-142, :23-51. */
+int tpcf_count(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int mode,
+               uint64_t *hist, tpcf_stats *stats);
+
+/* Count many jobs (for instance every voxel and term of a survey) in one
+ * persistent device launch.  hist[j] receives job j's histograms.  Replaces
+ * the per-voxel process fan-out of examples/build_voxel_submission_scripts.py
+ * :9-41 and examples/run_CPU_calculation_each_voxels.csh:60-67.  mode must be
+ * TPCF_MODE_FAST or TPCF_MODE_FAST64. */
+int tpcf_count_batch(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs,
+                     int mode, uint64_t *const *hist, tpcf_stats *stats);
+
+/* Serving: enqueue a job (admitted when its device workspace fits the
+ * memory budget; otherwise it waits in the queue), returns a ticket.  hist
+ * must stay valid until the ticket completes.  No reference counterpart
+ * (the reference runs one process per job). */
+int tpcf_submit(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int mode,
+                uint64_t *hist, int64_t *ticket);
+/* Poll: 1 = done (the ticket is then released), 0 = pending, < 0 = the
+ * job's error.  Wait: blocks, then TPCF_OK or the job's error. */
+int tpcf_poll(tpcf_ctx *ctx, int64_t ticket, tpcf_stats *stats);
+int tpcf_wait(tpcf_ctx *ctx, int64_t ticket, tpcf_stats *stats);
+
+/* Device bytes the job's workspace needs (admission footprint). */
+int64_t tpcf_job_footprint(const tpcf_job *job, const tpcf_binspec *spec, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
