@@ -71,6 +71,8 @@ class Stats(C.Structure):
         ("pairs_rechecked", C.c_uint64),
         ("device_ms", C.c_double),
         ("wall_ms", C.c_double),
+        ("pair_ms", C.c_double),
+        ("triplet_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -140,27 +140,42 @@ __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
 // 32-bit counters); 0xFF is the sentinel, and every valid value is < 128.
 //
 // LDS histogram of one term with n bins: unsorted cube word
-// z*n^2 + y*n + x for x = bin(i,j), y = bin(i,k), z = bin(j,k).  A lane whose
-// z is out of range adds into a dump word n^3 + lane + y*n + x instead of
-// branching, so the dump needs n^2 + 64 extra words.
-__host__ __device__ inline int r123_lds_words(int n) { return n * n * n + n * n + 64; }
+// z*sz + y*sy + x for x = bin(i,j), y = bin(i,k), z = bin(j,k), with odd
+// strides sy = n|1 and sz = (n*sy)|1 so that the lanes of one instruction
+// (same x, varying y and z) spread over all 32 banks.  A lane whose z is out
+// of range adds into a dump word n*sz + lane + y*sy + x instead of branching,
+// so the dump needs sz + 64 extra words.  COPIES privatized copies of the cube
+// (lane k writes copy k % COPIES) cut same-address serialization of LDS
+// atomics when few cells are hot; copies start 32/COPIES banks apart.
+__host__ __device__ inline int r123_sy(int n) { return n | 1; }
+__host__ __device__ inline int r123_sz(int n) { return (n * r123_sy(n)) | 1; }
+__host__ __device__ inline int r123_lds_words(int n) { return (n + 1) * r123_sz(n) + 64; }
+__host__ __device__ inline int r123_copy_stride(int n, int copies) {
+    return ((r123_lds_words(n) + 31) & ~31) + (copies > 1 ? 32 / copies : 0);
+}
 
-// Fold the privatized unsorted cube into the term's sorted uint64 histogram
-// and clear it (dump words included).
+// Fold the privatized unsorted cubes into the term's sorted uint64 histogram
+// and clear them (dump words included).
+template <int COPIES>
 __device__ inline void r123_flush(uint32_t *hist, int n, unsigned long long *g) {
     __syncthreads();
-    const int n2 = n * n, n3 = n2 * n;
+    const int sy = r123_sy(n), sz = r123_sz(n);
     const int words = r123_lds_words(n);
+    const int stride = r123_copy_stride(n, COPIES);
     for (int c = threadIdx.x; c < words; c += blockDim.x) {
-        const uint32_t v = hist[c];
-        if (c < n3 && v) {
-            int z = c / n2, y = (c / n) % n, x = c % n, t;
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < COPIES; ++k) {
+            v += hist[k * stride + c];
+            hist[k * stride + c] = 0u;
+        }
+        int z = c / sz, y = (c % sz) / sy, x = (c % sz) % sy, t;
+        if (z < n && y < n && x < n && v) {
             if (x > y) { t = x; x = y; y = t; }
             if (y > z) { t = y; y = z; z = t; }
             if (x > y) { t = x; x = y; y = t; }
             atomicAdd(g + (x * n + y) * n + z, (unsigned long long)v);
         }
-        hist[c] = 0u;
     }
     __syncthreads();
 }
@@ -174,14 +189,14 @@ __device__ inline void lds_add1(uint32_t *hist, uint32_t byte_addr) {
 
 // Persistent: workgroups pull items (term, A, B, C-range) from a global
 // counter until the queue is drained; the waves of a workgroup split the C
-// blocks of an item (wave w takes C = c0+w, c0+w+WAVES, ...) and share one LDS
-// histogram.  Lane k owns point k of block C: z = bin(j,k) is preloaded for
-// the 64 columns j of block B, y = bin(i,k) is per lane and fixed for a row
-// i of block A, and x = bin(i,j) is uniform, read with scalar loads.  A row
-// whose 64 x are all valid takes the branch-free path (one s_bfe, one
-// v_add3, one ds_add per 64 triplets); a row with none valid is skipped;
-// mixed rows test x per column.
-template <int WAVES>
+// blocks of an item (wave w takes C = c0+w, c0+w+WAVES, ...) and share the
+// LDS histogram copies.  Lane k owns point k of block C: z = bin(j,k) is
+// preloaded for the 64 columns j of block B, y = bin(i,k) is per lane and
+// fixed for a row i of block A, and x = bin(i,j) is uniform, read with scalar
+// loads.  A row whose 64 x are all valid takes the branch-free path (one
+// s_bfe, one v_add3, one ds_add per 64 triplets); a row with none valid is
+// skipped; mixed rows test x per column.
+template <int WAVES, int COPIES>
 __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletItem *__restrict__ items,
                                                                   int nitems, int *__restrict__ counter,
                                                                   const TripletTerm *__restrict__ terms,
@@ -215,7 +230,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
         const int c1 = __builtin_amdgcn_readfirstlane(I.c1);
         const unsigned long long add = (unsigned long long)(c1 - c0) * (kBlock * kBlock * kBlock);
         if (term != cur || since_flush + add > flush_limit) {
-            if (cur >= 0) r123_flush(hist, cur_n, cur_hist);
+            if (cur >= 0) r123_flush<COPIES>(hist, cur_n, cur_hist);
             cur = term;
             cur_n = n;
             cur_hist = T.hist;
@@ -223,26 +238,27 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
         }
         since_flush += add;
         const int64_t nblk = T.nblk;
-        const uint32_t n1 = (uint32_t)n, n2 = n1 * n1;
-        const uint32_t dump = (n2 * n1 + (uint32_t)lane) * 4u;
+        const uint32_t sy = (uint32_t)r123_sy(n), sz = (uint32_t)r123_sz(n);
+        const uint32_t copy_base = (uint32_t)((lane % COPIES) * r123_copy_stride(n, COPIES)) * 4u;
+        const uint32_t dump = copy_base + ((uint32_t)n * sz + (uint32_t)lane) * 4u;
         const uint8_t *tAB = T.tiles + tile_index(A, B, nblk) * kTileBytes;
         for (int C = c0 + wave; C < c1; C += WAVES) {
             const uint8_t *tAC = T.tiles + tile_index(A, C, nblk) * kTileBytes;
             const uint8_t *tBC = T.tiles + tile_index(B, C, nblk) * kTileBytes;
-            // z of the 64 columns as byte offsets of the z slab, or the
-            // lane's dump slot for the sentinel.
+            // z of the 64 columns as byte offsets of the z slab in the lane's
+            // copy, or the lane's dump slot for the sentinel.
             uint32_t Zs[kBlock];
 #pragma unroll
             for (int j = 0; j < kBlock; ++j) Zs[j] = tBC[j * kBlock + lane];
 #pragma unroll
-            for (int j = 0; j < kBlock; ++j) Zs[j] = Zs[j] < 128u ? __umul24(Zs[j], n2) : dump;
+            for (int j = 0; j < kBlock; ++j) Zs[j] = Zs[j] < 128u ? __umul24(Zs[j], sz) + copy_base : dump;
             for (int i = 0; i < kBlock; ++i) {
                 const uint32_t y4 = tAC[i * kBlock + lane];
                 const uint32_t xv = tAB[i * kBlock + lane];
                 const unsigned long long rowmask = __ballot(xv < 128u);
                 if (rowmask == 0ull) continue;
                 if (y4 >= 128u) continue;
-                const uint32_t Ys = __umul24(y4, n1);
+                const uint32_t Ys = __umul24(y4, sy);
                 scalar_u32_ptr xrow = (scalar_u32_ptr)(tAB + i * kBlock);
                 uint32_t xw[16];
 #pragma unroll
@@ -269,7 +285,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
             }
         }
     }
-    if (cur >= 0) r123_flush(hist, cur_n, cur_hist);
+    if (cur >= 0) r123_flush<COPIES>(hist, cur_n, cur_hist);
 }
 
 // ---------------------------------------------------------------------------
@@ -389,39 +405,32 @@ hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st) {
     return hipGetLastError();
 }
 
-int r123_lds_bytes(int n) { return r123_lds_words(n) * 4; }
+#define TPCF_R123_SHAPES(X) X(4, 1) X(8, 1) X(16, 1) X(8, 2) X(16, 2) X(8, 4) X(16, 4) X(8, 8) X(16, 8)
 
-const void *r123_triplet_kernel_ptr(int waves) {
-    switch (waves) {
-    case 4: return reinterpret_cast<const void *>(&r123_triplet_kernel<4>);
-    case 8: return reinterpret_cast<const void *>(&r123_triplet_kernel<8>);
-    case 16: return reinterpret_cast<const void *>(&r123_triplet_kernel<16>);
-    default: return nullptr;
-    }
+const void *r123_triplet_kernel_ptr(int waves, int copies) {
+#define TPCF_PTR(W, K) \
+    if (waves == W && copies == K) return reinterpret_cast<const void *>(&r123_triplet_kernel<W, K>);
+    TPCF_R123_SHAPES(TPCF_PTR)
+#undef TPCF_PTR
+    return nullptr;
 }
 
+int r123_lds_bytes(int n, int copies) { return copies * r123_copy_stride(n, copies) * 4; }
+
 hipError_t launch_r123_triplets(const TripletItem *items, int nitems, int *counter, const TripletTerm *terms,
-                                int max_n, int waves, int grid, hipStream_t st) {
+                                int max_n, int waves, int copies, int grid, hipStream_t st) {
     if (max_n < 1 || max_n > kMaxLdsR123Bins) return hipErrorInvalidValue;
-    const int words = r123_lds_words(max_n);
+    const int words = copies * r123_copy_stride(max_n, copies);
     const size_t lds = (size_t)words * 4;
-    switch (waves) {
-    case 4:
-        hipLaunchKernelGGL(r123_triplet_kernel<4>, dim3(grid), dim3(256), lds, st, items, nitems, counter, terms,
-                           words);
-        break;
-    case 8:
-        hipLaunchKernelGGL(r123_triplet_kernel<8>, dim3(grid), dim3(512), lds, st, items, nitems, counter, terms,
-                           words);
-        break;
-    case 16:
-        hipLaunchKernelGGL(r123_triplet_kernel<16>, dim3(grid), dim3(1024), lds, st, items, nitems, counter, terms,
-                           words);
-        break;
-    default:
-        return hipErrorInvalidValue;
+#define TPCF_LAUNCH(W, K)                                                                                    \
+    if (waves == W && copies == K) {                                                                         \
+        hipLaunchKernelGGL((r123_triplet_kernel<W, K>), dim3(grid), dim3(W * 64), lds, st, items, nitems, counter, \
+                           terms, words);                                                                    \
+        return hipGetLastError();                                                                            \
     }
-    return hipGetLastError();
+    TPCF_R123_SHAPES(TPCF_LAUNCH)
+#undef TPCF_LAUNCH
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_r123_triplets_wide(const TripletItem *items, int nitems, const TripletTerm *terms, int grid,

@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 
 #include "tpcf_host.h"
 
@@ -276,14 +277,39 @@ struct Workspace {
     }
 };
 
-static int grid_for(tpcf_ctx *ctx, int waves, int max_n) {
-    const void *k = r123_triplet_kernel_ptr(waves);
+// Launch shape of the LDS triplet kernel for bins up to max_n: as many
+// privatized histogram copies (4, 2 or 1) as fit the LDS budget, and enough
+// waves per workgroup for four waves per SIMD at the resulting occupancy.
+// TPCF_R123_SHAPE="waves,copies[,wg_per_cu]" overrides (tuning runs).
+struct TripletShape {
+    int waves, copies, grid;
+};
+
+static TripletShape triplet_shape(tpcf_ctx *ctx, int max_n) {
+    constexpr int kLdsBudget = 155 * 1024;
+    TripletShape s{8, 1, 0};
+    for (int k : {8, 4, 2, 1})
+        if (r123_lds_bytes(max_n, k) <= kLdsBudget) {
+            s.copies = k;
+            break;
+        }
+    const int lds = r123_lds_bytes(max_n, s.copies);
+    s.waves = lds > kLdsBudget / 2 ? 16 : 8;
+    int per_cu_cap = 0;
+    if (ctx->shape.waves > 0 && r123_triplet_kernel_ptr(ctx->shape.waves, ctx->shape.copies) &&
+        r123_lds_bytes(max_n, ctx->shape.copies) <= kLdsBudget) {
+        s.waves = ctx->shape.waves;
+        s.copies = ctx->shape.copies;
+        per_cu_cap = ctx->shape.per_cu;
+    }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, waves * 64, r123_lds_bytes(max_n)) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, r123_triplet_kernel_ptr(s.waves, s.copies),
+                                                     s.waves * 64, r123_lds_bytes(max_n, s.copies)) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
-    if (ctx->shape.per_cu > 0) per_cu = std::min(per_cu, ctx->shape.per_cu);
-    return ctx->num_cu * per_cu;
+    if (per_cu_cap > 0) per_cu = std::min(per_cu, per_cu_cap);
+    s.grid = ctx->num_cu * per_cu;
+    return s;
 }
 
 // Launch the triplet stage of several jobs as one persistent kernel.
@@ -321,6 +347,7 @@ static int launch_triplets(tpcf_ctx *ctx, Workspace &ws, const std::vector<const
     HIP_TRY(hipMemcpyAsync(q + o_items, items.data(), sizeof(TripletItem) * items.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(q + o_terms, terms.data(), sizeof(TripletTerm) * terms.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(q + o_ctr), 1, 0u, st));
+    HIP_TRY(hipEventRecord(ws.ev[3], st));
     if (wide) {
         // A batch mixing wide and narrow specs runs everything on the wide
         // kernel, whose pair values are unscaled.
@@ -328,10 +355,10 @@ static int launch_triplets(tpcf_ctx *ctx, Workspace &ws, const std::vector<const
                                           reinterpret_cast<const TripletTerm *>(q + o_terms), ctx->num_cu * 8, st));
         return TPCF_OK;
     }
-    const int waves = ctx->shape.waves;
+    const TripletShape sh = triplet_shape(ctx, max_n);
     HIP_TRY(launch_r123_triplets(reinterpret_cast<const TripletItem *>(q + o_items), (int)items.size(),
                                  reinterpret_cast<int *>(q + o_ctr), reinterpret_cast<const TripletTerm *>(q + o_terms),
-                                 max_n, waves, grid_for(ctx, waves, max_n), st));
+                                 max_n, sh.waves, sh.copies, sh.grid, st));
     return TPCF_OK;
 }
 
@@ -360,6 +387,16 @@ static int launch_plain(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tp
     return TPCF_OK;
 }
 
+// Pair-stage and triplet-kernel times from the workspace events.
+static int stage_times(Workspace &ws, tpcf_stats *s) {
+    float a = 0.f, b = 0.f;
+    HIP_TRY(hipEventElapsedTime(&a, ws.ev[0], ws.ev[1]));
+    HIP_TRY(hipEventElapsedTime(&b, ws.ev[3], ws.ev[2]));
+    s->pair_ms = a;
+    s->triplet_ms = b;
+    return TPCF_OK;
+}
+
 // Run one job synchronously on stream st with workspace ws.
 static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf_binspec &spec, int mode,
                    uint64_t *hist, tpcf_stats *stats, hipStream_t st) {
@@ -371,6 +408,8 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
     char *base = reinterpret_cast<char *>(ws.region.p);
     if (int e = stage_job(job, P, base, st)) return e;
     HIP_TRY(hipEventRecord(ws.ev[0], st));
+    HIP_TRY(hipEventRecord(ws.ev[1], st));
+    HIP_TRY(hipEventRecord(ws.ev[3], st));
     if (mode == TPCF_MODE_PLAIN) {
         if (P.L.nblk) {
             if (int e = launch_plain(ctx, ws, P, job, base, st)) return e;
@@ -398,6 +437,7 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
         stats->pairs_rechecked = mode == TPCF_MODE_FAST ? rechecked : 0;
         stats->device_ms = ms;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (int e = stage_times(ws, stats)) return e;
     }
     return TPCF_OK;
 }
@@ -425,6 +465,7 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     for (int j = 0; j < njobs; ++j)
         if (int e = stage_job(jobs[j], plans[j], base + off[j], st)) return e;
     HIP_TRY(hipEventRecord(ws.ev[0], st));
+    HIP_TRY(hipEventRecord(ws.ev[3], st));
     int max_n = 1;
     for (int j = 0; j < njobs; ++j) max_n = std::max(max_n, plans[j].n);
     const int shift = max_n <= kMaxLdsR123Bins ? 2 : 0;
@@ -468,6 +509,11 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     }
     tot.device_ms = ms;
     tot.wall_ms = wall;
+    if (int e = stage_times(ws, &tot)) return e;
+    for (int j = 0; j < njobs && per; ++j) {
+        per[j].pair_ms = tot.pair_ms;
+        per[j].triplet_ms = tot.triplet_ms;
+    }
     if (total) *total = tot;
     return TPCF_OK;
 }
@@ -523,9 +569,13 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) c->budget = (uint64_t)(0.9 * (double)fr);
     c->ws.reset(new Workspace());
     // r123 triplet kernels raise their dynamic LDS limit once.
-    for (int w : {4, 8, 16}) {
-        const void *k = r123_triplet_kernel_ptr(w);
-        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+    for (int w : {4, 8, 16})
+        for (int k : {1, 2, 4, 8})
+            if (const void *fn = r123_triplet_kernel_ptr(w, k))
+                (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+    if (const char *env = std::getenv("TPCF_R123_SHAPE")) {
+        int w = 0, k = 0, p = 0;
+        if (std::sscanf(env, "%d,%d,%d", &w, &k, &p) >= 2) c->shape = LaunchShape{w, k, p};
     }
     *out = c;
     return TPCF_OK;

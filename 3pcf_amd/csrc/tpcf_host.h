@@ -46,7 +46,8 @@ struct Layout {
 
 // Tuned launch shape of the triplet kernels.
 struct LaunchShape {
-    int waves = 4;     // waves per workgroup
+    int waves = 0;     // waves per workgroup (0 = automatic)
+    int copies = 1;    // privatized LDS histogram copies
     int per_cu = 0;    // workgroups per CU (0 = occupancy limit)
 };
 

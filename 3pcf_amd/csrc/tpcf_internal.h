@@ -68,13 +68,13 @@ struct PlainArgs {
 // Launchers (kernels_r123.hip).
 hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st);
 hipError_t launch_r123_triplets(const TripletItem *items, int nitems, int *counter, const TripletTerm *terms,
-                                int max_n, int waves, int grid, hipStream_t st);
+                                int max_n, int waves, int copies, int grid, hipStream_t st);
 hipError_t launch_r123_plain(const PlainArgs &P, int grid, hipStream_t st);
 hipError_t launch_r123_triplets_wide(const TripletItem *items, int nitems, const TripletTerm *terms, int grid,
                                      hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_to_f32(const double *u64, int64_t npad, const double o[3], float4 *u32, hipStream_t st);
-int r123_lds_bytes(int n);
-const void *r123_triplet_kernel_ptr(int waves);
+int r123_lds_bytes(int n, int copies);
+const void *r123_triplet_kernel_ptr(int waves, int copies);
 
 }  // namespace tpcf

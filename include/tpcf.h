@@ -108,8 +108,10 @@ typedef struct {
     uint64_t triplets;         /* unique triplets classified (all terms) */
     uint64_t pairs;            /* pairs binned in the pair stage */
     uint64_t pairs_rechecked;  /* pairs recomputed in FP64 (FAST) */
-    double device_ms;          /* kernel time measured with HIP events */
-    double wall_ms;            /* host wall time of the call */
+    double device_ms;          /* device time of the counting stages (HIP events) */
+    double wall_ms;            /* host wall time of the call (serving: submit to done) */
+    double pair_ms;            /* pair stage (FAST modes) */
+    double triplet_ms;         /* triplet kernel alone */
 } tpcf_stats;
 
 /* ---- library / context ------------------------------------------------- */
