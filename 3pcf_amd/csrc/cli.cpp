@@ -128,7 +128,7 @@ int main(int argc, char **argv) {
         spec.ax[1] = {16, 0.9, 4.1, TPCF_LIN};
         spec.ax[2] = {32, 0.0, 1.0, TPCF_LIN};
     }
-    int mode = TPCF_MODE_FAST;
+    int mode = -1;  // default: FAST for (r1,r2,r3), PLAIN (FP64) for the triangle parametrizations
     int format = triprog ? 0 : 1;
     bool have_r123_range = false;
     double r_lo = 0, r_hi = 0;
@@ -191,6 +191,7 @@ int main(int argc, char **argv) {
         default: break;
         }
     }
+    if (mode < 0) mode = spec.param == TPCF_R123 ? TPCF_MODE_FAST : TPCF_MODE_PLAIN;
     if (have_r123_range) {
         if (r_n > 0) spec.ax[0].n = r_n;
         if (r_hi > r_lo) {
@@ -219,6 +220,8 @@ int main(int argc, char **argv) {
             return 1;
         }
         cat[i].real = rp;
+    }
+    for (int i = 0; i < 3; ++i) {
         int same = -1;
         for (int j = 0; j < i; ++j)
             if (cat[j].real == cat[i].real) same = j;

@@ -149,12 +149,16 @@ struct JobPlan {
     double origin[3] = {0, 0, 0};
     float rmax = 0.f;
     float rel = 0.f;
-    std::vector<double> T64;
+    std::vector<double> T64;   // squared thresholds of axis 0 (distance / s)
     std::vector<float> T32;
+    std::vector<double> Eq;    // SQT/JBJ: q edges
+    std::vector<double> Et;    // SQT: cos(pi*edge) of theta/pi; JBJ: (r3-r2)/r1 edges
+    int nq = 0, nt = 0;
     std::vector<uint8_t> blk_count;
     uint64_t triplets = 0;
     // byte offsets inside the job's device region
-    size_t o_u64 = 0, o_u32 = 0, o_blk = 0, o_T64 = 0, o_T32 = 0, o_cnt = 0, o_hist = 0, o_tiles = 0;
+    size_t o_u64 = 0, o_u32 = 0, o_blk = 0, o_T64 = 0, o_T32 = 0, o_Eq = 0, o_Et = 0, o_cnt = 0, o_hist = 0,
+           o_tiles = 0;
     size_t bytes = 0;
     bool need_tiles = true;
 };
@@ -164,8 +168,8 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     if (int e = binspec_check(spec)) return e;
     if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_PLAIN && mode != TPCF_MODE_FAST64)
         return set_error(TPCF_EINVAL, "unknown mode %d", mode);
-    if (spec.param != TPCF_R123)
-        return set_error(TPCF_EINVAL, "parametrization %d is not available on the device yet", spec.param);
+    if (spec.param != TPCF_R123 && mode != TPCF_MODE_PLAIN)
+        return set_error(TPCF_EINVAL, "(s,q,theta) and JBJ histograms run in TPCF_MODE_PLAIN");
     P.L = make_layout(job);
     P.param = spec.param;
     P.n = spec.ax[0].n;
@@ -178,6 +182,18 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     sq_thresholds(e, P.n, P.T64.data());
     P.T32.resize(P.n + 1);
     for (int m = 0; m <= P.n; ++m) P.T32[m] = (float)P.T64[m];
+    if (spec.param != TPCF_R123) {
+        P.nq = spec.ax[1].n;
+        P.nt = spec.ax[2].n;
+        P.Eq.resize(P.nq + 1);
+        P.Et.resize(P.nt + 1);
+        axis_edges(spec.ax[1], P.Eq.data());
+        axis_edges(spec.ax[2], P.Et.data());
+        // theta/pi edges become a descending cosine table: theta >= pi*t
+        // <=> cos(theta) <= cos(pi*t) on [0, pi].
+        if (spec.param == TPCF_SQT)
+            for (double &t : P.Et) t = std::cos(M_PI * t);
+    }
     P.blk_count.assign(P.L.nblk, 0);
     for (int s = 0; s < P.L.nseg; ++s)
         for (int64_t b = 0; b < P.L.seg_nblk[s]; ++b)
@@ -211,6 +227,8 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     P.o_blk = o; o = align256(o + P.L.nblk + 1);
     P.o_T64 = o; o = align256(o + sizeof(double) * (P.n + 1));
     P.o_T32 = o; o = align256(o + sizeof(float) * (P.n + 1));
+    P.o_Eq = o; o = align256(o + sizeof(double) * P.Eq.size());
+    P.o_Et = o; o = align256(o + sizeof(double) * P.Et.size());
     P.o_cnt = o; o = align256(o + 64);
     P.o_hist = o; o = align256(o + sizeof(uint64_t) * P.cells * P.nterms);
     P.o_tiles = o; o = align256(o + (P.need_tiles ? (size_t)tile_count(P.L.nblk) * kTileBytes : 0));
@@ -234,6 +252,10 @@ static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStrea
     HIP_TRY(hipMemcpyAsync(base + P.o_blk, P.blk_count.data(), P.blk_count.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(base + P.o_T64, P.T64.data(), sizeof(double) * P.T64.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(base + P.o_T32, P.T32.data(), sizeof(float) * P.T32.size(), hipMemcpyHostToDevice, st));
+    if (!P.Eq.empty()) {
+        HIP_TRY(hipMemcpyAsync(base + P.o_Eq, P.Eq.data(), sizeof(double) * P.Eq.size(), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(base + P.o_Et, P.Et.data(), sizeof(double) * P.Et.size(), hipMemcpyHostToDevice, st));
+    }
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(base + P.o_cnt), 16, 0u, st));
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(base + P.o_hist), 2 * P.cells * P.nterms, 0u, st));
     if (P.need_tiles && P.L.npad) HIP_TRY(launch_to_f32(u64, P.L.npad, P.origin, reinterpret_cast<float4 *>(base + P.o_u32), st));
@@ -370,6 +392,25 @@ static int launch_plain(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tp
         if (items.empty()) continue;
         if (int e = ws.queue.reserve(sizeof(TripletItem) * items.size())) return e;
         HIP_TRY(hipMemcpyAsync(ws.queue.p, items.data(), sizeof(TripletItem) * items.size(), hipMemcpyHostToDevice, st));
+        if (P.param != TPCF_R123) {
+            SqtArgs S{};
+            S.u64 = reinterpret_cast<const double *>(base + P.o_u64);
+            S.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
+            S.Ts = reinterpret_cast<const double *>(base + P.o_T64);
+            S.Eq = reinterpret_cast<const double *>(base + P.o_Eq);
+            S.Et = reinterpret_cast<const double *>(base + P.o_Et);
+            S.items = reinterpret_cast<const TripletItem *>(ws.queue.p);
+            S.nitems = (int)items.size();
+            S.ns = P.n;
+            S.nq = P.nq;
+            S.nt = P.nt;
+            S.npad = P.L.npad;
+            S.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist) + (size_t)t * P.cells;
+            const int grid = std::min<int64_t>((int64_t)items.size(), (int64_t)ctx->num_cu * 2);
+            HIP_TRY(launch_sqt_plain(S, P.param, grid, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            continue;
+        }
         PlainArgs A{};
         A.u64 = reinterpret_cast<const double *>(base + P.o_u64);
         A.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
@@ -573,6 +614,9 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
         for (int k : {1, 2, 4, 8})
             if (const void *fn = r123_triplet_kernel_ptr(w, k))
                 (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+    for (int p : {kSqt, kJbj})
+        (void)hipFuncSetAttribute(sqt_plain_kernel_ptr(p), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024 - 8 * 1024);
     if (const char *env = std::getenv("TPCF_R123_SHAPE")) {
         int w = 0, k = 0, p = 0;
         if (std::sscanf(env, "%d,%d,%d", &w, &k, &p) >= 2) c->shape = LaunchShape{w, k, p};

@@ -65,6 +65,25 @@ struct PlainArgs {
     unsigned long long *hist;  // sorted cells, n^3
 };
 
+// Parametrization codes (equal to TPCF_SQT / TPCF_JBJ of the C ABI).
+constexpr int kSqt = 1, kJbj = 2;
+
+// (s,q,theta) / JBJ FP64 kernel of one job term.
+struct SqtArgs {
+    const double *u64;
+    const uint8_t *blk_count;
+    const double *Ts;  // squared thresholds of the s axis [0..ns]
+    const double *Eq;  // q edges [0..nq]
+    const double *Et;  // SQT: cos(pi*edge) of the theta/pi axis; JBJ: edges of (r3-r2)/r1
+    const TripletItem *items;
+    int nitems;
+    int ns, nq, nt;
+    int64_t npad;
+    unsigned long long *hist;  // ns*nq*nt uint64
+};
+hipError_t launch_sqt_plain(const SqtArgs &P, int param, int grid, hipStream_t st);
+const void *sqt_plain_kernel_ptr(int param);
+
 // Launchers (kernels_r123.hip).
 hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st);
 hipError_t launch_r123_triplets(const TripletItem *items, int nitems, int *counter, const TripletTerm *terms,

@@ -1,0 +1,78 @@
+"""GPU parity of the (s,q,theta) and JBJ triangle parametrizations against
+the clean64 oracle (bit-exact integer counts)."""
+import importlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+tp = importlib.import_module("3pcf_amd")
+synth = importlib.import_module("3pcf_amd.synth")
+
+pytestmark = pytest.mark.gpu
+
+# The reference's compile-time spec (TRI:31-42) and the four-term config's
+# runtime spec (s in [2,20) 10 log bins, q in [1,3) 8 bins, theta/pi 16 bins).
+REF_SPEC = [(50, 2.0, 12.0, "lin"), (16, 0.9, 4.1, "lin"), (32, 0.0, 1.0, "lin")]
+CFG2_SPEC = [(10, 2.0, 20.0, "log"), (8, 1.0, 3.0, "lin"), (16, 0.0, 1.0, "lin")]
+JBJ_SPEC = [(20, 1.0, 30.0, "log"), (10, 0.0, 1.0, "lin"), (12, 0.0, 1.2, "lin")]
+
+
+def run(ctx, segs, terms, param, axes, mode=tp.MODE_PLAIN):
+    cats = [ctx.catalog(s) for s in segs]
+    job = tp.make_job([(c, 0, c.n) for c in cats], terms)
+    spec = tp.binspec(param, [tp.axis(*a) for a in axes])
+    return ctx.count(job, spec, mode)
+
+
+def oracle(segs, terms, param, axes):
+    return orc.clean_count(segs, terms, param, [orc.axis(*a) for a in axes])
+
+
+@pytest.mark.parametrize("axes", [REF_SPEC, CFG2_SPEC], ids=["ref", "cfg2"])
+def test_sqt_four_terms(ctx, axes):
+    print(f"case sqt four terms {axes[0]}", flush=True)
+    d = synth.neyman_scott(160, nclust=12, sigma=2.0, size=25.0, seed=21)
+    r = synth.uniform_cube(140, 25.0, seed=22)
+    got, st = run(ctx, [d, r], tp.FOUR_TERMS, tp.SQT, axes)
+    want = oracle([d, r], tp.FOUR_TERMS, orc.SQT, axes)
+    assert np.array_equal(got, want)
+    assert int(got.sum()) > 0
+
+
+def test_jbj(ctx):
+    print("case jbj", flush=True)
+    d = synth.uniform_cube(200, 20.0, seed=23)
+    got, _ = run(ctx, [d], [(0, 0, 0)], tp.JBJ, JBJ_SPEC)
+    assert np.array_equal(got, oracle([d], [(0, 0, 0)], orc.JBJ, JBJ_SPEC))
+    assert int(got.sum()) > 0
+
+
+def test_sqt_exact_geometry(ctx):
+    """Integer lattice: right angles, equilateral-free, collinear and
+    isoceles triangles with sides landing on q edges and theta edges."""
+    print("case sqt lattice", flush=True)
+    g = np.stack(np.meshgrid(np.arange(4), np.arange(4), np.arange(3), indexing="ij"), -1).reshape(-1, 3)
+    pts = g.astype(np.float64) * 2.0
+    axes = [(6, 1.0, 7.0, "lin"), (4, 1.0, 3.0, "lin"), (4, 0.0, 1.0, "lin")]
+    got, _ = run(ctx, [pts], [(0, 0, 0)], tp.SQT, axes)
+    assert np.array_equal(got, oracle([pts], [(0, 0, 0)], orc.SQT, axes))
+    # 3-4-5 right triangles: theta/pi = 0.5 exactly on an edge of 4 bins.
+    tri = np.array([[0.0, 0.0, 0.0], [3.0, 0.0, 0.0], [0.0, 4.0, 0.0]])
+    got, _ = run(ctx, [tri], [(0, 0, 0)], tp.SQT, axes)
+    assert np.array_equal(got, oracle([tri], [(0, 0, 0)], orc.SQT, axes))
+
+
+def test_sqt_duplicates_dropped(ctx):
+    print("case sqt duplicates", flush=True)
+    d = synth.uniform_cube(60, 8.0, seed=24)
+    d = np.vstack([d, d[:10]])
+    got, _ = run(ctx, [d], [(0, 0, 0)], tp.SQT, REF_SPEC)
+    assert np.array_equal(got, oracle([d], [(0, 0, 0)], orc.SQT, REF_SPEC))
+
+
+def test_sqt_fast_mode_refused(ctx):
+    d = synth.uniform_cube(20, 8.0, seed=25)
+    with pytest.raises(tp.TpcfError):
+        run(ctx, [d], [(0, 0, 0)], tp.SQT, REF_SPEC, mode=tp.MODE_FAST)
