@@ -51,6 +51,17 @@ def headline_spec(rel: float = 1e-5):
     return tp.r123_spec(20, 1.0, 350.0, "log", rel)
 
 
+def measured_traffic(args):
+    """HBM bytes per launch of the triplet kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by the copy-kernel
+    calibration), or None for other workloads."""
+    path = os.path.join(ROOT, "profiles", "r123_triplet_kernel_traffic.json")
+    if not (args.nd == 50000 and args.nr == 50000 and args.mode == "fast" and os.path.exists(path)):
+        return None
+    t = json.load(open(path))
+    return (t["fetch_size_kib"] * t["fetch_factor"] + t["write_size_kib"] * t["write_factor"]) * 1024.0
+
+
 def cpu_baseline(d, r, seconds_hint: float = 20.0):
     """Oracle (clean64, all host threads) on the first m data + m randoms."""
     from oracle import oracle as orc
@@ -80,9 +91,21 @@ def main(argv=None) -> int:
     ap.add_argument("--nd", type=int, default=50000)
     ap.add_argument("--nr", type=int, default=50000)
     ap.add_argument("--mode", choices=sorted(tp.MODES), default="fast")
-    ap.add_argument("--waves", type=int, default=0, help="triplet kernel waves per workgroup (0 = library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE "
+                         "counters against known bytes) and exit")
     args = ap.parse_args(argv)
+    if args.copy_probe_mib:
+        import torch
+
+        src = torch.ones(args.copy_probe_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": src.numel() * 4, "copy_probe_bytes_written": dst.numel() * 4}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               "(the path shards by splitting work items over devices, see DESIGN.md)", file=sys.stderr)
@@ -136,6 +159,9 @@ def main(argv=None) -> int:
             "pairs_rechecked_fraction": (stats[0].pairs_rechecked / stats[0].pairs) if stats[0].pairs else 0.0,
             "pair_stage_ms": stats[0].pair_ms,
             "triplet_kernel_ms": stats[0].triplet_ms,
+            # nominal LDS traffic of the pair-bin form: 8 B per triplet against
+            # 128 B/clk/CU x 256 CUs x 2.4 GHz
+            "lds_roofline_frac": (8.0 * triplets / tri_s) / (128 * 256 * 2.4e9) if tri_s > 0 else 0.0,
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -144,7 +170,7 @@ def main(argv=None) -> int:
             "peak": VALU_PEAK / 1e12,
             "unit": "TOP/s (VALU lane-ops, 6 per triplet)",
             "frac": achieved / VALU_PEAK,
-            "traffic": None,
+            "traffic": measured_traffic(args),
         },
     }
     if not args.no_cpu_baseline:
