@@ -106,7 +106,16 @@ _SIGNATURES = {
     "tpcf_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Stats)]),
     "tpcf_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Stats)]),
     "tpcf_job_footprint": (C.c_int64, [C.POINTER(Job), C.POINTER(BinSpec), C.c_int]),
+    "tpcf_batch_create": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(Job), C.POINTER(BinSpec), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)],
+    ),
+    "tpcf_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(Stats)]),
+    "tpcf_batch_info": (C.c_char_p, [C.c_void_p]),
+    "tpcf_batch_destroy": (C.c_int, [C.c_void_p]),
 }
+
+BATCH_AUTOTUNE, BATCH_GRAPH = 1, 2
 
 _lib = None
 
@@ -258,6 +267,10 @@ class Context:
         _check(self.lib.tpcf_count_batch(self.handle, JA, SA, n, mode, ptrs, C.byref(st)))
         return outs, st
 
+    def batch(self, jobs: Sequence[Job], specs: Sequence[BinSpec], mode: int = MODE_FAST,
+              autotune: bool = True, graph: bool = True) -> "Batch":
+        return Batch(self, jobs, specs, mode, (BATCH_AUTOTUNE if autotune else 0) | (BATCH_GRAPH if graph else 0))
+
     def footprint(self, job: Job, spec: BinSpec, mode: int = MODE_FAST) -> int:
         return _check(self.lib.tpcf_job_footprint(C.byref(job), C.byref(spec), mode))
 
@@ -277,6 +290,41 @@ class Context:
         st = Stats()
         _check(self.lib.tpcf_wait(self.handle, ticket, C.byref(st)))
         return st
+
+
+class Batch:
+    """A prepared batch (tpcf_batch): staged once, replayed by run()."""
+
+    def __init__(self, ctx: Context, jobs: Sequence[Job], specs: Sequence[BinSpec], mode: int, flags: int):
+        self.ctx = ctx
+        n = len(jobs)
+        self._jobs = (Job * n)(*jobs)
+        self._specs = (BinSpec * n)(*specs)
+        self.outs = [np.zeros((j.nterms, hist_cells(s)), dtype=np.uint64) for j, s in zip(jobs, specs)]
+        self._ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in self.outs])
+        self.handle = C.c_void_p()
+        _check(ctx.lib.tpcf_batch_create(ctx.handle, self._jobs, self._specs, n, mode, flags, C.byref(self.handle)))
+
+    def run(self):
+        """Recompute every histogram; returns (list of arrays, Stats)."""
+        st = Stats()
+        _check(self.ctx.lib.tpcf_batch_run(self.handle, self._ptrs, C.byref(st)))
+        return self.outs, st
+
+    @property
+    def info(self) -> str:
+        return self.ctx.lib.tpcf_batch_info(self.handle).decode()
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.tpcf_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def four_term_job(data: Catalog, randoms: Catalog) -> Job:

@@ -405,7 +405,9 @@ hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st) {
     return hipGetLastError();
 }
 
-#define TPCF_R123_SHAPES(X) X(4, 1) X(8, 1) X(16, 1) X(8, 2) X(16, 2) X(8, 4) X(16, 4) X(8, 8) X(16, 8)
+#define TPCF_R123_SHAPES(X)                                                                                    \
+    X(2, 1) X(4, 1) X(8, 1) X(16, 1) X(2, 2) X(4, 2) X(8, 2) X(16, 2) X(2, 4) X(4, 4) X(8, 4) X(16, 4) X(2, 8)   \
+        X(4, 8) X(8, 8) X(16, 8)
 
 const void *r123_triplet_kernel_ptr(int waves, int copies) {
 #define TPCF_PTR(W, K) \

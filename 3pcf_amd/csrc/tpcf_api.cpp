@@ -334,13 +334,17 @@ static TripletShape triplet_shape(tpcf_ctx *ctx, int max_n) {
     return s;
 }
 
-// Launch the triplet stage of several jobs as one persistent kernel.
-static int launch_triplets(tpcf_ctx *ctx, Workspace &ws, const std::vector<const JobPlan *> &plans,
-                           const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases,
-                           hipStream_t st) {
+// The triplet work queue of several jobs: one TripletTerm per (job, term)
+// and the items of every term, longest C ranges first within a term
+// (dynamic LPT order for the persistent kernel).
+struct Queue {
     std::vector<TripletTerm> terms;
     std::vector<TripletItem> items;
     int max_n = 1;
+};
+
+static void build_queue(const std::vector<const JobPlan *> &plans, const std::vector<const tpcf_job *> &jobs,
+                        const std::vector<char *> &bases, Queue &Q) {
     for (size_t j = 0; j < plans.size(); ++j) {
         const JobPlan &P = *plans[j];
         for (int t = 0; t < P.nterms; ++t) {
@@ -349,39 +353,67 @@ static int launch_triplets(tpcf_ctx *ctx, Workspace &ws, const std::vector<const
             T.nblk = P.L.nblk;
             T.hist = reinterpret_cast<unsigned long long *>(bases[j] + P.o_hist) + (size_t)t * P.cells;
             T.n = P.n;
-            const size_t before = items.size();
-            term_items(P.L, jobs[j]->term[t], (int)terms.size(), items);
-            // Longest C ranges first within a term (dynamic LPT order).
-            std::stable_sort(items.begin() + before, items.end(), [](const TripletItem &a, const TripletItem &b) {
+            const size_t before = Q.items.size();
+            term_items(P.L, jobs[j]->term[t], (int)Q.terms.size(), Q.items);
+            std::stable_sort(Q.items.begin() + before, Q.items.end(), [](const TripletItem &a, const TripletItem &b) {
                 return (a.c1 - a.c0) > (b.c1 - b.c0);
             });
-            terms.push_back(T);
-            max_n = std::max(max_n, P.n);
+            Q.terms.push_back(T);
+            Q.max_n = std::max(Q.max_n, P.n);
         }
     }
-    if (items.empty()) return TPCF_OK;
-    const bool wide = max_n > kMaxLdsR123Bins;
-    const size_t o_items = 0;
-    const size_t o_terms = align256(sizeof(TripletItem) * items.size());
-    const size_t o_ctr = align256(o_terms + sizeof(TripletTerm) * terms.size());
-    if (int e = ws.queue.reserve(o_ctr + 256)) return e;
-    char *q = reinterpret_cast<char *>(ws.queue.p);
-    HIP_TRY(hipMemcpyAsync(q + o_items, items.data(), sizeof(TripletItem) * items.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(q + o_terms, terms.data(), sizeof(TripletTerm) * terms.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(q + o_ctr), 1, 0u, st));
-    HIP_TRY(hipEventRecord(ws.ev[3], st));
-    if (wide) {
-        // A batch mixing wide and narrow specs runs everything on the wide
-        // kernel, whose pair values are unscaled.
-        HIP_TRY(launch_r123_triplets_wide(reinterpret_cast<const TripletItem *>(q + o_items), (int)items.size(),
-                                          reinterpret_cast<const TripletTerm *>(q + o_terms), ctx->num_cu * 8, st));
+}
+
+// Device copy of a queue.
+struct QueueDev {
+    const TripletItem *items = nullptr;
+    const TripletTerm *terms = nullptr;
+    int *counter = nullptr;
+    int nitems = 0;
+    int max_n = 1;
+};
+
+static int upload_queue(DevBuf &buf, const Queue &Q, hipStream_t st, QueueDev &D) {
+    D.nitems = (int)Q.items.size();
+    D.max_n = Q.max_n;
+    if (Q.items.empty()) return TPCF_OK;
+    const size_t o_terms = align256(sizeof(TripletItem) * Q.items.size());
+    const size_t o_ctr = align256(o_terms + sizeof(TripletTerm) * Q.terms.size());
+    if (int e = buf.reserve(o_ctr + 256)) return e;
+    char *q = reinterpret_cast<char *>(buf.p);
+    HIP_TRY(hipMemcpyAsync(q, Q.items.data(), sizeof(TripletItem) * Q.items.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(q + o_terms, Q.terms.data(), sizeof(TripletTerm) * Q.terms.size(), hipMemcpyHostToDevice,
+                           st));
+    D.items = reinterpret_cast<const TripletItem *>(q);
+    D.terms = reinterpret_cast<const TripletTerm *>(q + o_terms);
+    D.counter = reinterpret_cast<int *>(q + o_ctr);
+    return TPCF_OK;
+}
+
+// Reset the queue counter and launch the triplet kernel (kernels only, so
+// the sequence can be captured in a graph).
+static int launch_queue(tpcf_ctx *ctx, const QueueDev &D, const TripletShape &sh, hipStream_t st) {
+    if (!D.nitems) return TPCF_OK;
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(D.counter), 1, 0u, st));
+    if (D.max_n > kMaxLdsR123Bins) {
+        // Wide specs: pair values are unscaled; global-atomic kernel.
+        HIP_TRY(launch_r123_triplets_wide(D.items, D.nitems, D.terms, ctx->num_cu * 8, st));
         return TPCF_OK;
     }
-    const TripletShape sh = triplet_shape(ctx, max_n);
-    HIP_TRY(launch_r123_triplets(reinterpret_cast<const TripletItem *>(q + o_items), (int)items.size(),
-                                 reinterpret_cast<int *>(q + o_ctr), reinterpret_cast<const TripletTerm *>(q + o_terms),
-                                 max_n, sh.waves, sh.copies, sh.grid, st));
+    HIP_TRY(launch_r123_triplets(D.items, D.nitems, D.counter, D.terms, D.max_n, sh.waves, sh.copies, sh.grid, st));
     return TPCF_OK;
+}
+
+// Launch the triplet stage of several jobs as one persistent kernel.
+static int launch_triplets(tpcf_ctx *ctx, Workspace &ws, const std::vector<const JobPlan *> &plans,
+                           const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases,
+                           hipStream_t st) {
+    Queue Q;
+    build_queue(plans, jobs, bases, Q);
+    QueueDev D;
+    if (int e = upload_queue(ws.queue, Q, st, D)) return e;
+    HIP_TRY(hipEventRecord(ws.ev[3], st));
+    return launch_queue(ctx, D, triplet_shape(ctx, Q.max_n), st);
 }
 
 static int launch_plain(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tpcf_job &job, char *base,
@@ -610,7 +642,7 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) c->budget = (uint64_t)(0.9 * (double)fr);
     c->ws.reset(new Workspace());
     // r123 triplet kernels raise their dynamic LDS limit once.
-    for (int w : {4, 8, 16})
+    for (int w : {2, 4, 8, 16})
         for (int k : {1, 2, 4, 8})
             if (const void *fn = r123_triplet_kernel_ptr(w, k))
                 (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
@@ -727,5 +759,222 @@ int tpcf_count_batch(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *sp
 }
 
 void tpcf_free(void *p) { free(p); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// prepared batches: stage once, autotune the launch shape, capture the
+// counting sequence (histogram reset, pair stage, persistent triplet kernel)
+// in a HIP graph and replay it
+// ---------------------------------------------------------------------------
+
+struct tpcf_batch {
+    tpcf_ctx *ctx = nullptr;
+    int mode = TPCF_MODE_FAST;
+    int njobs = 0;
+    std::vector<tpcf_job> jobs;
+    std::vector<JobPlan> plans;
+    std::vector<size_t> off;
+    DevBuf region, queue;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    QueueDev qd;
+    int shift = 2;
+    TripletShape shape{8, 1, 0};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string tuning;  // autotuner log
+    ~tpcf_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace tpcf {
+
+// The counting sequence of a prepared batch: kernels only.
+static int batch_compute(tpcf_batch *b, const TripletShape &sh, hipStream_t st) {
+    char *base = reinterpret_cast<char *>(b->region.p);
+    for (int j = 0; j < b->njobs; ++j) {
+        const JobPlan &P = b->plans[j];
+        char *jb = base + b->off[j];
+        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(jb + P.o_cnt), 16, 0u, st));
+        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(jb + P.o_hist), 2 * P.cells * P.nterms, 0u, st));
+    }
+    for (int j = 0; j < b->njobs; ++j)
+        if (b->plans[j].L.nblk)
+            HIP_TRY(launch_pair_tiles(pair_job(b->plans[j], base + b->off[j], b->shift), b->mode == TPCF_MODE_FAST,
+                                      st));
+    return launch_queue(b->ctx, b->qd, sh, st);
+}
+
+static int batch_time(tpcf_batch *b, const TripletShape &sh, float &ms) {
+    HIP_TRY(hipEventRecord(b->ev[0], b->st));
+    if (int e = batch_compute(b, sh, b->st)) return e;
+    HIP_TRY(hipEventRecord(b->ev[1], b->st));
+    HIP_TRY(hipEventSynchronize(b->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    return TPCF_OK;
+}
+
+// Candidate launch shapes: 2..16 waves per workgroup (128..1024 threads),
+// the largest LDS histogram copy counts that fit, and the occupancy limit or
+// half of it as workgroups per CU.
+static int batch_autotune(tpcf_batch *b) {
+    constexpr int kLdsBudget = 155 * 1024;
+    const int n = b->qd.max_n;
+    if (n > kMaxLdsR123Bins || !b->qd.nitems) return TPCF_OK;
+    std::vector<TripletShape> cands;
+    int kmax = 1;
+    for (int k : {8, 4, 2, 1})
+        if (r123_lds_bytes(n, k) <= kLdsBudget) {
+            kmax = k;
+            break;
+        }
+    for (int k : {kmax, kmax / 2})
+        for (int w : {2, 4, 8, 16}) {
+            if (k < 1 || !r123_triplet_kernel_ptr(w, k)) continue;
+            int occ = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, r123_triplet_kernel_ptr(w, k), w * 64,
+                                                             r123_lds_bytes(n, k)) != hipSuccess || occ < 1)
+                continue;
+            cands.push_back({w, k, b->ctx->num_cu * occ});
+            if (occ > 1) cands.push_back({w, k, b->ctx->num_cu * std::max(1, occ / 2)});
+        }
+    float best = 1e30f;
+    char line[128];
+    for (const auto &c : cands) {
+        float ms = 0.f;
+        if (int e = batch_time(b, c, ms)) return e;
+        std::snprintf(line, sizeof line, "waves=%d copies=%d wg=%d: %.3f ms\n", c.waves, c.copies, c.grid, ms);
+        b->tuning += line;
+        if (ms < best) {
+            best = ms;
+            b->shape = c;
+        }
+    }
+    return TPCF_OK;
+}
+
+}  // namespace tpcf
+
+extern "C" {
+
+int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode, int flags,
+                      tpcf_batch **out) {
+    clear_error();
+    if (!ctx || !jobs || !specs || njobs < 1 || !out) return set_error(TPCF_EINVAL, "bad batch arguments");
+    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_FAST64)
+        return set_error(TPCF_EINVAL, "batches run the FAST paths only");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<tpcf_batch> b(new tpcf_batch());
+    b->ctx = ctx;
+    b->mode = mode;
+    b->njobs = njobs;
+    b->jobs.assign(jobs, jobs + njobs);
+    b->plans.resize(njobs);
+    b->off.resize(njobs);
+    size_t bytes = 0;
+    int max_n = 1;
+    for (int j = 0; j < njobs; ++j) {
+        if (int e = plan_job(jobs[j], specs[j], mode, b->plans[j])) return e;
+        b->off[j] = bytes;
+        bytes = align256(bytes + b->plans[j].bytes);
+        max_n = std::max(max_n, b->plans[j].n);
+    }
+    b->shift = max_n <= kMaxLdsR123Bins ? 2 : 0;
+    HIP_TRY(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+    for (auto &e : b->ev) HIP_TRY(hipEventCreate(&e));
+    if (int e = b->region.reserve(std::max<size_t>(bytes, 256))) return e;
+    char *base = reinterpret_cast<char *>(b->region.p);
+    std::vector<const JobPlan *> pp;
+    std::vector<const tpcf_job *> jj;
+    std::vector<char *> bb;
+    for (int j = 0; j < njobs; ++j) {
+        if (int e = stage_job(b->jobs[j], b->plans[j], base + b->off[j], b->st)) return e;
+        if (!b->plans[j].L.nblk) continue;
+        pp.push_back(&b->plans[j]);
+        jj.push_back(&b->jobs[j]);
+        bb.push_back(base + b->off[j]);
+    }
+    Queue Q;
+    build_queue(pp, jj, bb, Q);
+    if (int e = upload_queue(b->queue, Q, b->st, b->qd)) return e;
+    HIP_TRY(hipStreamSynchronize(b->st));
+    b->shape = triplet_shape(ctx, b->qd.max_n);
+    if (flags & TPCF_BATCH_AUTOTUNE)
+        if (int e = batch_autotune(b.get())) return e;
+    if (flags & TPCF_BATCH_GRAPH) {
+        HIP_TRY(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
+        const int rc = batch_compute(b.get(), b->shape, b->st);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(b->st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ce != hipSuccess) return set_error(TPCF_EDEVICE, "graph capture failed: %s", hipGetErrorString(ce));
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    *out = b.release();
+    return TPCF_OK;
+}
+
+int tpcf_batch_run(tpcf_batch *b, uint64_t *const *hist, tpcf_stats *stats) {
+    clear_error();
+    if (!b || !hist) return set_error(TPCF_EINVAL, "NULL argument");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipEventRecord(b->ev[0], b->st));
+    if (b->exec) HIP_TRY(hipGraphLaunch(b->exec, b->st));
+    else if (int e = batch_compute(b, b->shape, b->st)) return e;
+    HIP_TRY(hipEventRecord(b->ev[1], b->st));
+    char *base = reinterpret_cast<char *>(b->region.p);
+    std::vector<uint64_t> rech(b->njobs, 0);
+    for (int j = 0; j < b->njobs; ++j) {
+        const JobPlan &P = b->plans[j];
+        HIP_TRY(hipMemcpyAsync(hist[j], base + b->off[j] + P.o_hist, sizeof(uint64_t) * P.cells * P.nterms,
+                               hipMemcpyDeviceToHost, b->st));
+        HIP_TRY(hipMemcpyAsync(&rech[j], base + b->off[j] + P.o_cnt, 8, hipMemcpyDeviceToHost, b->st));
+    }
+    HIP_TRY(hipStreamSynchronize(b->st));
+    if (stats) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+        tpcf_stats s{};
+        for (int j = 0; j < b->njobs; ++j) {
+            s.triplets += b->plans[j].triplets;
+            int64_t nu = 0;
+            for (int q = 0; q < b->plans[j].L.nseg; ++q) nu += b->plans[j].L.seg_n[q];
+            s.pairs += (uint64_t)(nu * (nu - 1) / 2);
+            s.pairs_rechecked += b->mode == TPCF_MODE_FAST ? rech[j] : 0;
+        }
+        s.device_ms = ms;
+        s.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        *stats = s;
+    }
+    return TPCF_OK;
+}
+
+const char *tpcf_batch_info(const tpcf_batch *b) {
+    if (!b) return "";
+    static thread_local std::string info;
+    char head[160];
+    std::snprintf(head, sizeof head, "shape waves=%d copies=%d wg=%d graph=%d items=%d\n", b->shape.waves,
+                  b->shape.copies, b->shape.grid, b->exec ? 1 : 0, b->qd.nitems);
+    info = head + b->tuning;
+    return info.c_str();
+}
+
+int tpcf_batch_destroy(tpcf_batch *b) {
+    if (!b) return TPCF_OK;
+    (void)hipSetDevice(b->ctx->device);
+    delete b;
+    return TPCF_OK;
+}
 
 }  // extern "C"

@@ -171,6 +171,23 @@ int tpcf_count(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int
 int tpcf_count_batch(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs,
                      int mode, uint64_t *const *hist, tpcf_stats *stats);
 
+/* Prepared batch: the jobs are planned and staged on the device once; each
+ * tpcf_batch_run() recomputes every histogram (histogram reset, pair stage,
+ * one persistent triplet launch) -- replayed from a HIP graph with
+ * TPCF_BATCH_GRAPH, with the triplet kernel's launch shape (waves per
+ * workgroup, LDS histogram copies, workgroups per CU) picked by timing the
+ * candidates on the batch itself with TPCF_BATCH_AUTOTUNE.  The catalogs must
+ * stay alive until tpcf_batch_destroy().  No reference counterpart (the
+ * reference runs one process per voxel and term). */
+typedef struct tpcf_batch tpcf_batch;
+enum { TPCF_BATCH_AUTOTUNE = 1, TPCF_BATCH_GRAPH = 2 };
+int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode,
+                      int flags, tpcf_batch **out);
+int tpcf_batch_run(tpcf_batch *batch, uint64_t *const *hist, tpcf_stats *stats);
+/* Chosen launch shape and the autotuner's timings, as text. */
+const char *tpcf_batch_info(const tpcf_batch *batch);
+int tpcf_batch_destroy(tpcf_batch *batch);
+
 /* Serving: enqueue a job (admitted when its device workspace fits the
  * memory budget; otherwise it waits in the queue), returns a ticket.  hist
  * must stay valid until the ticket completes.  No reference counterpart

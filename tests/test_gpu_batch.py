@@ -1,0 +1,102 @@
+"""GPU: batching (one persistent launch over many jobs, prepared batches
+replayed from a HIP graph, autotuned shapes) and serving (tpcf_submit:
+admission by device memory, results identical alone and in the mix)."""
+import importlib
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+tp = importlib.import_module("3pcf_amd")
+synth = importlib.import_module("3pcf_amd.synth")
+
+pytestmark = pytest.mark.gpu
+
+
+def voxel_jobs(ctx, nvox=8, seed=300):
+    jobs, specs, wants, keep = [], [], [], []
+    sizes = synth.lognormal_voxel_sizes(nvox, 60 * nvox, lo=20, hi=200, seed=seed)
+    for v in range(nvox):
+        d = synth.wedge(int(sizes[v]), 2.0, 2.0, 500.0, 520.0, seed=seed + v, clustered=True, nclust=5)
+        r = synth.wedge(int(sizes[v]), 2.0, 2.0, 500.0, 520.0, seed=seed + 100 + v)
+        cd, cr = ctx.catalog(d), ctx.catalog(r)
+        keep += [cd, cr]
+        jobs.append(tp.four_term_job(cd, cr))
+        specs.append(tp.r123_spec(12, 1.0, 30.0, "log"))
+        wants.append(orc.clean_count([d, r], tp.FOUR_TERMS, orc.R123, [orc.axis(12, 1.0, 30.0, "log")]))
+    return jobs, specs, wants, keep
+
+
+def test_prepared_batch_graph_replays(ctx):
+    print("case prepared batch graph", flush=True)
+    jobs, specs, wants, keep = voxel_jobs(ctx)
+    b = ctx.batch(jobs, specs, autotune=True, graph=True)
+    assert "graph=1" in b.info
+    for rep in range(3):
+        outs, st = b.run()
+        for o, w in zip(outs, wants):
+            assert np.array_equal(o, w), rep
+        # eager work between replays
+        ctx.count(jobs[0], specs[0])
+    b.close()
+
+
+def test_prepared_batch_without_graph(ctx):
+    print("case prepared batch eager", flush=True)
+    jobs, specs, wants, keep = voxel_jobs(ctx, nvox=4, seed=400)
+    b = ctx.batch(jobs, specs, autotune=False, graph=False)
+    outs, _ = b.run()
+    for o, w in zip(outs, wants):
+        assert np.array_equal(o, w)
+    b.close()
+
+
+def test_serving_alone_equals_mix(ctx):
+    """Each job's histogram is the same when it runs alone and when it is
+    admitted into a mix of concurrently arriving jobs of other sizes and
+    bin specs (tolerance: exact)."""
+    print("case serving mix", flush=True)
+    rng = np.random.Generator(np.random.PCG64(7))
+    jobs, specs, keep = [], [], []
+    for k in range(10):
+        n = int(rng.integers(40, 300))
+        d = synth.uniform_cube(n, 30.0, seed=500 + k)
+        r = synth.uniform_cube(n, 30.0, seed=600 + k)
+        cd, cr = ctx.catalog(d), ctx.catalog(r)
+        keep += [cd, cr]
+        nb = int(rng.integers(8, 33))
+        spec = tp.r123_spec(nb, 0.5, 52.0, "log" if k % 2 else "lin")
+        terms = [t for t in tp.FOUR_TERMS if rng.random() < 0.7] or [tp.FOUR_TERMS[0]]
+        jobs.append(tp.make_job([(cd, 0, n), (cr, 0, n)], terms))
+        specs.append(spec)
+    alone = [ctx.count(j, s)[0] for j, s in zip(jobs, specs)]
+    tickets = []
+    for j, s in zip(jobs, specs):
+        tickets.append(ctx.submit(j, s))
+        time.sleep(0.001)
+    for (t, out), a in zip(tickets, alone):
+        st = ctx.wait(t)
+        assert np.array_equal(out, a)
+        assert st.wall_ms > 0
+
+
+def test_serving_admission_rejects_oversized(ctx):
+    print("case serving admission", flush=True)
+    d = synth.uniform_cube(2000, 30.0, seed=700)
+    cd = ctx.catalog(d)
+    job = tp.make_job([(cd, 0, cd.n)], [(0, 0, 0)])
+    spec = tp.r123_spec(10, 0.0, 60.0)
+    fp = ctx.footprint(job, spec)
+    ctx.set_memory_budget(fp // 2)
+    try:
+        t, out = ctx.submit(job, spec)
+        with pytest.raises(tp.TpcfError) as e:
+            ctx.wait(t)
+        assert e.value.code == -5
+    finally:
+        ctx.set_memory_budget(1 << 40)
+    t, out = ctx.submit(job, spec)
+    ctx.wait(t)
+    assert int(out.sum()) > 0
