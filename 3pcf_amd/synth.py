@@ -57,6 +57,34 @@ def wedge(n: int, ra_deg: float = 10.0, dec_deg: float = 10.0, r_lo: float = 500
     return centres[pick] + g.normal(0.0, sigma, (n, 3))
 
 
+def angular_voxels(nside: int = 8, total: int = 150000, ra_deg: float = 20.0, dec_deg: float = 20.0,
+                   r_lo: float = 500.0, r_hi: float = 800.0, lo: int = 500, hi: int = 8000, seed: int = 0,
+                   nclust_per_voxel: int = 8, sigma: float = 2.0):
+    """Survey split into nside x nside RA/Dec cells of a wedge: per cell a
+    clustered data catalog and a uniform random catalog of the same uneven
+    (lognormal, clipped to [lo, hi], summing to total) size.  Returns a list
+    of (data, randoms) (n, 3) arrays."""
+    sizes = lognormal_voxel_sizes(nside * nside, total, lo, hi, seed=seed)
+    out = []
+    for v, n in enumerate(sizes):
+        a, b = divmod(v, nside)
+        g = _rng(seed * 1000 + v + 1)
+        ra0, dec0 = ra_deg * a / nside, dec_deg * b / nside
+        dra, ddec = ra_deg / nside, dec_deg / nside
+
+        def draw(m):
+            ra = np.deg2rad(ra0 + g.random(m) * dra)
+            s0, s1 = np.sin(np.deg2rad(dec0)), np.sin(np.deg2rad(dec0 + ddec))
+            dec = np.arcsin(s0 + (s1 - s0) * g.random(m))
+            r = r_lo + (r_hi - r_lo) * g.random(m)
+            return np.stack([r * np.cos(dec) * np.cos(ra), r * np.cos(dec) * np.sin(ra), r * np.sin(dec)], axis=1)
+
+        centres = draw(nclust_per_voxel)
+        data = centres[g.integers(0, nclust_per_voxel, int(n))] + g.normal(0.0, sigma, (int(n), 3))
+        out.append((data, draw(int(n))))
+    return out
+
+
 def lognormal_voxel_sizes(nvox: int, total: int, lo: int = 500, hi: int = 8000, seed: int = 0) -> np.ndarray:
     """Uneven voxel occupancies: lognormal, clipped to [lo, hi], rescaled to
     sum to total (clipping again after rescaling, remainder spread)."""

@@ -83,6 +83,189 @@ def cpu_baseline(d, r, seconds_hint: float = 20.0):
     }
 
 
+def pct(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return 0.0
+    k = min(len(xs) - 1, max(0, int(math.ceil(q / 100.0 * len(xs))) - 1))
+    return xs[k]
+
+
+def bench_batch(args) -> int:
+    """64 angular voxels (8x8 cells of a 20x20 deg wedge, r in [500,800]
+    Mpc/h) of 150k clustered data + 150k randoms with lognormal occupancy
+    500..8000; all four terms of every voxel, 20 log bins on [1,50) Mpc/h;
+    FAST path as one prepared batch (autotuned shape, HIP graph) against the
+    FP64 plain kernel running the voxel jobs one at a time."""
+    vox = synth.angular_voxels(8, 150000, seed=0)
+    spec = tp.r123_spec(20, 1.0, 50.0, "log", 1e-5)
+    reps = max(1, args.steps)
+    with tp.Context(0) as ctx:
+        cats = [(ctx.catalog(d), ctx.catalog(r)) for d, r in vox]
+        jobs = [tp.four_term_job(cd, cr) for cd, cr in cats]
+        t0 = time.perf_counter()
+        b = ctx.batch(jobs, [spec] * len(jobs), tp.MODE_FAST, autotune=True, graph=True)
+        prep_s = time.perf_counter() - t0
+        for _ in range(args.warmup):
+            b.run()
+        lat = []
+        tri_ms = []
+        for _ in range(reps):
+            t1 = time.perf_counter()
+            outs, st = b.run()
+            lat.append(time.perf_counter() - t1)
+            tri_ms.append(st.device_ms)
+        fast = [o.copy() for o in outs]
+        info = b.info
+        b.close()
+        # plain FP64, one voxel job per launch
+        plain_lat = []
+        exact = True
+        for j, job in enumerate(jobs):
+            t1 = time.perf_counter()
+            out, _ = ctx.count(job, spec, tp.MODE_PLAIN)
+            plain_lat.append(time.perf_counter() - t1)
+            exact &= bool(np.array_equal(out, fast[j]))
+    nv = len(jobs)
+    mean = sum(lat) / len(lat)
+    fast_rate = nv / mean
+    plain_rate = nv / sum(plain_lat)
+    triplets = st.triplets
+    dev_s = sum(tri_ms) / len(tri_ms) / 1e3
+    summed = np.sum(np.stack(fast), axis=0)
+    line = {
+        "metric": "batched voxel jobs/s (64 voxels x 4 terms) vs plain FP64 one job at a time, exact",
+        "value": fast_rate,
+        "unit": "voxel-jobs/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": args.warmup,
+        "ms_per_step": mean * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8 pair bins, u32/u64 counts (pairs fp32 + fp64 recheck)",
+        "data": "synthetic",
+        "config": {
+            "workload": "64 angular voxels (8x8 of a 20x20 deg wedge, r 500-800 Mpc/h), 150k clustered data + "
+                        "150k randoms, lognormal occupancy 501-7998 per voxel, four terms per voxel, 20 log bins "
+                        "on [1,50) Mpc/h",
+            "triplets_per_step": triplets,
+            "triplets_in_range": int(summed.sum()),
+            "batch_latency_p50_ms": pct(lat, 50) * 1e3,
+            "batch_latency_p99_ms": pct(lat, 99) * 1e3,
+            "prepare_s": prep_s,
+            "launch": info.splitlines()[0] if info else "",
+            "plain_fp64_voxel_jobs_per_s": plain_rate,
+            "plain_job_latency_p50_ms": pct(plain_lat, 50) * 1e3,
+            "plain_job_latency_p99_ms": pct(plain_lat, 99) * 1e3,
+            "speedup_vs_plain": fast_rate / plain_rate,
+            "histograms_identical_to_plain": exact,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": OPS_PER_TRIPLET * triplets / dev_s / 1e12,
+            "peak": VALU_PEAK / 1e12,
+            "unit": "TOP/s (VALU lane-ops, 6 per triplet)",
+            "frac": OPS_PER_TRIPLET * triplets / dev_s / VALU_PEAK,
+            "traffic": None,
+        },
+    }
+    print(json.dumps(line))
+    sys.stderr.write(info)
+    return 0
+
+
+def request_stream(n: int, n_max: int, seed: int = 0):
+    """Poisson arrivals (mean 20 ms apart); catalog size log-uniform in
+    [1000, n_max]; a uniform random non-empty subset of the four terms; a bin
+    spec from {r123-lin, r123-log} with 8..32 bins."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    reqs = []
+    for k in range(n):
+        t += g.exponential(0.020)
+        size = int(round(math.exp(g.uniform(math.log(1000), math.log(n_max)))))
+        mask = int(g.integers(1, 16))
+        terms = [tp.FOUR_TERMS[b] for b in range(4) if mask >> b & 1]
+        nb = int(g.integers(8, 33))
+        scale = "log" if g.random() < 0.5 else "lin"
+        spec = tp.r123_spec(nb, 1.0, 150.0, scale) if scale == "log" else tp.r123_spec(nb, 0.0, 150.0, "lin")
+        reqs.append((t, size, terms, spec, 1000 + k))
+    return reqs
+
+
+def bench_serve(args) -> int:
+    reqs = request_stream(args.requests, args.n_max)
+    with tp.Context(0) as ctx:
+        jobs = []
+        keep = []
+        for (_, size, terms, spec, seed) in reqs:
+            d = synth.neyman_scott(size, nclust=40, sigma=2.0, size=100.0, seed=seed)
+            r = synth.uniform_cube(size, 100.0, seed=seed + 7)
+            cd, cr = ctx.catalog(d), ctx.catalog(r)
+            keep += [cd, cr]
+            jobs.append(tp.make_job([(cd, 0, size), (cr, 0, size)], terms))
+        # warm the workers and kernels
+        t, _ = ctx.submit(jobs[0], reqs[0][3])
+        ctx.wait(t)
+        start = time.perf_counter()
+        tickets = []
+        for (arr, _, _, spec, _), job in zip(reqs, jobs):
+            now = time.perf_counter() - start
+            if arr > now:
+                time.sleep(arr - now)
+            tickets.append(ctx.submit(job, spec))
+        lat = []
+        outs = []
+        for tk, out in tickets:
+            st = ctx.wait(tk)
+            lat.append(st.wall_ms / 1e3)
+            outs.append(out)
+        span = time.perf_counter() - start
+        # plain FP64, one job at a time in arrival order (single server queue)
+        clock = 0.0
+        plain_lat = []
+        exact = True
+        tot_trip = 0
+        for (arr, _, _, spec, _), job, o in zip(reqs, jobs, outs):
+            t1 = time.perf_counter()
+            out, st = ctx.count(job, spec, tp.MODE_PLAIN)
+            dt = time.perf_counter() - t1
+            tot_trip += st.triplets
+            clock = max(clock, arr) + dt
+            plain_lat.append(clock - arr)
+            exact &= bool(np.array_equal(out, o))
+    n = len(reqs)
+    line = {
+        "metric": "serving throughput (jobs/s) with p50/p99 latency vs plain FP64 one job at a time, exact",
+        "value": n / span,
+        "unit": "jobs/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": span * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8 pair bins, u32/u64 counts (pairs fp32 + fp64 recheck)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{n} requests, Poisson arrivals (mean 50/s), N log-uniform in [1000,{args.n_max}] data + "
+                        "as many randoms, random non-empty term subset, r123 lin/log spec with 8-32 bins",
+            "triplets": tot_trip,
+            "latency_p50_ms": pct(lat, 50) * 1e3,
+            "latency_p99_ms": pct(lat, 99) * 1e3,
+            "plain_fp64_jobs_per_s": n / max(clock, 1e-9),
+            "plain_latency_p50_ms": pct(plain_lat, 50) * 1e3,
+            "plain_latency_p99_ms": pct(plain_lat, 99) * 1e3,
+            "histograms_identical_to_plain": exact,
+        },
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +275,12 @@ def main(argv=None) -> int:
     ap.add_argument("--nr", type=int, default=50000)
     ap.add_argument("--mode", choices=sorted(tp.MODES), default="fast")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["headline", "batch", "serve"], default="headline",
+                    help="headline: 50k+50k four terms (default); batch: 64 angular voxels of 150k+150k in one "
+                         "prepared, graph-replayed persistent launch vs one FP64 plain job at a time; serve: a "
+                         "Poisson request stream through the serving queue vs plain one-at-a-time")
+    ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
+    ap.add_argument("--n-max", type=int, default=8000, help="serve: largest catalog size of a request")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE "
                          "counters against known bytes) and exit")
@@ -108,8 +297,13 @@ def main(argv=None) -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(the path shards by splitting work items over devices, see DESIGN.md)", file=sys.stderr)
+              "(the work queue would be split over devices with one histogram reduction at the end; "
+              "only the single-device path is built and measured)", file=sys.stderr)
         return 2
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     d, r = headline_catalogs(args.nd, args.nr)
     spec = headline_spec()
