@@ -17,6 +17,7 @@
 #define _GNU_SOURCE
 #include <getopt.h>
 #include <limits.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -253,7 +254,7 @@ int main(int argc, char **argv)
         nb[a] = A->n;
         lo[a] = A->lo;
         hi[a] = A->hi;
-        w[a] = (A->hi - A->lo) / A->n;
+        w[a] = A->scale == ORC_LOG ? log(A->hi / A->lo) / A->n : (A->hi - A->lo) / A->n;
     }
     return orc_write_tri(out, ng, nb, lo, hi, w, h) ? 1 : 0;
 }

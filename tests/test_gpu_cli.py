@@ -1,0 +1,98 @@
+"""GPU: the drop-in programs write, byte for byte, the histogram file the
+oracle's command line writes for the same argv in clean64 semantics (file
+identity terms, voxel chunks, default and runtime bin specs), and the voxel
+drivers + reducer reproduce the full run."""
+import importlib
+import os
+import subprocess
+
+import pytest
+
+from oracle import oracle as orc
+
+tp = importlib.import_module("3pcf_amd")
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+INP = os.path.join(HERE, "golden", "inputs")
+DRV = os.path.join(os.path.dirname(HERE), "3pcf_amd", "drivers")
+
+
+def prog(name):
+    return os.path.join(tp.BIN_DIR, name)
+
+
+def run(cmd, cwd=INP):
+    print("run", " ".join(os.path.basename(c) if os.path.isabs(c) else c for c in cmd), flush=True)
+    subprocess.run(cmd, cwd=cwd, check=True, stdout=subprocess.DEVNULL)
+
+
+ORDERS = {"DDD": ["d200.dat"] * 3, "DDR": ["d200.dat", "d200.dat", "r180.dat"],
+          "DRR": ["d200.dat", "r180.dat", "r180.dat"], "DRD": ["d200.dat", "r180.dat", "d200.dat"]}
+
+
+@pytest.mark.parametrize("order", sorted(ORDERS))
+def test_triangle_program_default_spec(order, tmp_path):
+    orc.build()
+    a, b = tmp_path / "a.dat", tmp_path / "b.dat"
+    run([prog("3pcf_C_version_triangle_parametrization"), *ORDERS[order], "-o", str(a), "--quiet"])
+    run([orc.CLI, "clean-sqt", "--threads", "8", *ORDERS[order], "-o", str(b)])
+    assert a.read_bytes() == b.read_bytes()
+
+
+def test_triangle_program_voxel_and_runtime_spec(tmp_path):
+    orc.build()
+    spec = ["--axis", "0,10,2,20,log", "--axis", "1,8,1,3", "--axis", "2,16,0,1"]
+    for vox in ("001000001", "000001001", "001001000"):
+        a, b = tmp_path / "a.dat", tmp_path / "b.dat"
+        run([prog("3pcf_C_version_triangle_parametrization"), *ORDERS["DDR"], *spec, "-X", "2", "-x", vox,
+             "-o", str(a), "--quiet"])
+        run([orc.CLI, "clean-sqt", "--threads", "8", *ORDERS["DDR"], *spec, "-X", "2", "-x", vox, "-o", str(b)])
+        assert a.read_bytes() == b.read_bytes(), vox
+
+
+def test_jbj_program(tmp_path):
+    orc.build()
+    a, b = tmp_path / "a.dat", tmp_path / "b.dat"
+    run([prog("3pcf_C_version_triangle_parametrization_JBJ"), *ORDERS["DRR"], "-o", str(a), "--quiet"])
+    run([orc.CLI, "clean-jbj", "--threads", "8", *ORDERS["DRR"], "-o", str(b)])
+    assert a.read_bytes() == b.read_bytes()
+
+
+@pytest.mark.parametrize("mode", ["fast", "plain", "fast64"])
+def test_plain_program_r123(mode, tmp_path):
+    orc.build()
+    a, b = tmp_path / "a.dat", tmp_path / "b.dat"
+    files = ["a60.cat", "a60.cat", "b50.cat"]
+    run([prog("3pcf_C_version"), *files, "--format", "tri", "--mode", mode, "-o", str(a), "--quiet"])
+    run([orc.CLI, "clean-r123", "--count-format", "--threads", "8", "--axis", "0,64,0,41.569219381653056",
+         *files, "-o", str(b)])
+    assert a.read_bytes() == b.read_bytes()
+    a2 = tmp_path / "a2.dat"
+    run([prog("3pcf_C_version"), *files, "--bins", "16", "--lo", "1", "--hi", "2", "--log", "--mode", mode,
+         "--format", "tri", "-o", str(a2), "--quiet"])
+    b2 = tmp_path / "b2.dat"
+    run([orc.CLI, "clean-r123", "--count-format", "--threads", "8", "--axis", "0,16,1,2,log", *files, "-o", str(b2)])
+    assert a2.read_bytes() == b2.read_bytes()
+
+
+def test_voxel_driver_plus_reducer_equals_full_run(tmp_path):
+    """run_CPU_calculation_each_voxels.sh over every chunk triple, summed with
+    sum_voxels.py, equals the whole-catalog run (the clean voxel split
+    partitions the triplets)."""
+    env = dict(os.environ, INPUT0=os.path.join(INP, "d200.dat"), INPUT1=os.path.join(INP, "r180.dat"),
+               TAG="t", GLOBAL_PARAMS="--quiet --axis 0,10,1,12 --axis 1,8,1,3 --axis 2,8,0,1")
+    files = []
+    for a in range(2):
+        for b in range(2):
+            for c in range(2):
+                idx = f"{a:03d}{b:03d}{c:03d}"
+                subprocess.run(["bash", os.path.join(DRV, "run_CPU_calculation_each_voxels.sh"), "1", "1", "2", idx],
+                               cwd=tmp_path, env=env, check=True, capture_output=True)
+                files.append(str(tmp_path / f"DDR_voxel2_{idx}_t_1k.dat"))
+    summed = subprocess.run(["python3", os.path.join(DRV, "sum_voxels.py"), *files], check=True,
+                            capture_output=True).stdout
+    full = tmp_path / "full.dat"
+    run([prog("3pcf_C_version_triangle_parametrization"), os.path.join(INP, "d200.dat"),
+         os.path.join(INP, "d200.dat"), os.path.join(INP, "r180.dat"), "--axis", "0,10,1,12", "--axis", "1,8,1,3",
+         "--axis", "2,8,0,1", "-o", str(full), "--quiet"])
+    assert summed == full.read_bytes() + b"\n"
