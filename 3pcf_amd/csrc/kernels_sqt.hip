@@ -66,6 +66,43 @@ __device__ inline void sqt_flush(uint32_t *h, int cells, unsigned long long *g) 
     __syncthreads();
 }
 
+// The FP64 definition of one triangle's entries, from its three squared
+// sides in any order: sort, IEEE sqrt and divisions, bins against the host
+// tables (sT: squared s thresholds, sQ: q edges, sA: cosine table for SQT or
+// (r3-r2)/r1 edges for JBJ).  Triangles with a zero side are dropped.
+template <int PARAM>
+__device__ inline void sqt_entries_fp64(double s1, double s2, double s3, const double *sT, const double *sQ,
+                                        const double *sA, int ns, int nq, int nt, uint32_t *hist) {
+    if (s1 == 0.0 || s2 == 0.0 || s3 == 0.0) return;
+    double t;
+    if (s1 > s2) { t = s1; s1 = s2; s2 = t; }
+    if (s2 > s3) { t = s2; s2 = s3; s3 = t; }
+    if (s1 > s2) { t = s1; s1 = s2; s2 = t; }
+    const double l1 = sqrt(s1), l2 = sqrt(s2), l3 = sqrt(s3);
+    const int is1 = bin_asc(s1, sT, ns);
+    if (PARAM == kJbj) {
+        if (is1 < 0) return;
+        const int iq = bin_asc(l1 / l2, sQ, nq);
+        const int iv = bin_asc((l3 - l2) / l1, sA, nt);
+        if (iq >= 0 && iv >= 0) lds_inc(hist, (is1 * nq + iq) * nt + iv);
+        return;
+    }
+    if (is1 >= 0) {
+        int iq = bin_asc(l2 / l1, sQ, nq);
+        int it = bin_cos(((s1 + s2) - s3) / ((2.0 * l1) * l2), sA, nt);
+        if (iq >= 0 && it >= 0) lds_inc(hist, (is1 * nq + iq) * nt + it);
+        iq = bin_asc(l3 / l1, sQ, nq);
+        it = bin_cos(((s1 + s3) - s2) / ((2.0 * l1) * l3), sA, nt);
+        if (iq >= 0 && it >= 0) lds_inc(hist, (is1 * nq + iq) * nt + it);
+    }
+    const int is2 = bin_asc(s2, sT, ns);
+    if (is2 >= 0) {
+        const int iq = bin_asc(l3 / l2, sQ, nq);
+        const int it = bin_cos(((s2 + s3) - s1) / ((2.0 * l2) * l3), sA, nt);
+        if (iq >= 0 && it >= 0) lds_inc(hist, (is2 * nq + iq) * nt + it);
+    }
+}
+
 }  // namespace
 
 // One entry point for both parametrizations: PARAM == kSqt or kJbj.  Lane k owns point k of block C; triplets are (i in A,
@@ -107,37 +144,8 @@ __global__ __launch_bounds__(256) void sqt_plain_kernel(SqtArgs P) {
                 for (int j = (A == B ? i + 1 : 0); j < cntB; ++j) {
                     if (!kval || (B == C && lane <= j)) continue;
                     const int64_t gj = (int64_t)B * kBlock + j;
-                    double s1 = sq64(P.u64, np, gi, gj);
-                    double s2 = sq64(P.u64, np, gi, gk);
-                    double s3 = sq64(P.u64, np, gj, gk);
-                    if (s1 == 0.0 || s2 == 0.0 || s3 == 0.0) continue;
-                    double t;
-                    if (s1 > s2) { t = s1; s1 = s2; s2 = t; }
-                    if (s2 > s3) { t = s2; s2 = s3; s3 = t; }
-                    if (s1 > s2) { t = s1; s1 = s2; s2 = t; }
-                    const double l1 = sqrt(s1), l2 = sqrt(s2), l3 = sqrt(s3);
-                    const int is1 = bin_asc(s1, sT, ns);   // thresholds on squared sides
-                    if (PARAM == kJbj) {
-                        if (is1 < 0) continue;
-                        const int iq = bin_asc(l1 / l2, sQ, nq);
-                        const int iv = bin_asc((l3 - l2) / l1, sA, nt);
-                        if (iq >= 0 && iv >= 0) lds_inc(hist, (is1 * nq + iq) * nt + iv);
-                        continue;
-                    }
-                    if (is1 >= 0) {
-                        int iq = bin_asc(l2 / l1, sQ, nq);
-                        int it2 = bin_cos(((s1 + s2) - s3) / ((2.0 * l1) * l2), sA, nt);
-                        if (iq >= 0 && it2 >= 0) lds_inc(hist, (is1 * nq + iq) * nt + it2);
-                        iq = bin_asc(l3 / l1, sQ, nq);
-                        it2 = bin_cos(((s1 + s3) - s2) / ((2.0 * l1) * l3), sA, nt);
-                        if (iq >= 0 && it2 >= 0) lds_inc(hist, (is1 * nq + iq) * nt + it2);
-                    }
-                    const int is2 = bin_asc(s2, sT, ns);
-                    if (is2 >= 0) {
-                        const int iq = bin_asc(l3 / l2, sQ, nq);
-                        const int it2 = bin_cos(((s2 + s3) - s1) / ((2.0 * l2) * l3), sA, nt);
-                        if (iq >= 0 && it2 >= 0) lds_inc(hist, (is2 * nq + iq) * nt + it2);
-                    }
+                    sqt_entries_fp64<PARAM>(sq64(P.u64, np, gi, gj), sq64(P.u64, np, gi, gk), sq64(P.u64, np, gj, gk),
+                                            sT, sQ, sA, ns, nq, nt, hist);
                 }
             }
         }

@@ -1,0 +1,346 @@
+// kernels_sqt_fast.hip -- FAST form of the (s,q,theta) histograms on gfx950.
+//
+// Pair stage (sqt_pairs_kernel): for every pair of a job's union layout, in
+// FP64, the squared side s, its exact s bin against the squared thresholds,
+// and a code: the s bin, kCodeNear (no s bin but short enough to be a q or
+// angle side), kCodeFar (longer than s_hi*(1+q_hi): no entry of any triangle
+// containing it can be in range) or kCodeZero (coincident points); plus the
+// side length rounded to FP32.
+//
+// Triplet stage (sqt_fast_kernel): persistent over (term, A, B, C-range)
+// items like the (r1,r2,r3) kernel.  Per row i of block A, phase 1 walks the
+// 64 columns j of block B with x = code(i,j) uniform and z = code(j,k)
+// preloaded per lane, and marks candidate triangles (no far or zero side, at
+// least one side with an s bin) in a per-lane 64-bit mask.  Phase 2 pulls
+// every lane's candidates in turn, so lanes with candidates run the entry
+// arithmetic together: sides sorted in FP32, q and the cosine-rule value in
+// FP32 with rigorous error bounds, bins from the FP32 copies of the host
+// tables.  A triangle whose side order is not certain, or any of whose values
+// lies within its error bound of an edge, is recomputed from the FP64
+// coordinates with the same code as the plain kernel, so the histograms equal
+// the FP64 definition bit for bit.
+#include "tpcf_internal.h"
+
+namespace tpcf {
+
+namespace {
+
+__device__ inline double sq64f(const double *u, int64_t npad, int64_t a, int64_t b) {
+    const double dx = u[a] - u[b];
+    const double dy = u[npad + a] - u[npad + b];
+    const double dz = u[2 * npad + a] - u[2 * npad + b];
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+__device__ inline int bin_asc64(double v, const double *e, int n) {
+    if (!(v >= e[0]) || v >= e[n]) return -1;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (v >= e[mid]) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline int bin_cos64(double c, const double *C, int n) {
+    if (!(c <= C[0]) || c <= C[n]) return -1;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (c <= C[mid]) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline void inc(uint32_t *h, int cell) {
+    __hip_atomic_fetch_add(h + cell, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// FP64 definition of one triangle (the plain kernel's arithmetic).
+__device__ inline void entries_fp64(double s1, double s2, double s3, const double *sT, const double *sQ,
+                                    const double *sA, int ns, int nq, int nt, uint32_t *hist) {
+    if (s1 == 0.0 || s2 == 0.0 || s3 == 0.0) return;
+    double t;
+    if (s1 > s2) { t = s1; s1 = s2; s2 = t; }
+    if (s2 > s3) { t = s2; s2 = s3; s3 = t; }
+    if (s1 > s2) { t = s1; s1 = s2; s2 = t; }
+    const double l1 = sqrt(s1), l2 = sqrt(s2), l3 = sqrt(s3);
+    const int is1 = bin_asc64(s1, sT, ns);
+    if (is1 >= 0) {
+        int iq = bin_asc64(l2 / l1, sQ, nq);
+        int it = bin_cos64(((s1 + s2) - s3) / ((2.0 * l1) * l2), sA, nt);
+        if (iq >= 0 && it >= 0) inc(hist, (is1 * nq + iq) * nt + it);
+        iq = bin_asc64(l3 / l1, sQ, nq);
+        it = bin_cos64(((s1 + s3) - s2) / ((2.0 * l1) * l3), sA, nt);
+        if (iq >= 0 && it >= 0) inc(hist, (is1 * nq + iq) * nt + it);
+    }
+    const int is2 = bin_asc64(s2, sT, ns);
+    if (is2 >= 0) {
+        const int iq = bin_asc64(l3 / l2, sQ, nq);
+        const int it = bin_cos64(((s2 + s3) - s1) / ((2.0 * l2) * l3), sA, nt);
+        if (iq >= 0 && it >= 0) inc(hist, (is2 * nq + iq) * nt + it);
+    }
+}
+
+// Largest m in [-1, n] with e[m] <= v for ascending FP32 edges e[0..n].
+__device__ inline int search_asc32(float v, const float *e, int n) {
+    if (!(v >= e[0])) return -1;
+    int lo = 0, hi = n + 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (e[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Largest m in [-1, n] with c <= C[m] for the descending FP32 cosine table.
+__device__ inline int search_desc32(float c, const float *C, int n) {
+    if (!(c <= C[0])) return -1;
+    int lo = 0, hi = n + 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (c <= C[mid]) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+constexpr float kU = 5.9604644775390625e-8f;  // 2^-24
+
+// One entry in FP32.  Returns the histogram cell, -1 for "out of range"
+// (certain), and sets *unsure when some value lies within its error bound of
+// an edge.  Error bounds, with every FP32 side within 2^-25 (relative) of the
+// FP32 rounding of the FP64 side: q = b/a is within 1.6 u of the FP64
+// quotient (6 u used); the cosine-rule value within
+// 3.8 u (sa+sb+sc)/den + 2.6 u |c| (5 u and 4 u used), plus 2 u for the FP32
+// rounding of the table.
+__device__ inline int entry32(int sbin, float a, float b, float num, float den, float ssum, const float *fQ,
+                              const float *fA, int nq, int nt, bool *unsure) {
+    const float q = b / a;
+    const int mq = search_asc32(q, fQ, nq);
+    const float dq = 6.f * kU * q;
+    if (mq >= 0 && !(q - fQ[mq] > dq + kU * fQ[mq])) *unsure = true;
+    if (mq < nq && !(fQ[mq + 1] - q > dq + kU * fQ[mq + 1])) *unsure = true;
+    const float c = num / den;
+    const float dc = 5.f * kU * ssum / den + 4.f * kU * fabsf(c) + 2.f * kU;
+    const int mt = search_desc32(c, fA, nt);
+    if (mt >= 0 && !(fA[mt] - c > dc)) *unsure = true;
+    if (mt < nt && !(c - fA[mt + 1] > dc)) *unsure = true;
+    if (mq < 0 || mq >= nq || mt < 0 || mt >= nt) return -1;
+    return (sbin * nq + mq) * nt + mt;
+}
+
+__device__ inline void sqt_flush64(uint32_t *h, int cells, unsigned long long *g) {
+    __syncthreads();
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+        const uint32_t v = h[c];
+        if (v) {
+            atomicAdd(g + c, (unsigned long long)v);
+            h[c] = 0u;
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace
+
+// One 64x64 tile (R, C), R <= C, per 256-thread workgroup; thread t owns row
+// t/4 and 16 consecutive columns.
+__global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
+    const int64_t R = blockIdx.y, C = blockIdx.x;
+    if (C < R) return;
+    __shared__ double sT[kMaxBins + 1];
+    const int t = threadIdx.x;
+    for (int m = t; m <= J.ns; m += 256) sT[m] = J.Ts[m];
+    __syncthreads();
+    const int cntR = J.blk_count[R], cntC = J.blk_count[C];
+    const int i = t >> 2, jb = (t & 3) * 16;
+    const int64_t gi = R * kBlock + i;
+    uint32_t packed[4] = {0u, 0u, 0u, 0u};
+    float dist[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int j = jb + q;
+        uint32_t code = kCodeFar;
+        float d = 0.f;
+        if (i < cntR && j < cntC && (R != C || j > i)) {
+            const double s = sq64f(J.u64, J.npad, gi, C * kBlock + j);
+            d = (float)sqrt(s);
+            if (s == 0.0) {
+                code = kCodeZero;
+            } else if (!(s > J.far_s)) {
+                const int m = bin_asc64(s, sT, J.ns);
+                code = m >= 0 ? (uint32_t)m : kCodeNear;
+            }
+        }
+        dist[q] = d;
+        packed[q >> 2] |= code << (8 * (q & 3));
+    }
+    const int64_t tile = tile_index(R, C, J.nblk);
+    *reinterpret_cast<uint4 *>(J.codes + tile * kTileBytes + i * kBlock + jb) =
+        make_uint4(packed[0], packed[1], packed[2], packed[3]);
+    float4 *dd = reinterpret_cast<float4 *>(J.dists + tile * kTileBytes + i * kBlock + jb);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dd[q] = make_float4(dist[4 * q], dist[4 * q + 1], dist[4 * q + 2], dist[4 * q + 3]);
+}
+
+typedef const uint32_t __attribute__((address_space(4))) *scalar_u32_ptr;
+
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
+    extern __shared__ uint32_t hist[];
+    __shared__ double sT[kMaxBins + 1], sQ[kMaxBins + 1], sA[kMaxBins + 1];
+    __shared__ float fQ[kMaxBins + 1], fA[kMaxBins + 1];
+    __shared__ int s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ns = P.ns, nq = P.nq, nt = P.nt, cells = P.cells;
+    for (int m = threadIdx.x; m <= kMaxBins; m += blockDim.x) {
+        if (m <= ns) sT[m] = P.Ts[m];
+        if (m <= nq) {
+            sQ[m] = P.Eq[m];
+            fQ[m] = (float)P.Eq[m];
+        }
+        if (m <= nt) {
+            sA[m] = P.Et[m];
+            fA[m] = (float)P.Et[m];
+        }
+    }
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) hist[c] = 0u;
+    int cur = -1;
+    unsigned long long since = 0, nre = 0;
+    const int64_t np = P.npad, nblk = P.nblk;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= P.nitems) break;
+        const TripletItem I = P.items[it];
+        const int term = __builtin_amdgcn_readfirstlane(I.term);
+        const int A = __builtin_amdgcn_readfirstlane(I.A);
+        const int B = __builtin_amdgcn_readfirstlane(I.B);
+        const int c0 = __builtin_amdgcn_readfirstlane(I.c0);
+        const int c1 = __builtin_amdgcn_readfirstlane(I.c1);
+        const unsigned long long add = 3ull * (unsigned long long)(c1 - c0) * (kBlock * kBlock * kBlock);
+        if (term != cur || since + add > 0xC0000000ull) {
+            if (cur >= 0) sqt_flush64(hist, cells, P.hist + (size_t)cur * cells);
+            cur = term;
+            since = 0;
+        }
+        since += add;
+        const int64_t tAB = tile_index(A, B, nblk) * kTileBytes;
+        const uint8_t *cAB = P.codes + tAB;
+        const float *dAB = P.dists + tAB;
+        for (int C = c0 + wave; C < c1; C += WAVES) {
+            const int64_t tAC = tile_index(A, C, nblk) * kTileBytes;
+            const int64_t tBC = tile_index(B, C, nblk) * kTileBytes;
+            const uint8_t *cAC = P.codes + tAC, *cBC = P.codes + tBC;
+            const float *dAC = P.dists + tAC, *dBC = P.dists + tBC;
+            uint32_t Zc[kBlock];
+#pragma unroll
+            for (int j = 0; j < kBlock; ++j) Zc[j] = cBC[j * kBlock + lane];
+            for (int i = 0; i < kBlock; ++i) {
+                const uint32_t xv = cAB[i * kBlock + lane];
+                const unsigned long long rowmask = __ballot(xv < kCodeZero);
+                if (rowmask == 0ull) continue;
+                const uint32_t yc = cAC[i * kBlock + lane];
+                if (yc >= kCodeZero) continue;
+                const bool yshort = yc < kCodeNear;
+                scalar_u32_ptr xrow = (scalar_u32_ptr)(cAB + i * kBlock);
+                uint32_t xw[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) xw[q] = xrow[q];
+                // Phase 1: candidate columns of this lane.
+                uint32_t blo = 0u, bhi = 0u;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = q * 4 + r;
+                        const uint32_t xc = (xw[q] >> (8 * r)) & 0xFFu;
+                        if (xc < kCodeZero) {
+                            const uint32_t zc = Zc[j];
+                            const bool cand = zc < kCodeZero && (xc < kCodeNear || yshort || zc < kCodeNear);
+                            if (cand) {
+                                if (j < 32) blo |= 1u << j;
+                                else bhi |= 1u << (j - 32);
+                            }
+                        }
+                    }
+                }
+                if (!(blo | bhi)) continue;
+                // Phase 2: every candidate triangle of this lane.
+                const float yd = dAC[i * kBlock + lane];
+                while (blo | bhi) {
+                    int j;
+                    if (blo) {
+                        j = __builtin_ctz(blo);
+                        blo &= blo - 1u;
+                    } else {
+                        j = 32 + __builtin_ctz(bhi);
+                        bhi &= bhi - 1u;
+                    }
+                    const uint32_t xc = cAB[i * kBlock + j];
+                    const float xd = dAB[i * kBlock + j];
+                    const uint32_t zc = cBC[j * kBlock + lane];
+                    const float zd = dBC[j * kBlock + lane];
+                    // Sort the sides by FP32 length, carrying their codes.
+                    float l1 = xd, l2 = yd, l3 = zd;
+                    uint32_t k1 = xc, k2 = yc, k3 = zc;
+                    if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
+                    if (l2 > l3) { float t = l2; l2 = l3; l3 = t; uint32_t u = k2; k2 = k3; k3 = u; }
+                    if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
+                    bool unsure = P.force_fp64 != 0;
+                    // Side order certain only when lengths differ by more
+                    // than their FP32 rounding.
+                    if (!(l2 - l1 > 4.f * kU * l2) || !(l3 - l2 > 4.f * kU * l3)) unsure = true;
+                    int cell[3] = {-1, -1, -1};
+                    if (!unsure) {
+                        const float s1 = l1 * l1, s2 = l2 * l2, s3 = l3 * l3;
+                        const float ssum = (s1 + s2) + s3;
+                        if (k1 < kCodeNear) {
+                            cell[0] = entry32((int)k1, l1, l2, (s1 + s2) - s3, (2.f * l1) * l2, ssum, fQ, fA, nq,
+                                              nt, &unsure);
+                            cell[1] = entry32((int)k1, l1, l3, (s1 + s3) - s2, (2.f * l1) * l3, ssum, fQ, fA, nq,
+                                              nt, &unsure);
+                        }
+                        if (k2 < kCodeNear)
+                            cell[2] = entry32((int)k2, l2, l3, (s2 + s3) - s1, (2.f * l2) * l3, ssum, fQ, fA, nq,
+                                              nt, &unsure);
+                    }
+                    if (!unsure) {
+#pragma unroll
+                        for (int e = 0; e < 3; ++e)
+                            if (cell[e] >= 0) inc(hist, cell[e]);
+                    } else {
+                        ++nre;
+                        const int64_t gi = (int64_t)A * kBlock + i, gj = (int64_t)B * kBlock + j,
+                                      gk = (int64_t)C * kBlock + lane;
+                        entries_fp64(sq64f(P.u64, np, gi, gj), sq64f(P.u64, np, gi, gk), sq64f(P.u64, np, gj, gk),
+                                     sT, sQ, sA, ns, nq, nt, hist);
+                    }
+                }
+            }
+        }
+    }
+    if (cur >= 0) sqt_flush64(hist, cells, P.hist + (size_t)cur * cells);
+    if (nre) atomicAdd(P.rechecked, nre);
+}
+
+hipError_t launch_sqt_pairs(const SqtPairJob &J, hipStream_t st) {
+    hipLaunchKernelGGL(sqt_pairs_kernel, dim3((unsigned)J.nblk, (unsigned)J.nblk), dim3(256), 0, st, J);
+    return hipGetLastError();
+}
+
+const void *sqt_fast_kernel_ptr() { return reinterpret_cast<const void *>(&sqt_fast_kernel<8>); }
+
+hipError_t launch_sqt_fast(const SqtFastArgs &P, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(sqt_fast_kernel<8>, dim3(grid), dim3(512), (size_t)P.cells * 4, st, P);
+    return hipGetLastError();
+}
+
+}  // namespace tpcf

@@ -32,6 +32,9 @@ struct Ticket {
     std::chrono::steady_clock::time_point t_submit;
 };
 
+// Jobs that do not join a batched persistent launch.
+static bool solo(const Ticket &t) { return t.mode == TPCF_MODE_PLAIN || t.spec.param != TPCF_R123; }
+
 static constexpr int kWorkers = 3;
 static constexpr int kMaxBatch = 64;
 
@@ -48,12 +51,13 @@ static void worker_main(tpcf_ctx *ctx) {
             std::unique_lock<std::mutex> lk(ctx->mu);
             for (;;) {
                 if (ctx->stopping && ctx->pending.empty()) break;
-                // First fit in arrival order; PLAIN jobs run alone.
+                // First fit in arrival order; PLAIN jobs and triangle
+                // parametrizations run alone.
                 for (auto it = ctx->pending.begin(); it != ctx->pending.end() && (int)batch.size() < kMaxBatch;) {
                     auto t = ctx->tickets[*it];
-                    const bool plain = t->mode == TPCF_MODE_PLAIN;
+                    const bool plain = solo(*t);
                     if (plain && !batch.empty()) { ++it; continue; }
-                    if (!batch.empty() && batch[0]->mode == TPCF_MODE_PLAIN) break;
+                    if (!batch.empty() && solo(*batch[0])) break;
                     if (!batch.empty() && t->mode != batch[0]->mode) { ++it; continue; }
                     if ((uint64_t)t->footprint > ctx->budget) {
                         t->state = 3;
@@ -82,7 +86,7 @@ static void worker_main(tpcf_ctx *ctx) {
         }
         std::vector<int> rcs(batch.size(), TPCF_OK);
         std::vector<std::string> errs(batch.size());
-        if (batch[0]->mode == TPCF_MODE_PLAIN) {
+        if (solo(*batch[0])) {
             rcs[0] = run_single(ctx, *ws, batch[0]->job, batch[0]->spec, batch[0]->mode, batch[0]->hist,
                                 &batch[0]->stats, st);
             if (rcs[0]) errs[0] = tpcf_last_error();

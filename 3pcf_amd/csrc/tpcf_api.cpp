@@ -158,9 +158,10 @@ struct JobPlan {
     uint64_t triplets = 0;
     // byte offsets inside the job's device region
     size_t o_u64 = 0, o_u32 = 0, o_blk = 0, o_T64 = 0, o_T32 = 0, o_Eq = 0, o_Et = 0, o_cnt = 0, o_hist = 0,
-           o_tiles = 0;
+           o_tiles = 0, o_dists = 0;
     size_t bytes = 0;
     bool need_tiles = true;
+    double far_s = 0;  // SQT FAST: squared sides above this cannot be in any entry
 };
 
 static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, JobPlan &P) {
@@ -168,8 +169,10 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     if (int e = binspec_check(spec)) return e;
     if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_PLAIN && mode != TPCF_MODE_FAST64)
         return set_error(TPCF_EINVAL, "unknown mode %d", mode);
-    if (spec.param != TPCF_R123 && mode != TPCF_MODE_PLAIN)
-        return set_error(TPCF_EINVAL, "(s,q,theta) and JBJ histograms run in TPCF_MODE_PLAIN");
+    if (spec.param == TPCF_JBJ && mode != TPCF_MODE_PLAIN)
+        return set_error(TPCF_EINVAL, "JBJ histograms run in TPCF_MODE_PLAIN");
+    if (spec.param == TPCF_SQT && mode != TPCF_MODE_PLAIN && spec.ax[0].n > kMaxFastSqtBins)
+        return set_error(TPCF_EINVAL, "the FAST (s,q,theta) kernel takes up to %d s bins", kMaxFastSqtBins);
     P.L = make_layout(job);
     P.param = spec.param;
     P.n = spec.ax[0].n;
@@ -232,6 +235,15 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     P.o_cnt = o; o = align256(o + 64);
     P.o_hist = o; o = align256(o + sizeof(uint64_t) * P.cells * P.nterms);
     P.o_tiles = o; o = align256(o + (P.need_tiles ? (size_t)tile_count(P.L.nblk) * kTileBytes : 0));
+    const bool sqt_fast = spec.param == TPCF_SQT && mode != TPCF_MODE_PLAIN;
+    P.o_dists = o; o = align256(o + (sqt_fast ? (size_t)tile_count(P.L.nblk) * kTileBytes * sizeof(float) : 0));
+    if (sqt_fast) {
+        // No entry of a triangle with a side >= s_hi*(1+q_hi) can be in range
+        // (its s side is < s_hi, the q side < q_hi*s_hi, the third side at
+        // most their sum); a relative margin keeps the cut conservative.
+        const double dmax = spec.ax[0].hi * (1.0 + std::max(spec.ax[1].hi, 1.0)) * (1.0 + 1e-6);
+        P.far_s = dmax * dmax;
+    }
     P.bytes = o;
     return TPCF_OK;
 }
@@ -460,6 +472,66 @@ static int launch_plain(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tp
     return TPCF_OK;
 }
 
+// FAST (s,q,theta): FP64 pair codes and FP32 side lengths, then the
+// persistent candidate kernel over the items of every term (FP32 entries,
+// FP64 recheck near edges; FAST64 sends every candidate through FP64).
+static int launch_sqt_fast_job(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tpcf_job &job, char *base,
+                               int mode, hipStream_t st) {
+    SqtPairJob J{};
+    J.u64 = reinterpret_cast<const double *>(base + P.o_u64);
+    J.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
+    J.codes = reinterpret_cast<uint8_t *>(base + P.o_tiles);
+    J.dists = reinterpret_cast<float *>(base + P.o_dists);
+    J.Ts = reinterpret_cast<const double *>(base + P.o_T64);
+    J.ns = P.n;
+    J.far_s = P.far_s;
+    J.npad = P.L.npad;
+    J.nblk = P.L.nblk;
+    HIP_TRY(launch_sqt_pairs(J, st));
+    HIP_TRY(hipEventRecord(ws.ev[1], st));
+    std::vector<TripletItem> items;
+    for (int t = 0; t < P.nterms; ++t) {
+        const size_t before = items.size();
+        term_items(P.L, job.term[t], t, items);
+        std::stable_sort(items.begin() + before, items.end(), [](const TripletItem &a, const TripletItem &b) {
+            return (a.c1 - a.c0) > (b.c1 - b.c0);
+        });
+    }
+    if (items.empty()) return TPCF_OK;
+    const size_t o_ctr = align256(sizeof(TripletItem) * items.size());
+    if (int e = ws.queue.reserve(o_ctr + 256)) return e;
+    char *q = reinterpret_cast<char *>(ws.queue.p);
+    HIP_TRY(hipMemcpyAsync(q, items.data(), sizeof(TripletItem) * items.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(q + o_ctr), 1, 0u, st));
+    SqtFastArgs A{};
+    A.codes = J.codes;
+    A.dists = J.dists;
+    A.u64 = J.u64;
+    A.Ts = J.Ts;
+    A.Eq = reinterpret_cast<const double *>(base + P.o_Eq);
+    A.Et = reinterpret_cast<const double *>(base + P.o_Et);
+    A.items = reinterpret_cast<const TripletItem *>(q);
+    A.nitems = (int)items.size();
+    A.counter = reinterpret_cast<int *>(q + o_ctr);
+    A.ns = P.n;
+    A.nq = P.nq;
+    A.nt = P.nt;
+    A.cells = (int)P.cells;
+    A.force_fp64 = mode == TPCF_MODE_FAST64;
+    A.npad = P.L.npad;
+    A.nblk = P.L.nblk;
+    A.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist);
+    A.rechecked = reinterpret_cast<unsigned long long *>(base + P.o_cnt);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sqt_fast_kernel_ptr(), 512, (size_t)P.cells * 4) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    HIP_TRY(hipEventRecord(ws.ev[3], st));
+    HIP_TRY(launch_sqt_fast(A, ctx->num_cu * per_cu, st));
+    return TPCF_OK;
+}
+
 // Pair-stage and triplet-kernel times from the workspace events.
 static int stage_times(Workspace &ws, tpcf_stats *s) {
     float a = 0.f, b = 0.f;
@@ -487,6 +559,8 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
         if (P.L.nblk) {
             if (int e = launch_plain(ctx, ws, P, job, base, st)) return e;
         }
+    } else if (P.param == TPCF_SQT && P.L.nblk) {
+        if (int e = launch_sqt_fast_job(ctx, ws, P, job, base, mode, st)) return e;
     } else if (P.L.nblk) {
         HIP_TRY(launch_pair_tiles(pair_job(P, base, P.n <= kMaxLdsR123Bins ? 2 : 0), mode == TPCF_MODE_FAST, st));
         HIP_TRY(hipEventRecord(ws.ev[1], st));
@@ -528,6 +602,8 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     size_t bytes = 0;
     std::vector<size_t> off(njobs);
     for (int j = 0; j < njobs; ++j) {
+        if (specs[j].param != TPCF_R123)
+            return set_error(TPCF_EINVAL, "batches hold (r1,r2,r3) jobs; count triangle parametrizations one by one");
         if (int e = plan_job(jobs[j], specs[j], mode, plans[j])) return e;
         off[j] = bytes;
         bytes = align256(bytes + plans[j].bytes);
@@ -649,6 +725,8 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     for (int p : {kSqt, kJbj})
         (void)hipFuncSetAttribute(sqt_plain_kernel_ptr(p), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024 - 8 * 1024);
+    (void)hipFuncSetAttribute(sqt_fast_kernel_ptr(), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024 - 10 * 1024);
     if (const char *env = std::getenv("TPCF_R123_SHAPE")) {
         int w = 0, k = 0, p = 0;
         if (std::sscanf(env, "%d,%d,%d", &w, &k, &p) >= 2) c->shape = LaunchShape{w, k, p};
@@ -880,6 +958,8 @@ int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *s
     size_t bytes = 0;
     int max_n = 1;
     for (int j = 0; j < njobs; ++j) {
+        if (specs[j].param != TPCF_R123)
+            return set_error(TPCF_EINVAL, "batches hold (r1,r2,r3) jobs; count triangle parametrizations one by one");
         if (int e = plan_job(jobs[j], specs[j], mode, b->plans[j])) return e;
         b->off[j] = bytes;
         bytes = align256(bytes + b->plans[j].bytes);

@@ -84,6 +84,40 @@ struct SqtArgs {
 hipError_t launch_sqt_plain(const SqtArgs &P, int param, int grid, hipStream_t st);
 const void *sqt_plain_kernel_ptr(int param);
 
+// (s,q,theta) FAST form.  Pair codes: s bin (< kCodeNear), or one of
+constexpr uint32_t kCodeFar = 0xFF;   // not a pair, or too long for any entry
+constexpr uint32_t kCodeZero = 0xFE;  // coincident points: the triangle is dropped
+constexpr uint32_t kCodeNear = 0xFD;  // no s bin, short enough to be a q/theta side
+constexpr int kMaxFastSqtBins = 253;
+
+struct SqtPairJob {
+    const double *u64;
+    const uint8_t *blk_count;
+    uint8_t *codes;  // 4 KiB per tile
+    float *dists;    // 16 KiB per tile: (float)sqrt(s) of the FP64 squared side
+    const double *Ts;
+    int ns;
+    double far_s;    // squared sides above this are kCodeFar
+    int64_t npad, nblk;
+};
+struct SqtFastArgs {
+    const uint8_t *codes;
+    const float *dists;
+    const double *u64;
+    const double *Ts, *Eq, *Et;
+    const TripletItem *items;
+    int nitems;
+    int *counter;
+    int ns, nq, nt, cells;
+    int force_fp64;  // every candidate triangle through the FP64 definition
+    int64_t npad, nblk;
+    unsigned long long *hist;       // term t at hist + t*cells
+    unsigned long long *rechecked;  // triangles recomputed in FP64
+};
+hipError_t launch_sqt_pairs(const SqtPairJob &J, hipStream_t st);
+hipError_t launch_sqt_fast(const SqtFastArgs &P, int grid, hipStream_t st);
+const void *sqt_fast_kernel_ptr();
+
 // Launchers (kernels_r123.hip).
 hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st);
 hipError_t launch_r123_triplets(const TripletItem *items, int nitems, int *counter, const TripletTerm *terms,

@@ -30,12 +30,16 @@ def oracle(segs, terms, param, axes):
     return orc.clean_count(segs, terms, param, [orc.axis(*a) for a in axes])
 
 
+SQT_MODES = [tp.MODE_PLAIN, tp.MODE_FAST, tp.MODE_FAST64]
+
+
+@pytest.mark.parametrize("mode", SQT_MODES)
 @pytest.mark.parametrize("axes", [REF_SPEC, CFG2_SPEC], ids=["ref", "cfg2"])
-def test_sqt_four_terms(ctx, axes):
-    print(f"case sqt four terms {axes[0]}", flush=True)
+def test_sqt_four_terms(ctx, axes, mode):
+    print(f"case sqt four terms {axes[0]} mode={mode}", flush=True)
     d = synth.neyman_scott(160, nclust=12, sigma=2.0, size=25.0, seed=21)
     r = synth.uniform_cube(140, 25.0, seed=22)
-    got, st = run(ctx, [d, r], tp.FOUR_TERMS, tp.SQT, axes)
+    got, st = run(ctx, [d, r], tp.FOUR_TERMS, tp.SQT, axes, mode)
     want = oracle([d, r], tp.FOUR_TERMS, orc.SQT, axes)
     assert np.array_equal(got, want)
     assert int(got.sum()) > 0
@@ -49,30 +53,47 @@ def test_jbj(ctx):
     assert int(got.sum()) > 0
 
 
-def test_sqt_exact_geometry(ctx):
-    """Integer lattice: right angles, equilateral-free, collinear and
-    isoceles triangles with sides landing on q edges and theta edges."""
-    print("case sqt lattice", flush=True)
+@pytest.mark.parametrize("mode", SQT_MODES)
+def test_sqt_exact_geometry(ctx, mode):
+    """Integer lattice: right angles, collinear and isoceles triangles with
+    sides landing exactly on s, q and theta edges (the FAST kernel must send
+    them to its FP64 recheck)."""
+    print(f"case sqt lattice mode={mode}", flush=True)
     g = np.stack(np.meshgrid(np.arange(4), np.arange(4), np.arange(3), indexing="ij"), -1).reshape(-1, 3)
     pts = g.astype(np.float64) * 2.0
     axes = [(6, 1.0, 7.0, "lin"), (4, 1.0, 3.0, "lin"), (4, 0.0, 1.0, "lin")]
-    got, _ = run(ctx, [pts], [(0, 0, 0)], tp.SQT, axes)
+    got, st = run(ctx, [pts], [(0, 0, 0)], tp.SQT, axes, mode)
     assert np.array_equal(got, oracle([pts], [(0, 0, 0)], orc.SQT, axes))
+    if mode == tp.MODE_FAST:
+        assert st.pairs_rechecked > 0
     # 3-4-5 right triangles: theta/pi = 0.5 exactly on an edge of 4 bins.
     tri = np.array([[0.0, 0.0, 0.0], [3.0, 0.0, 0.0], [0.0, 4.0, 0.0]])
-    got, _ = run(ctx, [tri], [(0, 0, 0)], tp.SQT, axes)
+    got, _ = run(ctx, [tri], [(0, 0, 0)], tp.SQT, axes, mode)
     assert np.array_equal(got, oracle([tri], [(0, 0, 0)], orc.SQT, axes))
 
 
-def test_sqt_duplicates_dropped(ctx):
-    print("case sqt duplicates", flush=True)
+@pytest.mark.parametrize("mode", SQT_MODES)
+def test_sqt_duplicates_dropped(ctx, mode):
+    print(f"case sqt duplicates mode={mode}", flush=True)
     d = synth.uniform_cube(60, 8.0, seed=24)
     d = np.vstack([d, d[:10]])
-    got, _ = run(ctx, [d], [(0, 0, 0)], tp.SQT, REF_SPEC)
+    got, _ = run(ctx, [d], [(0, 0, 0)], tp.SQT, REF_SPEC, mode)
     assert np.array_equal(got, oracle([d], [(0, 0, 0)], orc.SQT, REF_SPEC))
 
 
-def test_sqt_fast_mode_refused(ctx):
+def test_sqt_fast_far_from_origin_and_ragged(ctx):
+    """Wedge coordinates ~600 Mpc/h from the origin, ragged sizes, three
+    segments; FAST equals the FP64 oracle."""
+    print("case sqt fast wedge", flush=True)
+    a = synth.wedge(150, 1.0, 1.0, 600.0, 620.0, seed=26, clustered=True, nclust=6)
+    b = synth.wedge(97, 1.0, 1.0, 600.0, 620.0, seed=27)
+    axes = [(10, 1.0, 15.0, "log"), (8, 1.0, 4.0, "lin"), (12, 0.0, 1.0, "lin")]
+    terms = [(0, 0, 0), (0, 0, 1), (0, 1, 1), (1, 1, 1)]
+    got, st = run(ctx, [a, b], terms, tp.SQT, axes, tp.MODE_FAST)
+    assert np.array_equal(got, oracle([a, b], terms, orc.SQT, axes))
+
+
+def test_jbj_fast_mode_refused(ctx):
     d = synth.uniform_cube(20, 8.0, seed=25)
     with pytest.raises(tp.TpcfError):
-        run(ctx, [d], [(0, 0, 0)], tp.SQT, REF_SPEC, mode=tp.MODE_FAST)
+        run(ctx, [d], [(0, 0, 0)], tp.JBJ, JBJ_SPEC, mode=tp.MODE_FAST)
