@@ -187,13 +187,103 @@ __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
     for (int q = 0; q < 4; ++q) dd[q] = make_float4(dist[4 * q], dist[4 * q + 1], dist[4 * q + 2], dist[4 * q + 3]);
 }
 
-typedef const uint32_t __attribute__((address_space(4))) *scalar_u32_ptr;
+namespace {
 
+// Makes one wave's LDS stores visible to the other lanes of the same wave.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct SqtItemCtx {
+    const uint8_t *cAB;
+    const float *dAB;
+    int A, B, c0;
+};
+
+// Entries of the queued candidate triangles Q[0..cnt), one per lane.  An
+// entry packs i | j << 6 | k << 12 | (C - c0) << 18 for rows i of block A,
+// j of block B, k of block C.
+__device__ inline void sqt_drain(const SqtFastArgs &P, const SqtItemCtx &X, const uint32_t *Q, int cnt, int lane,
+                                 const double *sT, const double *sQ, const double *sA, const float *fQ,
+                                 const float *fA, uint32_t *hist, unsigned long long &nre) {
+    if (lane >= cnt) return;
+    const uint32_t e = Q[lane];
+    const int i = e & 63, j = (e >> 6) & 63, k = (e >> 12) & 63, C = X.c0 + (int)(e >> 18);
+    const int64_t tAC = tile_index(X.A, C, P.nblk) * kTileBytes;
+    const int64_t tBC = tile_index(X.B, C, P.nblk) * kTileBytes;
+    const uint32_t xc = X.cAB[i * kBlock + j], yc = P.codes[tAC + i * kBlock + k], zc = P.codes[tBC + j * kBlock + k];
+    const float xd = X.dAB[i * kBlock + j], yd = P.dists[tAC + i * kBlock + k], zd = P.dists[tBC + j * kBlock + k];
+    // Sort the sides by FP32 length, carrying their codes.
+    float l1 = xd, l2 = yd, l3 = zd;
+    uint32_t k1 = xc, k2 = yc, k3 = zc;
+    if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
+    if (l2 > l3) { float t = l2; l2 = l3; l3 = t; uint32_t u = k2; k2 = k3; k3 = u; }
+    if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
+    const int nq = P.nq, nt = P.nt;
+    bool unsure = P.force_fp64 != 0;
+    // Side order certain only when lengths differ by more than their FP32
+    // rounding.
+    if (!(l2 - l1 > 4.f * kU * l2) || !(l3 - l2 > 4.f * kU * l3)) unsure = true;
+    int cell[3] = {-1, -1, -1};
+    if (!unsure) {
+        const float s1 = l1 * l1, s2 = l2 * l2, s3 = l3 * l3;
+        const float ssum = (s1 + s2) + s3;
+        if (k1 < kCodeNear) {
+            cell[0] = entry32((int)k1, l1, l2, (s1 + s2) - s3, (2.f * l1) * l2, ssum, fQ, fA, nq, nt, &unsure);
+            cell[1] = entry32((int)k1, l1, l3, (s1 + s3) - s2, (2.f * l1) * l3, ssum, fQ, fA, nq, nt, &unsure);
+        }
+        if (k2 < kCodeNear)
+            cell[2] = entry32((int)k2, l2, l3, (s2 + s3) - s1, (2.f * l2) * l3, ssum, fQ, fA, nq, nt, &unsure);
+    }
+    if (!unsure) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            if (cell[m] >= 0) inc(hist, cell[m]);
+    } else {
+        ++nre;
+        const int64_t gi = (int64_t)X.A * kBlock + i, gj = (int64_t)X.B * kBlock + j, gk = (int64_t)C * kBlock + k;
+        entries_fp64(sq64f(P.u64, P.npad, gi, gj), sq64f(P.u64, P.npad, gi, gk), sq64f(P.u64, P.npad, gj, gk), sT,
+                     sQ, sA, P.ns, nq, nt, hist);
+    }
+}
+
+}  // namespace
+
+// Bits m of the lane's 64-bit words: code(m, lane) of a tile below kCodeZero
+// (near) and below kCodeNear (has an s bin); t points at the tile.
+__device__ inline void column_masks(const uint8_t *t, int lane, unsigned long long &nearm,
+                                    unsigned long long &shortm) {
+    uint32_t n0 = 0u, n1 = 0u, s0 = 0u, s1 = 0u;
+#pragma unroll
+    for (int m = 0; m < 32; ++m) {
+        const uint32_t a = t[m * kBlock + lane], b = t[(m + 32) * kBlock + lane];
+        n0 |= (uint32_t)(a < kCodeZero) << m;
+        s0 |= (uint32_t)(a < kCodeNear) << m;
+        n1 |= (uint32_t)(b < kCodeZero) << m;
+        s1 |= (uint32_t)(b < kCodeNear) << m;
+    }
+    nearm = (unsigned long long)n1 << 32 | n0;
+    shortm = (unsigned long long)s1 << 32 | s0;
+}
+
+// Persistent over (term, A, B, C-range) items.  Candidate triangles (no far
+// or zero side, at least one side with an s bin) are found with bit masks:
+// per item, for every row i of block A, the columns j whose side (i,j) is
+// near / has an s bin (two ballots per row, shared in LDS); per C block and
+// lane k, the rows j of side (j,k) and the rows i of side (i,k) likewise
+// (64-bit words in VGPRs).  A row's candidates for lane k are then
+//   X_near[i] & Z_near & Y_near(i) & (X_short[i] | Z_short | Y_short(i)).
+// Each wave takes the C blocks c0 + wave, c0 + wave + WAVES, ... and appends
+// its candidates to an LDS queue, drained 64 at a time with every lane busy.
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
     extern __shared__ uint32_t hist[];
     __shared__ double sT[kMaxBins + 1], sQ[kMaxBins + 1], sA[kMaxBins + 1];
     __shared__ float fQ[kMaxBins + 1], fA[kMaxBins + 1];
+    __shared__ uint32_t queue[WAVES][2 * kBlock];
+    __shared__ unsigned long long xnear[kBlock], xshort[kBlock];
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -210,9 +300,10 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         }
     }
     for (int c = threadIdx.x; c < cells; c += blockDim.x) hist[c] = 0u;
+    uint32_t *Q = queue[wave];
     int cur = -1;
     unsigned long long since = 0, nre = 0;
-    const int64_t np = P.npad, nblk = P.nblk;
+    const int64_t nblk = P.nblk;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
@@ -221,110 +312,81 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         if (it >= P.nitems) break;
         const TripletItem I = P.items[it];
         const int term = __builtin_amdgcn_readfirstlane(I.term);
-        const int A = __builtin_amdgcn_readfirstlane(I.A);
-        const int B = __builtin_amdgcn_readfirstlane(I.B);
-        const int c0 = __builtin_amdgcn_readfirstlane(I.c0);
+        SqtItemCtx X;
+        X.A = __builtin_amdgcn_readfirstlane(I.A);
+        X.B = __builtin_amdgcn_readfirstlane(I.B);
+        X.c0 = __builtin_amdgcn_readfirstlane(I.c0);
         const int c1 = __builtin_amdgcn_readfirstlane(I.c1);
-        const unsigned long long add = 3ull * (unsigned long long)(c1 - c0) * (kBlock * kBlock * kBlock);
+        const unsigned long long add = 3ull * (unsigned long long)(c1 - X.c0) * (kBlock * kBlock * kBlock);
         if (term != cur || since + add > 0xC0000000ull) {
             if (cur >= 0) sqt_flush64(hist, cells, P.hist + (size_t)cur * cells);
             cur = term;
             since = 0;
         }
         since += add;
-        const int64_t tAB = tile_index(A, B, nblk) * kTileBytes;
-        const uint8_t *cAB = P.codes + tAB;
-        const float *dAB = P.dists + tAB;
-        for (int C = c0 + wave; C < c1; C += WAVES) {
-            const int64_t tAC = tile_index(A, C, nblk) * kTileBytes;
-            const int64_t tBC = tile_index(B, C, nblk) * kTileBytes;
-            const uint8_t *cAC = P.codes + tAC, *cBC = P.codes + tBC;
-            const float *dAC = P.dists + tAC, *dBC = P.dists + tBC;
-            uint32_t Zc[kBlock];
-#pragma unroll
-            for (int j = 0; j < kBlock; ++j) Zc[j] = cBC[j * kBlock + lane];
+        const int64_t tAB = tile_index(X.A, X.B, nblk) * kTileBytes;
+        X.cAB = P.codes + tAB;
+        X.dAB = P.dists + tAB;
+        for (int i = wave; i < kBlock; i += WAVES) {
+            const uint32_t xv = X.cAB[i * kBlock + lane];
+            const unsigned long long n = __ballot(xv < kCodeZero), sh = __ballot(xv < kCodeNear);
+            if (lane == 0) {
+                xnear[i] = n;
+                xshort[i] = sh;
+            }
+        }
+        __syncthreads();
+        int qn = 0;  // queued entries of this wave
+        for (int C = X.c0 + wave; C < c1; C += WAVES) {
+            unsigned long long zn, zs, yn, ys;
+            column_masks(P.codes + tile_index(X.B, C, nblk) * kTileBytes, lane, zn, zs);
+            if (__ballot(zn != 0ull) == 0ull) continue;
+            column_masks(P.codes + tile_index(X.A, C, nblk) * kTileBytes, lane, yn, ys);
+            const uint32_t coff = (uint32_t)(C - X.c0) << 18;
             for (int i = 0; i < kBlock; ++i) {
-                const uint32_t xv = cAB[i * kBlock + lane];
-                const unsigned long long rowmask = __ballot(xv < kCodeZero);
-                if (rowmask == 0ull) continue;
-                const uint32_t yc = cAC[i * kBlock + lane];
-                if (yc >= kCodeZero) continue;
-                const bool yshort = yc < kCodeNear;
-                scalar_u32_ptr xrow = (scalar_u32_ptr)(cAB + i * kBlock);
-                uint32_t xw[16];
-#pragma unroll
-                for (int q = 0; q < 16; ++q) xw[q] = xrow[q];
-                // Phase 1: candidate columns of this lane.
-                uint32_t blo = 0u, bhi = 0u;
-#pragma unroll
-                for (int q = 0; q < 16; ++q) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int j = q * 4 + r;
-                        const uint32_t xc = (xw[q] >> (8 * r)) & 0xFFu;
-                        if (xc < kCodeZero) {
-                            const uint32_t zc = Zc[j];
-                            const bool cand = zc < kCodeZero && (xc < kCodeNear || yshort || zc < kCodeNear);
-                            if (cand) {
-                                if (j < 32) blo |= 1u << j;
-                                else bhi |= 1u << (j - 32);
-                            }
+                const unsigned long long xn = xnear[i];
+                if (xn == 0ull) continue;
+                const unsigned long long xs = xshort[i];
+                const unsigned long long yb = 1ull << i;
+                unsigned long long m = (yn & yb) ? (xn & zn & ((ys & yb) ? ~0ull : (xs | zs))) : 0ull;
+                uint32_t blo = (uint32_t)m, bhi = (uint32_t)(m >> 32);
+                // Append this row's candidates, one per lane per round.
+                for (;;) {
+                    const bool has = (blo | bhi) != 0u;
+                    const unsigned long long act = __ballot(has);
+                    if (act == 0ull) break;
+                    if (has) {
+                        int j;
+                        if (blo) {
+                            j = __builtin_ctz(blo);
+                            blo &= blo - 1u;
+                        } else {
+                            j = 32 + __builtin_ctz(bhi);
+                            bhi &= bhi - 1u;
                         }
+                        const int slot = qn + (int)__builtin_amdgcn_mbcnt_hi(
+                                                  (uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+                        Q[slot] = (uint32_t)i | (uint32_t)j << 6 | (uint32_t)lane << 12 | coff;
                     }
-                }
-                if (!(blo | bhi)) continue;
-                // Phase 2: every candidate triangle of this lane.
-                const float yd = dAC[i * kBlock + lane];
-                while (blo | bhi) {
-                    int j;
-                    if (blo) {
-                        j = __builtin_ctz(blo);
-                        blo &= blo - 1u;
-                    } else {
-                        j = 32 + __builtin_ctz(bhi);
-                        bhi &= bhi - 1u;
-                    }
-                    const uint32_t xc = cAB[i * kBlock + j];
-                    const float xd = dAB[i * kBlock + j];
-                    const uint32_t zc = cBC[j * kBlock + lane];
-                    const float zd = dBC[j * kBlock + lane];
-                    // Sort the sides by FP32 length, carrying their codes.
-                    float l1 = xd, l2 = yd, l3 = zd;
-                    uint32_t k1 = xc, k2 = yc, k3 = zc;
-                    if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
-                    if (l2 > l3) { float t = l2; l2 = l3; l3 = t; uint32_t u = k2; k2 = k3; k3 = u; }
-                    if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
-                    bool unsure = P.force_fp64 != 0;
-                    // Side order certain only when lengths differ by more
-                    // than their FP32 rounding.
-                    if (!(l2 - l1 > 4.f * kU * l2) || !(l3 - l2 > 4.f * kU * l3)) unsure = true;
-                    int cell[3] = {-1, -1, -1};
-                    if (!unsure) {
-                        const float s1 = l1 * l1, s2 = l2 * l2, s3 = l3 * l3;
-                        const float ssum = (s1 + s2) + s3;
-                        if (k1 < kCodeNear) {
-                            cell[0] = entry32((int)k1, l1, l2, (s1 + s2) - s3, (2.f * l1) * l2, ssum, fQ, fA, nq,
-                                              nt, &unsure);
-                            cell[1] = entry32((int)k1, l1, l3, (s1 + s3) - s2, (2.f * l1) * l3, ssum, fQ, fA, nq,
-                                              nt, &unsure);
-                        }
-                        if (k2 < kCodeNear)
-                            cell[2] = entry32((int)k2, l2, l3, (s2 + s3) - s1, (2.f * l2) * l3, ssum, fQ, fA, nq,
-                                              nt, &unsure);
-                    }
-                    if (!unsure) {
-#pragma unroll
-                        for (int e = 0; e < 3; ++e)
-                            if (cell[e] >= 0) inc(hist, cell[e]);
-                    } else {
-                        ++nre;
-                        const int64_t gi = (int64_t)A * kBlock + i, gj = (int64_t)B * kBlock + j,
-                                      gk = (int64_t)C * kBlock + lane;
-                        entries_fp64(sq64f(P.u64, np, gi, gj), sq64f(P.u64, np, gi, gk), sq64f(P.u64, np, gj, gk),
-                                     sT, sQ, sA, ns, nq, nt, hist);
+                    qn += __popcll(act);
+                    if (qn >= kBlock) {
+                        wave_sync();
+                        sqt_drain(P, X, Q, kBlock, lane, sT, sQ, sA, fQ, fA, hist, nre);
+                        // Move the remainder (fewer than 64) to the front.
+                        const int rest = qn - kBlock;
+                        const uint32_t v = lane < rest ? Q[kBlock + lane] : 0u;
+                        wave_sync();
+                        if (lane < rest) Q[lane] = v;
+                        wave_sync();
+                        qn = rest;
                     }
                 }
             }
+        }
+        if (qn) {
+            wave_sync();
+            sqt_drain(P, X, Q, qn, lane, sT, sQ, sA, fQ, fA, hist, nre);
+            wave_sync();
         }
     }
     if (cur >= 0) sqt_flush64(hist, cells, P.hist + (size_t)cur * cells);

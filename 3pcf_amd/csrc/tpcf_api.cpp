@@ -717,16 +717,23 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) c->budget = (uint64_t)(0.9 * (double)fr);
     c->ws.reset(new Workspace());
-    // r123 triplet kernels raise their dynamic LDS limit once.
+    // Kernels with an LDS histogram raise their dynamic LDS limit once, to
+    // what their static LDS leaves of the 160 KiB.
+    std::vector<const void *> lds_kernels{sqt_plain_kernel_ptr(kSqt), sqt_plain_kernel_ptr(kJbj),
+                                          sqt_fast_kernel_ptr()};
     for (int w : {2, 4, 8, 16})
         for (int k : {1, 2, 4, 8})
-            if (const void *fn = r123_triplet_kernel_ptr(w, k))
-                (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
-    for (int p : {kSqt, kJbj})
-        (void)hipFuncSetAttribute(sqt_plain_kernel_ptr(p), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024 - 8 * 1024);
-    (void)hipFuncSetAttribute(sqt_fast_kernel_ptr(), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024 - 10 * 1024);
+            if (const void *fn = r123_triplet_kernel_ptr(w, k)) lds_kernels.push_back(fn);
+    for (const void *fn : lds_kernels) {
+        hipFuncAttributes fa{};
+        if (hipFuncGetAttributes(&fa, fn) != hipSuccess ||
+            hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(160 * 1024 - fa.sharedSizeBytes)) != hipSuccess) {
+            (void)hipGetLastError();
+            tpcf_ctx_destroy(c);
+            return set_error(TPCF_EDEVICE, "cannot raise the dynamic LDS limit of a kernel");
+        }
+    }
     if (const char *env = std::getenv("TPCF_R123_SHAPE")) {
         int w = 0, k = 0, p = 0;
         if (std::sscanf(env, "%d,%d,%d", &w, &k, &p) >= 2) c->shape = LaunchShape{w, k, p};
