@@ -84,52 +84,86 @@ __device__ inline void entries_fp64(double s1, double s2, double s3, const doubl
     }
 }
 
-// Largest m in [-1, n] with e[m] <= v for ascending FP32 edges e[0..n].
-__device__ inline int search_asc32(float v, const float *e, int n) {
-    if (!(v >= e[0])) return -1;
-    int lo = 0, hi = n + 1;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (e[mid] <= v) lo = mid;
-        else hi = mid;
-    }
-    return lo;
+constexpr float kU = 5.9604644775390625e-8f;  // 2^-24, FP32 unit roundoff
+
+// Bin hint on an axis: lin (v - lo) * inv, log (log2(v) - lo) * inv; -1 below,
+// n at or above the range (NaN gives -1).
+__device__ inline int guess_bin(float v, float lo, float inv, int lg, int n) {
+    const float g = (lg ? __log2f(v) - lo : v - lo) * inv;
+    if (!(g >= 0.f)) return -1;
+    if (!(g < (float)n)) return n;
+    return (int)g;
 }
 
-// Largest m in [-1, n] with c <= C[m] for the descending FP32 cosine table.
-__device__ inline int search_desc32(float c, const float *C, int n) {
-    if (!(c <= C[0])) return -1;
-    int lo = 0, hi = n + 1;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (c <= C[mid]) lo = mid;
-        else hi = mid;
-    }
-    return lo;
+// arccos(x)/pi for the theta hint, after Abramowitz & Stegun 4.4.46
+// (Hastings): arccos(x) = sqrt(1-x) * (a0 + a1 x + ... + a7 x^7) on [0,1],
+// absolute error below 2e-8; arccos(-x) = pi - arccos(x).
+__device__ inline float acos_over_pi(float c) {
+    const float x = fminf(fabsf(c), 1.f);
+    float p = -0.0012624911f;
+    p = p * x + 0.0066700901f;
+    p = p * x - 0.0170881256f;
+    p = p * x + 0.0308918810f;
+    p = p * x - 0.0501743046f;
+    p = p * x + 0.0889789874f;
+    p = p * x - 0.2145988016f;
+    p = p * x + 1.5707963050f;
+    const float a = sqrtf(1.f - x) * p * 0.318309886f;
+    return c < 0.f ? 1.f - a : a;
 }
 
-constexpr float kU = 5.9604644775390625e-8f;  // 2^-24
+// Bin m of v against ascending FP32 edges e[0..n] given the hint m in
+// [-1, n]: returns m (or -1 out of range) and sets *unsure unless v is more
+// than its error d (plus the edge's own FP32 rounding) away from every edge
+// that decides it.  A wrong hint always fails the test.
+__device__ inline int bracket_asc(float v, float d, const float *e, int n, int m, bool *unsure) {
+    if (m < 0) {
+        if (!(e[0] - v > d + kU * e[0])) *unsure = true;
+        return -1;
+    }
+    if (m >= n) {
+        if (!(v - e[n] > d + kU * e[n])) *unsure = true;
+        return -1;
+    }
+    const float a = e[m], b = e[m + 1];
+    if (!(v - a > d + kU * a) || !(b - v > d + kU * b)) *unsure = true;
+    return m;
+}
 
-// One entry in FP32.  Returns the histogram cell, -1 for "out of range"
-// (certain), and sets *unsure when some value lies within its error bound of
-// an edge.  Error bounds, with every FP32 side within 2^-25 (relative) of the
-// FP32 rounding of the FP64 side: q = b/a is within 1.6 u of the FP64
-// quotient (6 u used); the cosine-rule value within
-// 3.8 u (sa+sb+sc)/den + 2.6 u |c| (5 u and 4 u used), plus 2 u for the FP32
-// rounding of the table.
-__device__ inline int entry32(int sbin, float a, float b, float num, float den, float ssum, const float *fQ,
-                              const float *fA, int nq, int nt, bool *unsure) {
-    const float q = b / a;
-    const int mq = search_asc32(q, fQ, nq);
-    const float dq = 6.f * kU * q;
-    if (mq >= 0 && !(q - fQ[mq] > dq + kU * fQ[mq])) *unsure = true;
-    if (mq < nq && !(fQ[mq + 1] - q > dq + kU * fQ[mq + 1])) *unsure = true;
-    const float c = num / den;
-    const float dc = 5.f * kU * ssum / den + 4.f * kU * fabsf(c) + 2.f * kU;
-    const int mt = search_desc32(c, fA, nt);
-    if (mt >= 0 && !(fA[mt] - c > dc)) *unsure = true;
-    if (mt < nt && !(c - fA[mt + 1] > dc)) *unsure = true;
-    if (mq < 0 || mq >= nq || mt < 0 || mt >= nt) return -1;
+// Same for the descending cosine table (bin m iff C[m+1] < c <= C[m]); d
+// includes the table's rounding.
+__device__ inline int bracket_desc(float c, float d, const float *C, int n, int m, bool *unsure) {
+    if (m < 0) {
+        if (!(c - C[0] > d)) *unsure = true;
+        return -1;
+    }
+    if (m >= n) {
+        if (!(C[n] - c > d)) *unsure = true;
+        return -1;
+    }
+    if (!(C[m] - c > d) || !(c - C[m + 1] > d)) *unsure = true;
+    return m;
+}
+
+// One entry in FP32 from sides a <= b (rounded FP32 lengths of the FP64
+// sides, each within u of it), ra = 1/a and rden = 1/den from v_rcp_f32
+// (1 ulp = at most 2u).  Error bounds against the FP64 definition:
+//   q = b * ra: within (2u sides + 3u quotient) q, 6u q used;
+//   c = num * rden with num = (sa + sb) - sc, den = 2 a b (squares within 3u):
+//   |c - c64| <= 4u (sa+sb+sc)/den + 7u |c|; 5u, 7u |c| and 2u for the
+//   table's rounding used.
+// Returns the histogram cell, -1 for "out of range" (certain), and sets
+// *unsure when some value lies within its error bound of a deciding edge.
+__device__ inline int entry32(int sbin, float b, float ra, float num, float rden, float ssum, const float *fQ,
+                              const float *fA, const SqtFastArgs &P, bool *unsure) {
+    const int nq = P.nq, nt = P.nt;
+    const float q = b * ra;
+    const int mq = bracket_asc(q, 6.f * kU * q, fQ, nq, guess_bin(q, P.q_lo, P.q_inv, P.q_log, nq), unsure);
+    const float c = num * rden;
+    const float dc = 5.f * kU * ssum * rden + 7.f * kU * fabsf(c) + 2.f * kU;
+    const int mt =
+        bracket_desc(c, dc, fA, nt, guess_bin(acos_over_pi(c), P.t_lo, P.t_inv, P.t_log, nt), unsure);
+    if (mq < 0 || mt < 0) return -1;
     return (sbin * nq + mq) * nt + mt;
 }
 
@@ -224,18 +258,23 @@ __device__ inline void sqt_drain(const SqtFastArgs &P, const SqtItemCtx &X, cons
     const int nq = P.nq, nt = P.nt;
     bool unsure = P.force_fp64 != 0;
     // Side order certain only when lengths differ by more than their FP32
-    // rounding.
-    if (!(l2 - l1 > 4.f * kU * l2) || !(l3 - l2 > 4.f * kU * l3)) unsure = true;
+    // rounding; lengths far outside [1e-18, 1e18] go to FP64 (no FP32
+    // underflow or overflow below).
+    if (!(l2 - l1 > 4.f * kU * l2) || !(l3 - l2 > 4.f * kU * l3) || !(l1 > 1e-18f) || !(l3 < 1e18f)) unsure = true;
     int cell[3] = {-1, -1, -1};
     if (!unsure) {
         const float s1 = l1 * l1, s2 = l2 * l2, s3 = l3 * l3;
         const float ssum = (s1 + s2) + s3;
         if (k1 < kCodeNear) {
-            cell[0] = entry32((int)k1, l1, l2, (s1 + s2) - s3, (2.f * l1) * l2, ssum, fQ, fA, nq, nt, &unsure);
-            cell[1] = entry32((int)k1, l1, l3, (s1 + s3) - s2, (2.f * l1) * l3, ssum, fQ, fA, nq, nt, &unsure);
+            const float r1 = __builtin_amdgcn_rcpf(l1);
+            cell[0] = entry32((int)k1, l2, r1, (s1 + s2) - s3, __builtin_amdgcn_rcpf((2.f * l1) * l2), ssum, fQ, fA,
+                              P, &unsure);
+            cell[1] = entry32((int)k1, l3, r1, (s1 + s3) - s2, __builtin_amdgcn_rcpf((2.f * l1) * l3), ssum, fQ, fA,
+                              P, &unsure);
         }
         if (k2 < kCodeNear)
-            cell[2] = entry32((int)k2, l2, l3, (s2 + s3) - s1, (2.f * l2) * l3, ssum, fQ, fA, nq, nt, &unsure);
+            cell[2] = entry32((int)k2, l3, __builtin_amdgcn_rcpf(l2), (s2 + s3) - s1,
+                              __builtin_amdgcn_rcpf((2.f * l2) * l3), ssum, fQ, fA, P, &unsure);
     }
     if (!unsure) {
 #pragma unroll

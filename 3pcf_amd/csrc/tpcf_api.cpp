@@ -162,6 +162,7 @@ struct JobPlan {
     size_t bytes = 0;
     bool need_tiles = true;
     double far_s = 0;  // SQT FAST: squared sides above this cannot be in any entry
+    tpcf_axis axq{}, axt{};  // SQT: q and theta/pi axes
 };
 
 static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, JobPlan &P) {
@@ -188,6 +189,8 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     if (spec.param != TPCF_R123) {
         P.nq = spec.ax[1].n;
         P.nt = spec.ax[2].n;
+        P.axq = spec.ax[1];
+        P.axt = spec.ax[2];
         P.Eq.resize(P.nq + 1);
         P.Et.resize(P.nt + 1);
         axis_edges(spec.ax[1], P.Eq.data());
@@ -516,6 +519,13 @@ static int launch_sqt_fast_job(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, c
     A.ns = P.n;
     A.nq = P.nq;
     A.nt = P.nt;
+    const auto guess = [](const tpcf_axis &ax, float &lo, float &inv, int &lg) {
+        lg = ax.scale == TPCF_LOG;
+        lo = (float)(lg ? std::log2(ax.lo) : ax.lo);
+        inv = (float)(ax.n / (lg ? std::log2(ax.hi / ax.lo) : ax.hi - ax.lo));
+    };
+    guess(P.axq, A.q_lo, A.q_inv, A.q_log);
+    guess(P.axt, A.t_lo, A.t_inv, A.t_log);
     A.cells = (int)P.cells;
     A.force_fp64 = mode == TPCF_MODE_FAST64;
     A.npad = P.L.npad;

@@ -110,6 +110,11 @@ struct SqtFastArgs {
     int *counter;
     int ns, nq, nt, cells;
     int force_fp64;  // every candidate triangle through the FP64 definition
+    // Bin guesses of the q and theta/pi axes: lin (v - lo) * inv, log
+    // (log2(v) - lo) * inv with lo = log2(axis lo); only a hint, every bin is
+    // confirmed against the edge tables.
+    float q_lo, q_inv, t_lo, t_inv;
+    int q_log, t_log;
     int64_t npad, nblk;
     unsigned long long *hist;       // term t at hist + t*cells
     unsigned long long *rechecked;  // triangles recomputed in FP64

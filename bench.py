@@ -176,10 +176,25 @@ def bench_batch(args) -> int:
     return 0
 
 
+def request_spec(g):
+    """A bin spec uniform over {r123-lin, r123-log, sqt-lin, sqt-log} with
+    8..32 bins per axis."""
+    kind = int(g.integers(0, 4))
+    nb = [int(g.integers(8, 33)) for _ in range(3)]
+    if kind == 0:
+        return tp.r123_spec(nb[0], 0.0, 150.0, "lin"), "r123-lin"
+    if kind == 1:
+        return tp.r123_spec(nb[0], 1.0, 150.0, "log"), "r123-log"
+    scale = "lin" if kind == 2 else "log"
+    s_axis = tp.axis(nb[0], 2.0, 20.0, scale)
+    return tp.binspec(tp.SQT, [s_axis, tp.axis(nb[1], 1.0, 3.0, "lin"), tp.axis(nb[2], 0.0, 1.0, "lin")]), \
+        "sqt-" + scale
+
+
 def request_stream(n: int, n_max: int, seed: int = 0):
     """Poisson arrivals (mean 20 ms apart); catalog size log-uniform in
     [1000, n_max]; a uniform random non-empty subset of the four terms; a bin
-    spec from {r123-lin, r123-log} with 8..32 bins."""
+    spec from request_spec()."""
     g = np.random.Generator(np.random.PCG64(seed))
     t = 0.0
     reqs = []
@@ -188,10 +203,8 @@ def request_stream(n: int, n_max: int, seed: int = 0):
         size = int(round(math.exp(g.uniform(math.log(1000), math.log(n_max)))))
         mask = int(g.integers(1, 16))
         terms = [tp.FOUR_TERMS[b] for b in range(4) if mask >> b & 1]
-        nb = int(g.integers(8, 33))
-        scale = "log" if g.random() < 0.5 else "lin"
-        spec = tp.r123_spec(nb, 1.0, 150.0, scale) if scale == "log" else tp.r123_spec(nb, 0.0, 150.0, "lin")
-        reqs.append((t, size, terms, spec, 1000 + k))
+        spec, kind = request_spec(g)
+        reqs.append((t, size, terms, spec, 1000 + k, kind))
     return reqs
 
 
@@ -200,7 +213,7 @@ def bench_serve(args) -> int:
     with tp.Context(0) as ctx:
         jobs = []
         keep = []
-        for (_, size, terms, spec, seed) in reqs:
+        for (_, size, terms, spec, seed, _) in reqs:
             d = synth.neyman_scott(size, nclust=40, sigma=2.0, size=100.0, seed=seed)
             r = synth.uniform_cube(size, 100.0, seed=seed + 7)
             cd, cr = ctx.catalog(d), ctx.catalog(r)
@@ -211,7 +224,7 @@ def bench_serve(args) -> int:
         ctx.wait(t)
         start = time.perf_counter()
         tickets = []
-        for (arr, _, _, spec, _), job in zip(reqs, jobs):
+        for (arr, _, _, spec, _, _), job in zip(reqs, jobs):
             now = time.perf_counter() - start
             if arr > now:
                 time.sleep(arr - now)
@@ -228,7 +241,7 @@ def bench_serve(args) -> int:
         plain_lat = []
         exact = True
         tot_trip = 0
-        for (arr, _, _, spec, _), job, o in zip(reqs, jobs, outs):
+        for (arr, _, _, spec, _, _), job, o in zip(reqs, jobs, outs):
             t1 = time.perf_counter()
             out, st = ctx.count(job, spec, tp.MODE_PLAIN)
             dt = time.perf_counter() - t1
@@ -252,7 +265,9 @@ def bench_serve(args) -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{n} requests, Poisson arrivals (mean 50/s), N log-uniform in [1000,{args.n_max}] data + "
-                        "as many randoms, random non-empty term subset, r123 lin/log spec with 8-32 bins",
+                        "as many randoms, random non-empty term subset, spec uniform over r123-lin/r123-log/"
+                        "sqt-lin/sqt-log with 8-32 bins per axis",
+            "spec_mix": {k: sum(1 for q in reqs if q[5] == k) for k in ("r123-lin", "r123-log", "sqt-lin", "sqt-log")},
             "triplets": tot_trip,
             "latency_p50_ms": pct(lat, 50) * 1e3,
             "latency_p99_ms": pct(lat, 99) * 1e3,
@@ -260,6 +275,151 @@ def bench_serve(args) -> int:
             "plain_latency_p50_ms": pct(plain_lat, 50) * 1e3,
             "plain_latency_p99_ms": pct(plain_lat, 99) * 1e3,
             "histograms_identical_to_plain": exact,
+        },
+    }
+    print(json.dumps(line))
+    return 0
+
+
+# (s,q,theta): nominal work per triplet of the entry arithmetic (3 entries x
+# {q ratio test 4, cosine-rule value 5, edge search 1, index 2} lane-ops).
+SQT_OPS_PER_TRIPLET = 36.0
+
+
+def bench_sqt(args) -> int:
+    """Four-term (s,q,theta) histograms: 10k clustered data + 10k uniform
+    randoms in a 200 Mpc/h cube, s in [2,20) 10 log bins, q in [1,3) 8 bins,
+    theta/pi in [0,1) 16 bins; FAST (FP32 candidate arithmetic with FP64
+    recheck) against FAST64 and the FP64 plain kernel, all exact."""
+    n = args.sqt_n
+    d = synth.neyman_scott(n, nclust=200, sigma=2.0, size=200.0, seed=0)
+    r = synth.uniform_cube(n, 200.0, seed=1)
+    spec = tp.binspec(tp.SQT, [tp.axis(10, 2.0, 20.0, "log"), tp.axis(8, 1.0, 3.0, "lin"),
+                               tp.axis(16, 0.0, 1.0, "lin")])
+    reps = max(1, args.steps)
+    with tp.Context(0) as ctx:
+        cd, cr = ctx.catalog(d), ctx.catalog(r)
+        job = tp.four_term_job(cd, cr)
+        for _ in range(args.warmup):
+            ctx.count(job, spec, tp.MODE_FAST)
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out, st = ctx.count(job, spec, tp.MODE_FAST)
+            stats.append(st)
+        dt = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        out64, _ = ctx.count(job, spec, tp.MODE_FAST64)
+        dt64 = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        plain, _ = ctx.count(job, spec, tp.MODE_PLAIN)
+        dtp = time.perf_counter() - t1
+    triplets = stats[0].triplets
+    tri_s = sum(x.triplet_ms for x in stats) / len(stats) / 1e3
+    value = triplets * reps / dt
+    entries = int(out.sum())
+    achieved = SQT_OPS_PER_TRIPLET * triplets / tri_s
+    line = {
+        "metric": "(s,q,theta) four-term triplets/s, FAST vs plain FP64, exact",
+        "value": value,
+        "unit": "triplets/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / reps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 pair codes, f32 entries with f64 recheck, u32/u64 counts",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{n // 1000}k Neyman-Scott data (200 clusters, sigma 2 Mpc/h) + {n // 1000}k uniform "
+                        "randoms in a 200 Mpc/h cube, DDD/DDR/DRR/RRR, s [2,20) 10 log, q [1,3) 8 lin, "
+                        "theta/pi [0,1) 16 lin",
+            "triplets_per_step": triplets,
+            "histogram_entries": entries,
+            "triangles_rechecked_fp64": stats[0].pairs_rechecked,
+            "triplet_kernel_ms": stats[0].triplet_ms,
+            "pair_stage_ms": stats[0].pair_ms,
+            "fast64_s": dt64,
+            "plain_fp64_s": dtp,
+            "plain_fp64_triplets_per_s": triplets / dtp,
+            "speedup_vs_plain": dtp / (dt / reps),
+            "fast_equals_fast64": bool(np.array_equal(out, out64)),
+            "fast_equals_plain": bool(np.array_equal(out, plain)),
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved / 1e12,
+            "peak": VALU_PEAK / 1e12,
+            "unit": "TOP/s (VALU lane-ops, 36 nominal per triplet; the bit-mask filter settles most "
+                    "triplets without the entry arithmetic, so the fraction can exceed 1)",
+            "frac": achieved / VALU_PEAK,
+            "traffic": None,
+        },
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def bench_precision(args) -> int:
+    """Precision form: 20k clustered data + 20k uniform randoms in a
+    10x10 deg wedge (r 500-800 Mpc/h), 20 log bins on [1,50) Mpc/h, recheck
+    window 1e-5; FAST (FP32 pair stage + FP64 recheck) against FAST64 (FP64
+    pair stage), histograms identical, recheck fraction reported."""
+    n = args.precision_n
+    d = synth.wedge(n, 10.0, 10.0, 500.0, 800.0, seed=0, clustered=True, nclust=200)
+    r = synth.wedge(n, 10.0, 10.0, 500.0, 800.0, seed=1)
+    spec = tp.r123_spec(20, 1.0, 50.0, "log", 1e-5)
+    reps = max(1, args.steps)
+    with tp.Context(0) as ctx:
+        cd, cr = ctx.catalog(d), ctx.catalog(r)
+        job = tp.four_term_job(cd, cr)
+        for _ in range(args.warmup):
+            ctx.count(job, spec, tp.MODE_FAST)
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out, st = ctx.count(job, spec, tp.MODE_FAST)
+            stats.append(st)
+        dt = time.perf_counter() - t0
+        out64, st64 = ctx.count(job, spec, tp.MODE_FAST64)
+    triplets = stats[0].triplets
+    tri_s = sum(x.triplet_ms for x in stats) / len(stats) / 1e3
+    achieved = OPS_PER_TRIPLET * triplets / tri_s
+    line = {
+        "metric": "precision form triplets/s (FP32 pair stage + FP64 recheck), exact vs FP64",
+        "value": triplets * reps / dt,
+        "unit": "triplets/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / reps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8 pair bins, u32/u64 counts (pairs fp32 + fp64 recheck)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{n // 1000}k clustered data + {n // 1000}k uniform randoms in a 10x10 deg wedge, "
+                        "r 500-800 Mpc/h, four terms, 20 log bins on [1,50) Mpc/h, recheck window 1e-5",
+            "triplets_per_step": triplets,
+            "triplets_in_range": int(out.sum()),
+            "pairs": stats[0].pairs,
+            "pairs_rechecked": stats[0].pairs_rechecked,
+            "pairs_rechecked_fraction": stats[0].pairs_rechecked / stats[0].pairs if stats[0].pairs else 0.0,
+            "pair_stage_ms": stats[0].pair_ms,
+            "pair_stage_fp64_ms": st64.pair_ms,
+            "triplet_kernel_ms": stats[0].triplet_ms,
+            "fast_equals_fast64": bool(np.array_equal(out, out64)),
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved / 1e12,
+            "peak": VALU_PEAK / 1e12,
+            "unit": "TOP/s (VALU lane-ops, 6 per triplet)",
+            "frac": achieved / VALU_PEAK,
+            "traffic": None,
         },
     }
     print(json.dumps(line))
@@ -275,10 +435,14 @@ def main(argv=None) -> int:
     ap.add_argument("--nr", type=int, default=50000)
     ap.add_argument("--mode", choices=sorted(tp.MODES), default="fast")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["headline", "batch", "serve"], default="headline",
+    ap.add_argument("--workload", choices=["headline", "batch", "serve", "sqt", "precision"], default="headline",
                     help="headline: 50k+50k four terms (default); batch: 64 angular voxels of 150k+150k in one "
                          "prepared, graph-replayed persistent launch vs one FP64 plain job at a time; serve: a "
-                         "Poisson request stream through the serving queue vs plain one-at-a-time")
+                         "Poisson request stream through the serving queue vs plain one-at-a-time; sqt: 10k+10k "
+                         "four-term (s,q,theta) FAST vs plain FP64; precision: 20k+20k wedge, FP32 pair stage "
+                         "with FP64 recheck vs FP64 pair stage")
+    ap.add_argument("--sqt-n", type=int, default=10000, help="sqt: data (and randoms) catalog size")
+    ap.add_argument("--precision-n", type=int, default=20000, help="precision: data (and randoms) catalog size")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--n-max", type=int, default=8000, help="serve: largest catalog size of a request")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -304,6 +468,10 @@ def main(argv=None) -> int:
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "sqt":
+        return bench_sqt(args)
+    if args.workload == "precision":
+        return bench_precision(args)
 
     d, r = headline_catalogs(args.nd, args.nr)
     spec = headline_spec()
