@@ -308,12 +308,15 @@ def bench_sqt(args) -> int:
             out, st = ctx.count(job, spec, tp.MODE_FAST)
             stats.append(st)
         dt = time.perf_counter() - t0
-        t1 = time.perf_counter()
-        out64, _ = ctx.count(job, spec, tp.MODE_FAST64)
-        dt64 = time.perf_counter() - t1
-        t1 = time.perf_counter()
-        plain, _ = ctx.count(job, spec, tp.MODE_PLAIN)
-        dtp = time.perf_counter() - t1
+        out64 = plain = None
+        dt64 = dtp = None
+        if not args.no_plain:
+            t1 = time.perf_counter()
+            out64, _ = ctx.count(job, spec, tp.MODE_FAST64)
+            dt64 = time.perf_counter() - t1
+            t1 = time.perf_counter()
+            plain, _ = ctx.count(job, spec, tp.MODE_PLAIN)
+            dtp = time.perf_counter() - t1
     triplets = stats[0].triplets
     tri_s = sum(x.triplet_ms for x in stats) / len(stats) / 1e3
     value = triplets * reps / dt
@@ -343,10 +346,10 @@ def bench_sqt(args) -> int:
             "pair_stage_ms": stats[0].pair_ms,
             "fast64_s": dt64,
             "plain_fp64_s": dtp,
-            "plain_fp64_triplets_per_s": triplets / dtp,
-            "speedup_vs_plain": dtp / (dt / reps),
-            "fast_equals_fast64": bool(np.array_equal(out, out64)),
-            "fast_equals_plain": bool(np.array_equal(out, plain)),
+            "plain_fp64_triplets_per_s": triplets / dtp if dtp else None,
+            "speedup_vs_plain": dtp / (dt / reps) if dtp else None,
+            "fast_equals_fast64": bool(np.array_equal(out, out64)) if out64 is not None else None,
+            "fast_equals_plain": bool(np.array_equal(out, plain)) if plain is not None else None,
         },
         "roofline": {
             "bound": "valu",
@@ -441,6 +444,8 @@ def main(argv=None) -> int:
                          "Poisson request stream through the serving queue vs plain one-at-a-time; sqt: 10k+10k "
                          "four-term (s,q,theta) FAST vs plain FP64; precision: 20k+20k wedge, FP32 pair stage "
                          "with FP64 recheck vs FP64 pair stage")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="sqt: skip the FAST64 and FP64 plain comparison runs (for profiler runs)")
     ap.add_argument("--sqt-n", type=int, default=10000, help="sqt: data (and randoms) catalog size")
     ap.add_argument("--precision-n", type=int, default=20000, help="precision: data (and randoms) catalog size")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")

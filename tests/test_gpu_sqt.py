@@ -97,3 +97,31 @@ def test_jbj_fast_mode_refused(ctx):
     d = synth.uniform_cube(20, 8.0, seed=25)
     with pytest.raises(tp.TpcfError):
         run(ctx, [d], [(0, 0, 0)], tp.JBJ, JBJ_SPEC, mode=tp.MODE_FAST)
+
+
+def _tri_q_theta(p, i, j, k):
+    """FP64 q = l2/l1 and theta/pi of the entry at the shortest side, by the
+    definition's arithmetic (sorted squared sides, cosine rule)."""
+    def sq(a, b):
+        dx, dy, dz = p[a] - p[b]
+        return (dx * dx + dy * dy) + dz * dz
+    s1, s2, s3 = sorted([sq(i, j), sq(i, k), sq(j, k)])
+    l1, l2 = np.sqrt(s1), np.sqrt(s2)
+    c = ((s1 + s2) - s3) / ((2.0 * l1) * l2)
+    return l2 / l1, float(np.arccos(c) / np.pi)
+
+
+@pytest.mark.parametrize("mode", [tp.MODE_FAST, tp.MODE_FAST64])
+def test_sqt_fast_edges_on_data(ctx, mode):
+    """q and theta/pi edges placed exactly on values of triangles in the
+    catalog: ties and near-ties at the edge must be settled as the FP64
+    definition settles them."""
+    print(f"case sqt edges on data mode={mode}", flush=True)
+    d = synth.uniform_cube(96, 10.0, seed=31)
+    q0, t0 = _tri_q_theta(d, 3, 17, 42)
+    q1, t1 = _tri_q_theta(d, 5, 8, 60)
+    for axes in ([(12, 0.5, 9.0, "lin"), (8, q0, q0 + 2.0, "lin"), (10, t0, min(1.0, t0 + 0.5), "lin")],
+                 [(12, 0.5, 9.0, "log"), (6, q1, q1 * 3.0, "log"), (9, t1 * 0.5, t1, "lin")]):
+        got, st = run(ctx, [d], [(0, 0, 0)], tp.SQT, axes, mode)
+        assert np.array_equal(got, oracle([d], [(0, 0, 0)], orc.SQT, axes))
+        assert int(got.sum()) > 0
