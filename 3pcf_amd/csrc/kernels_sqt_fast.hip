@@ -290,6 +290,15 @@ __device__ inline void sqt_drain(const SqtFastArgs &P, const SqtItemCtx &X, cons
 
 }  // namespace
 
+// Candidate columns j of row i for this lane: sides (i,j), (i,k), (j,k) all
+// near and at least one with an s bin.
+__device__ inline unsigned long long row_candidates(unsigned long long xn, unsigned long long xs, int i,
+                                                    unsigned long long yn, unsigned long long ys,
+                                                    unsigned long long zn, unsigned long long zs) {
+    const unsigned long long yb = 1ull << i;
+    return (yn & yb) ? (xn & zn & ((ys & yb) ? ~0ull : (xs | zs))) : 0ull;
+}
+
 // Bits m of the lane's 64-bit words: code(m, lane) of a tile below kCodeZero
 // (near) and below kCodeNear (has an s bin); t points at the tile.
 __device__ inline void column_masks(const uint8_t *t, int lane, unsigned long long &nearm,
@@ -382,42 +391,77 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
             if (__ballot(zn != 0ull) == 0ull) continue;
             column_masks(P.codes + tile_index(X.A, C, nblk) * kTileBytes, lane, yn, ys);
             const uint32_t coff = (uint32_t)(C - X.c0) << 18;
+            // Pass 1: the rows holding at least one candidate of this lane,
+            // the lane's candidate count, and the rows nonempty for any lane.
+            unsigned long long rows = 0ull;
+            uint32_t tot = 0u;
+            int n_any = 0;
+#pragma unroll 8
             for (int i = 0; i < kBlock; ++i) {
-                const unsigned long long xn = xnear[i];
-                if (xn == 0ull) continue;
-                const unsigned long long xs = xshort[i];
-                const unsigned long long yb = 1ull << i;
-                unsigned long long m = (yn & yb) ? (xn & zn & ((ys & yb) ? ~0ull : (xs | zs))) : 0ull;
-                uint32_t blo = (uint32_t)m, bhi = (uint32_t)(m >> 32);
-                // Append this row's candidates, one per lane per round.
+                const unsigned long long m = row_candidates(xnear[i], xshort[i], i, yn, ys, zn, zs);
+                rows |= (unsigned long long)(m != 0ull) << i;
+                tot += (uint32_t)__popcll(m);
+                n_any += __ballot(m != 0ull) != 0ull;
+            }
+            uint32_t max_tot = tot;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) max_tot = max(max_tot, (uint32_t)__shfl_xor((int)max_tot, off));
+            max_tot = __builtin_amdgcn_readfirstlane(max_tot);
+            // Appends one entry per lane with has set (rows ri, columns j);
+            // drains the queue whenever 64 entries are waiting.  false when
+            // no lane had an entry.
+            auto push = [&](bool has, int ri, int j) -> bool {
+                const unsigned long long act = __ballot(has);
+                if (act == 0ull) return false;
+                if (has) {
+                    const int slot = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+                    Q[slot] = (uint32_t)ri | (uint32_t)j << 6 | (uint32_t)lane << 12 | coff;
+                }
+                qn += __popcll(act);
+                if (qn >= kBlock) {
+                    wave_sync();
+                    sqt_drain(P, X, Q, kBlock, lane, sT, sQ, sA, fQ, fA, hist, nre);
+                    // Move the remainder (fewer than 64) to the front.
+                    const int rest = qn - kBlock;
+                    const uint32_t v = lane < rest ? Q[kBlock + lane] : 0u;
+                    wave_sync();
+                    if (lane < rest) Q[lane] = v;
+                    wave_sync();
+                    qn = rest;
+                }
+                return true;
+            };
+            if (max_tot <= 2u * (uint32_t)n_any) {
+                // Sparse: one round per entry of the busiest lane over the
+                // whole C block (a row-by-row append needs at least one round
+                // per nonempty row, plus one to see it empty); a lane whose
+                // current row is used up moves on to its next row, which costs
+                // a reload in most rounds (factor 2 chosen by timing sparse
+                // and dense catalogs).
+                unsigned long long rm = 0ull;
+                int ri = 0;
                 for (;;) {
-                    const bool has = (blo | bhi) != 0u;
-                    const unsigned long long act = __ballot(has);
-                    if (act == 0ull) break;
-                    if (has) {
-                        int j;
-                        if (blo) {
-                            j = __builtin_ctz(blo);
-                            blo &= blo - 1u;
-                        } else {
-                            j = 32 + __builtin_ctz(bhi);
-                            bhi &= bhi - 1u;
-                        }
-                        const int slot = qn + (int)__builtin_amdgcn_mbcnt_hi(
-                                                  (uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
-                        Q[slot] = (uint32_t)i | (uint32_t)j << 6 | (uint32_t)lane << 12 | coff;
+                    if (rm == 0ull && rows != 0ull) {
+                        ri = __builtin_ctzll(rows);
+                        rows &= rows - 1ull;
+                        rm = row_candidates(xnear[ri], xshort[ri], ri, yn, ys, zn, zs);
                     }
-                    qn += __popcll(act);
-                    if (qn >= kBlock) {
-                        wave_sync();
-                        sqt_drain(P, X, Q, kBlock, lane, sT, sQ, sA, fQ, fA, hist, nre);
-                        // Move the remainder (fewer than 64) to the front.
-                        const int rest = qn - kBlock;
-                        const uint32_t v = lane < rest ? Q[kBlock + lane] : 0u;
-                        wave_sync();
-                        if (lane < rest) Q[lane] = v;
-                        wave_sync();
-                        qn = rest;
+                    const bool has = rm != 0ull;
+                    const int j = has ? __builtin_ctzll(rm) : 0;
+                    rm &= rm - 1ull;
+                    if (!push(has, ri, j)) break;
+                }
+            } else {
+                // Dense: row by row (every lane refills from the same row).
+                for (int i = 0; i < kBlock; ++i) {
+                    if (__ballot((rows >> i) & 1ull) == 0ull) continue;
+                    unsigned long long rm = row_candidates(xnear[i], xshort[i], i, yn, ys, zn, zs);
+                    for (;;) {
+                        const bool has = rm != 0ull;
+                        const int j = has ? __builtin_ctzll(rm) : 0;
+                        rm &= rm - 1ull;
+                        if (!push(has, i, j)) break;
                     }
                 }
             }
