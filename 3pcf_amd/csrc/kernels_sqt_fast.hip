@@ -3,22 +3,21 @@
 // Pair stage (sqt_pairs_kernel): for every pair of a job's union layout, in
 // FP64, the squared side s, its exact s bin against the squared thresholds,
 // and a code: the s bin, kCodeNear (no s bin but short enough to be a q or
-// angle side), kCodeFar (longer than s_hi*(1+q_hi): no entry of any triangle
-// containing it can be in range) or kCodeZero (coincident points); plus the
-// side length rounded to FP32.
+// angle side), kCodeFar (longer than s_hi*max(q_hi,1): no entry of any
+// triangle containing it can be in range) or kCodeZero (coincident points);
+// the side length rounded to FP32; and per tile the 64-bit row and column
+// masks of "near" and "has an s bin".
 //
 // Triplet stage (sqt_fast_kernel): persistent over (term, A, B, C-range)
-// items like the (r1,r2,r3) kernel.  Per row i of block A, phase 1 walks the
-// 64 columns j of block B with x = code(i,j) uniform and z = code(j,k)
-// preloaded per lane, and marks candidate triangles (no far or zero side, at
-// least one side with an s bin) in a per-lane 64-bit mask.  Phase 2 pulls
-// every lane's candidates in turn, so lanes with candidates run the entry
-// arithmetic together: sides sorted in FP32, q and the cosine-rule value in
-// FP32 with rigorous error bounds, bins from the FP32 copies of the host
-// tables.  A triangle whose side order is not certain, or any of whose values
-// lies within its error bound of an edge, is recomputed from the FP64
-// coordinates with the same code as the plain kernel, so the histograms equal
-// the FP64 definition bit for bit.
+// items like the (r1,r2,r3) kernel.  Candidate triangles (no far or zero
+// side, at least one side with an s bin) come from AND/OR of the masks, 64
+// columns at a time; they are appended to a per-wave LDS queue and drained 64
+// at a time, so lanes run the entry arithmetic together: sides sorted in
+// FP32, q and the cosine-rule value in FP32 with rigorous error bounds, bins
+// confirmed against the FP32 copies of the host tables.  A triangle whose side
+// order is not certain, or any of whose values lies within its error bound of
+// an edge, is recomputed from the FP64 coordinates with the same code as the
+// plain kernel, so the histograms equal the FP64 definition bit for bit.
 #include "tpcf_internal.h"
 
 namespace tpcf {
@@ -187,13 +186,16 @@ __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
     const int64_t R = blockIdx.y, C = blockIdx.x;
     if (C < R) return;
     __shared__ double sT[kMaxBins + 1];
+    __shared__ unsigned long long smask[kMaskWords];
     const int t = threadIdx.x;
     for (int m = t; m <= J.ns; m += 256) sT[m] = J.Ts[m];
+    for (int m = t; m < kMaskWords; m += 256) smask[m] = 0ull;
     __syncthreads();
     const int cntR = J.blk_count[R], cntC = J.blk_count[C];
     const int i = t >> 2, jb = (t & 3) * 16;
     const int64_t gi = R * kBlock + i;
     uint32_t packed[4] = {0u, 0u, 0u, 0u};
+    uint32_t rn = 0u, rs = 0u;  // this thread's 16 columns: near / with an s bin
     float dist[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -212,13 +214,25 @@ __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
         }
         dist[q] = d;
         packed[q >> 2] |= code << (8 * (q & 3));
+        if (code < kCodeZero) {
+            rn |= 1u << q;
+            atomicOr(&smask[kMaskColNear + j], 1ull << i);
+        }
+        if (code < kCodeNear) {
+            rs |= 1u << q;
+            atomicOr(&smask[kMaskColShort + j], 1ull << i);
+        }
     }
+    if (rn) atomicOr(&smask[kMaskRowNear + i], (unsigned long long)rn << jb);
+    if (rs) atomicOr(&smask[kMaskRowShort + i], (unsigned long long)rs << jb);
     const int64_t tile = tile_index(R, C, J.nblk);
     *reinterpret_cast<uint4 *>(J.codes + tile * kTileBytes + i * kBlock + jb) =
         make_uint4(packed[0], packed[1], packed[2], packed[3]);
     float4 *dd = reinterpret_cast<float4 *>(J.dists + tile * kTileBytes + i * kBlock + jb);
 #pragma unroll
     for (int q = 0; q < 4; ++q) dd[q] = make_float4(dist[4 * q], dist[4 * q + 1], dist[4 * q + 2], dist[4 * q + 3]);
+    __syncthreads();
+    for (int m = t; m < kMaskWords; m += 256) J.masks[tile * kMaskWords + m] = smask[m];
 }
 
 namespace {
@@ -236,22 +250,46 @@ struct SqtItemCtx {
     int A, B, c0;
 };
 
-// Entries of the queued candidate triangles Q[0..cnt), one per lane.  An
-// entry packs i | j << 6 | k << 12 | (C - c0) << 18 for rows i of block A,
-// j of block B, k of block C.
-__device__ inline void sqt_drain(const SqtFastArgs &P, const SqtItemCtx &X, const uint32_t *Q, int cnt, int lane,
-                                 const double *sT, const double *sQ, const double *sA, const float *fQ,
-                                 const float *fA, uint32_t *hist, unsigned long long &nre) {
-    if (lane >= cnt) return;
-    const uint32_t e = Q[lane];
-    const int i = e & 63, j = (e >> 6) & 63, k = (e >> 12) & 63, C = X.c0 + (int)(e >> 18);
-    const int64_t tAC = tile_index(X.A, C, P.nblk) * kTileBytes;
-    const int64_t tBC = tile_index(X.B, C, P.nblk) * kTileBytes;
-    const uint32_t xc = X.cAB[i * kBlock + j], yc = P.codes[tAC + i * kBlock + k], zc = P.codes[tBC + j * kBlock + k];
-    const float xd = X.dAB[i * kBlock + j], yd = P.dists[tAC + i * kBlock + k], zd = P.dists[tBC + j * kBlock + k];
+// A queued candidate triangle with the codes and FP32 lengths of its sides
+// (loads in flight until first used).
+struct SqtPending {
+    uint32_t e;  // i | j << 6 | k << 12 | (C - c0) << 18 (rows of blocks A, B, C)
+    uint32_t xc, yc, zc;
+    float xd, yd, zd;
+    bool live;
+};
+
+// Starts loading the sides of ring entries head .. head+cnt-1, one per lane.
+__device__ inline SqtPending sqt_gather(const SqtFastArgs &P, const SqtItemCtx &X, const uint32_t *Q,
+                                        uint32_t head, uint32_t qmask, int cnt, int lane) {
+    SqtPending p;
+    p.live = lane < cnt;
+    p.e = p.live ? Q[(head + (uint32_t)lane) & qmask] : 0u;
+    p.xc = p.yc = p.zc = kCodeFar;
+    p.xd = p.yd = p.zd = 0.f;
+    if (p.live) {
+        const uint32_t e = p.e;
+        const int i = e & 63, j = (e >> 6) & 63, k = (e >> 12) & 63, C = X.c0 + (int)(e >> 18);
+        const int64_t tAC = tile_index(X.A, C, P.nblk) * kTileBytes;
+        const int64_t tBC = tile_index(X.B, C, P.nblk) * kTileBytes;
+        p.xc = X.cAB[i * kBlock + j];
+        p.yc = P.codes[tAC + i * kBlock + k];
+        p.zc = P.codes[tBC + j * kBlock + k];
+        p.xd = X.dAB[i * kBlock + j];
+        p.yd = P.dists[tAC + i * kBlock + k];
+        p.zd = P.dists[tBC + j * kBlock + k];
+    }
+    return p;
+}
+
+// Histogram entries of a gathered candidate (FP32 with FP64 recheck).
+__device__ inline void sqt_entries(const SqtFastArgs &P, const SqtItemCtx &X, const SqtPending &p,
+                                   const double *sT, const double *sQ, const double *sA, const float *fQ,
+                                   const float *fA, uint32_t *hist, unsigned long long &nre) {
+    if (!p.live) return;
     // Sort the sides by FP32 length, carrying their codes.
-    float l1 = xd, l2 = yd, l3 = zd;
-    uint32_t k1 = xc, k2 = yc, k3 = zc;
+    float l1 = p.xd, l2 = p.yd, l3 = p.zd;
+    uint32_t k1 = p.xc, k2 = p.yc, k3 = p.zc;
     if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
     if (l2 > l3) { float t = l2; l2 = l3; l3 = t; uint32_t u = k2; k2 = k3; k3 = u; }
     if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
@@ -282,6 +320,8 @@ __device__ inline void sqt_drain(const SqtFastArgs &P, const SqtItemCtx &X, cons
             if (cell[m] >= 0) inc(hist, cell[m]);
     } else {
         ++nre;
+        const uint32_t e = p.e;
+        const int i = e & 63, j = (e >> 6) & 63, k = (e >> 12) & 63, C = X.c0 + (int)(e >> 18);
         const int64_t gi = (int64_t)X.A * kBlock + i, gj = (int64_t)X.B * kBlock + j, gk = (int64_t)C * kBlock + k;
         entries_fp64(sq64f(P.u64, P.npad, gi, gj), sq64f(P.u64, P.npad, gi, gk), sq64f(P.u64, P.npad, gj, gk), sT,
                      sQ, sA, P.ns, nq, nt, hist);
@@ -289,6 +329,19 @@ __device__ inline void sqt_drain(const SqtFastArgs &P, const SqtItemCtx &X, cons
 }
 
 }  // namespace
+
+// Inclusive sum of x over the lanes 0..lane of the wave: DPP shifts within
+// each row of 16 lanes, then the totals of rows 0..2 (read from lanes 15, 31,
+// 47) added to the rows above them.
+__device__ inline uint32_t wave_inclusive_sum(uint32_t x, int lane) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    const uint32_t r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31),
+                   r2 = __builtin_amdgcn_readlane(x, 47);
+    return x + (lane >= 16 ? r0 : 0u) + (lane >= 32 ? r1 : 0u) + (lane >= 48 ? r2 : 0u);
+}
 
 // Candidate columns j of row i for this lane: sides (i,j), (i,k), (j,k) all
 // near and at least one with an s bin.
@@ -299,29 +352,12 @@ __device__ inline unsigned long long row_candidates(unsigned long long xn, unsig
     return (yn & yb) ? (xn & zn & ((ys & yb) ? ~0ull : (xs | zs))) : 0ull;
 }
 
-// Bits m of the lane's 64-bit words: code(m, lane) of a tile below kCodeZero
-// (near) and below kCodeNear (has an s bin); t points at the tile.
-__device__ inline void column_masks(const uint8_t *t, int lane, unsigned long long &nearm,
-                                    unsigned long long &shortm) {
-    uint32_t n0 = 0u, n1 = 0u, s0 = 0u, s1 = 0u;
-#pragma unroll
-    for (int m = 0; m < 32; ++m) {
-        const uint32_t a = t[m * kBlock + lane], b = t[(m + 32) * kBlock + lane];
-        n0 |= (uint32_t)(a < kCodeZero) << m;
-        s0 |= (uint32_t)(a < kCodeNear) << m;
-        n1 |= (uint32_t)(b < kCodeZero) << m;
-        s1 |= (uint32_t)(b < kCodeNear) << m;
-    }
-    nearm = (unsigned long long)n1 << 32 | n0;
-    shortm = (unsigned long long)s1 << 32 | s0;
-}
-
 // Persistent over (term, A, B, C-range) items.  Candidate triangles (no far
-// or zero side, at least one side with an s bin) are found with bit masks:
-// per item, for every row i of block A, the columns j whose side (i,j) is
-// near / has an s bin (two ballots per row, shared in LDS); per C block and
-// lane k, the rows j of side (j,k) and the rows i of side (i,k) likewise
-// (64-bit words in VGPRs).  A row's candidates for lane k are then
+// or zero side, at least one side with an s bin) are found with the pair
+// stage's bit masks: per item, for every row i of block A, the columns j whose
+// side (i,j) is near / has an s bin (shared in LDS); per C block and lane k,
+// the rows j of side (j,k) and the rows i of side (i,k) likewise (64-bit
+// words in VGPRs).  A row's candidates for lane k are then
 //   X_near[i] & Z_near & Y_near(i) & (X_short[i] | Z_short | Y_short(i)).
 // Each wave takes the C blocks c0 + wave, c0 + wave + WAVES, ... and appends
 // its candidates to an LDS queue, drained 64 at a time with every lane busy.
@@ -330,7 +366,6 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
     extern __shared__ uint32_t hist[];
     __shared__ double sT[kMaxBins + 1], sQ[kMaxBins + 1], sA[kMaxBins + 1];
     __shared__ float fQ[kMaxBins + 1], fA[kMaxBins + 1];
-    __shared__ uint32_t queue[WAVES][2 * kBlock];
     __shared__ unsigned long long xnear[kBlock], xshort[kBlock];
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
@@ -348,7 +383,8 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         }
     }
     for (int c = threadIdx.x; c < cells; c += blockDim.x) hist[c] = 0u;
-    uint32_t *Q = queue[wave];
+    const uint32_t qcap = (uint32_t)P.qcap, qmask = qcap - 1u;
+    uint32_t *Q = hist + ((cells + 3) & ~3) + wave * qcap;
     int cur = -1;
     unsigned long long since = 0, nre = 0;
     const int64_t nblk = P.nblk;
@@ -375,100 +411,84 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         const int64_t tAB = tile_index(X.A, X.B, nblk) * kTileBytes;
         X.cAB = P.codes + tAB;
         X.dAB = P.dists + tAB;
-        for (int i = wave; i < kBlock; i += WAVES) {
-            const uint32_t xv = X.cAB[i * kBlock + lane];
-            const unsigned long long n = __ballot(xv < kCodeZero), sh = __ballot(xv < kCodeNear);
-            if (lane == 0) {
-                xnear[i] = n;
-                xshort[i] = sh;
-            }
+        const unsigned long long *mAB = P.masks + tile_index(X.A, X.B, nblk) * kMaskWords;
+        for (int i = threadIdx.x; i < kBlock; i += blockDim.x) {
+            xnear[i] = mAB[kMaskRowNear + i];
+            xshort[i] = mAB[kMaskRowShort + i];
         }
         __syncthreads();
-        int qn = 0;  // queued entries of this wave
-        for (int C = X.c0 + wave; C < c1; C += WAVES) {
-            unsigned long long zn, zs, yn, ys;
-            column_masks(P.codes + tile_index(X.B, C, nblk) * kTileBytes, lane, zn, zs);
-            if (__ballot(zn != 0ull) == 0ull) continue;
-            column_masks(P.codes + tile_index(X.A, C, nblk) * kTileBytes, lane, yn, ys);
-            const uint32_t coff = (uint32_t)(C - X.c0) << 18;
-            // Pass 1: the rows holding at least one candidate of this lane,
-            // the lane's candidate count, and the rows nonempty for any lane.
-            unsigned long long rows = 0ull;
-            uint32_t tot = 0u;
-            int n_any = 0;
-#pragma unroll 8
-            for (int i = 0; i < kBlock; ++i) {
-                const unsigned long long m = row_candidates(xnear[i], xshort[i], i, yn, ys, zn, zs);
-                rows |= (unsigned long long)(m != 0ull) << i;
-                tot += (uint32_t)__popcll(m);
-                n_any += __ballot(m != 0ull) != 0ull;
+        // This wave's ring of queued candidates: entries [qhead, qtail)
+        // (indices mod qcap); each packs i | j << 6 | k << 12 | (C - c0) << 18.
+        uint32_t qhead = 0u, qtail = 0u;
+        // The last 64 queued entries, their sides in flight: their entries
+        // are computed at the next drain (or the item's end), so the loads
+        // overlap the scan in between.
+        SqtPending pend;
+        pend.live = false;
+        auto drain_ready = [&]() {
+            while (qtail - qhead >= (uint32_t)kBlock) {
+                wave_sync();
+                sqt_entries(P, X, pend, sT, sQ, sA, fQ, fA, hist, nre);
+                pend = sqt_gather(P, X, Q, qhead, qmask, kBlock, lane);
+                qhead += kBlock;
             }
-            uint32_t max_tot = tot;
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) max_tot = max(max_tot, (uint32_t)__shfl_xor((int)max_tot, off));
-            max_tot = __builtin_amdgcn_readfirstlane(max_tot);
-            // Appends one entry per lane with has set (rows ri, columns j);
-            // drains the queue whenever 64 entries are waiting.  false when
-            // no lane had an entry.
-            auto push = [&](bool has, int ri, int j) -> bool {
-                const unsigned long long act = __ballot(has);
-                if (act == 0ull) return false;
-                if (has) {
-                    const int slot = qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
-                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
-                    Q[slot] = (uint32_t)ri | (uint32_t)j << 6 | (uint32_t)lane << 12 | coff;
-                }
-                qn += __popcll(act);
-                if (qn >= kBlock) {
-                    wave_sync();
-                    sqt_drain(P, X, Q, kBlock, lane, sT, sQ, sA, fQ, fA, hist, nre);
-                    // Move the remainder (fewer than 64) to the front.
-                    const int rest = qn - kBlock;
-                    const uint32_t v = lane < rest ? Q[kBlock + lane] : 0u;
-                    wave_sync();
-                    if (lane < rest) Q[lane] = v;
-                    wave_sync();
-                    qn = rest;
-                }
-                return true;
-            };
-            if (max_tot <= 2u * (uint32_t)n_any) {
-                // Sparse: one round per entry of the busiest lane over the
-                // whole C block (a row-by-row append needs at least one round
-                // per nonempty row, plus one to see it empty); a lane whose
-                // current row is used up moves on to its next row, which costs
-                // a reload in most rounds (factor 2 chosen by timing sparse
-                // and dense catalogs).
-                unsigned long long rm = 0ull;
-                int ri = 0;
-                for (;;) {
-                    if (rm == 0ull && rows != 0ull) {
-                        ri = __builtin_ctzll(rows);
-                        rows &= rows - 1ull;
-                        rm = row_candidates(xnear[ri], xshort[ri], ri, yn, ys, zn, zs);
+        };
+        for (int C = X.c0 + wave; C < c1; C += WAVES) {
+            // Lane k's column of tiles (B,C) and (A,C): bits j of side (j,k)
+            // and bits i of side (i,k).
+            const unsigned long long *mBC = P.masks + tile_index(X.B, C, nblk) * kMaskWords;
+            const unsigned long long zn = mBC[kMaskColNear + lane], zs = mBC[kMaskColShort + lane];
+            if (__ballot(zn != 0ull) == 0ull) continue;
+            const unsigned long long *mAC = P.masks + tile_index(X.A, C, nblk) * kMaskWords;
+            const unsigned long long yn = mAC[kMaskColNear + lane], ys = mAC[kMaskColShort + lane];
+            const uint32_t coff = (uint32_t)(C - X.c0) << 18;
+            for (int i = 0; i < kBlock; ++i) {
+                const unsigned long long xn = xnear[i];
+                if (xn == 0ull) continue;
+                unsigned long long m = row_candidates(xn, xshort[i], i, yn, ys, zn, zs);
+                const uint32_t cnt = (uint32_t)__popcll(m);
+                const uint32_t incl = wave_inclusive_sum(cnt, lane);
+                const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+                if (total == 0u) continue;
+                const uint32_t base = (uint32_t)i | (uint32_t)lane << 12 | coff;
+                if (qtail - qhead + total <= qcap) {
+                    // Compaction: every lane writes its candidates at its
+                    // offset in the row's block of the ring.
+                    uint32_t pos = qtail + incl - cnt;
+                    while (m != 0ull) {
+                        const int j = __builtin_ctzll(m);
+                        m &= m - 1ull;
+                        Q[pos & qmask] = base | (uint32_t)j << 6;
+                        ++pos;
                     }
-                    const bool has = rm != 0ull;
-                    const int j = has ? __builtin_ctzll(rm) : 0;
-                    rm &= rm - 1ull;
-                    if (!push(has, ri, j)) break;
-                }
-            } else {
-                // Dense: row by row (every lane refills from the same row).
-                for (int i = 0; i < kBlock; ++i) {
-                    if (__ballot((rows >> i) & 1ull) == 0ull) continue;
-                    unsigned long long rm = row_candidates(xnear[i], xshort[i], i, yn, ys, zn, zs);
+                    qtail += total;
+                } else {
+                    // The row does not fit: one entry per lane per round,
+                    // draining in between.
                     for (;;) {
-                        const bool has = rm != 0ull;
-                        const int j = has ? __builtin_ctzll(rm) : 0;
-                        rm &= rm - 1ull;
-                        if (!push(has, i, j)) break;
+                        const bool has = m != 0ull;
+                        const unsigned long long act = __ballot(has);
+                        if (act == 0ull) break;
+                        if (has) {
+                            const int j = __builtin_ctzll(m);
+                            m &= m - 1ull;
+                            const uint32_t slot = qtail + __builtin_amdgcn_mbcnt_hi(
+                                                              (uint32_t)(act >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+                            Q[slot & qmask] = base | (uint32_t)j << 6;
+                        }
+                        qtail += (uint32_t)__popcll(act);
+                        drain_ready();
                     }
                 }
+                drain_ready();
             }
         }
-        if (qn) {
+        sqt_entries(P, X, pend, sT, sQ, sA, fQ, fA, hist, nre);
+        if (qtail != qhead) {
             wave_sync();
-            sqt_drain(P, X, Q, qn, lane, sT, sQ, sA, fQ, fA, hist, nre);
+            sqt_entries(P, X, sqt_gather(P, X, Q, qhead, qmask, (int)(qtail - qhead), lane), sT, sQ, sA, fQ, fA,
+                        hist, nre);
             wave_sync();
         }
     }
@@ -481,10 +501,25 @@ hipError_t launch_sqt_pairs(const SqtPairJob &J, hipStream_t st) {
     return hipGetLastError();
 }
 
-const void *sqt_fast_kernel_ptr() { return reinterpret_cast<const void *>(&sqt_fast_kernel<8>); }
+const void *sqt_fast_kernel_ptr(int waves) {
+    if (waves == 8) return reinterpret_cast<const void *>(&sqt_fast_kernel<8>);
+    if (waves == 16) return reinterpret_cast<const void *>(&sqt_fast_kernel<16>);
+    return nullptr;
+}
 
-hipError_t launch_sqt_fast(const SqtFastArgs &P, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(sqt_fast_kernel<8>, dim3(grid), dim3(512), (size_t)P.cells * 4, st, P);
+size_t sqt_fast_lds_bytes(int cells, int waves, int qcap) {
+    return ((size_t)(cells + 3) & ~(size_t)3) * 4 + (size_t)waves * qcap * 4;
+}
+
+hipError_t launch_sqt_fast(const SqtFastArgs &P, int waves, int grid, hipStream_t st) {
+    if (P.qcap < 128 || (P.qcap & (P.qcap - 1))) return hipErrorInvalidValue;
+    const size_t lds = sqt_fast_lds_bytes(P.cells, waves, P.qcap);
+    if (waves == 8)
+        hipLaunchKernelGGL(sqt_fast_kernel<8>, dim3(grid), dim3(512), lds, st, P);
+    else if (waves == 16)
+        hipLaunchKernelGGL(sqt_fast_kernel<16>, dim3(grid), dim3(1024), lds, st, P);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

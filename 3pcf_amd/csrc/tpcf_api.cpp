@@ -158,7 +158,7 @@ struct JobPlan {
     uint64_t triplets = 0;
     // byte offsets inside the job's device region
     size_t o_u64 = 0, o_u32 = 0, o_blk = 0, o_T64 = 0, o_T32 = 0, o_Eq = 0, o_Et = 0, o_cnt = 0, o_hist = 0,
-           o_tiles = 0, o_dists = 0;
+           o_tiles = 0, o_dists = 0, o_masks = 0;
     size_t bytes = 0;
     bool need_tiles = true;
     double far_s = 0;  // SQT FAST: squared sides above this cannot be in any entry
@@ -240,6 +240,8 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     P.o_tiles = o; o = align256(o + (P.need_tiles ? (size_t)tile_count(P.L.nblk) * kTileBytes : 0));
     const bool sqt_fast = spec.param == TPCF_SQT && mode != TPCF_MODE_PLAIN;
     P.o_dists = o; o = align256(o + (sqt_fast ? (size_t)tile_count(P.L.nblk) * kTileBytes * sizeof(float) : 0));
+    P.o_masks = o;
+    o = align256(o + (sqt_fast ? (size_t)tile_count(P.L.nblk) * kMaskWords * sizeof(unsigned long long) : 0));
     if (sqt_fast) {
         // No entry of a triangle with a side >= s_hi*(1+q_hi) can be in range
         // (its s side is < s_hi, the q side < q_hi*s_hi, the third side at
@@ -485,6 +487,7 @@ static int launch_sqt_fast_job(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, c
     J.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
     J.codes = reinterpret_cast<uint8_t *>(base + P.o_tiles);
     J.dists = reinterpret_cast<float *>(base + P.o_dists);
+    J.masks = reinterpret_cast<unsigned long long *>(base + P.o_masks);
     J.Ts = reinterpret_cast<const double *>(base + P.o_T64);
     J.ns = P.n;
     J.far_s = P.far_s;
@@ -509,6 +512,7 @@ static int launch_sqt_fast_job(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, c
     SqtFastArgs A{};
     A.codes = J.codes;
     A.dists = J.dists;
+    A.masks = J.masks;
     A.u64 = J.u64;
     A.Ts = J.Ts;
     A.Eq = reinterpret_cast<const double *>(base + P.o_Eq);
@@ -532,13 +536,42 @@ static int launch_sqt_fast_job(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, c
     A.nblk = P.L.nblk;
     A.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist);
     A.rechecked = reinterpret_cast<unsigned long long *>(base + P.o_cnt);
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sqt_fast_kernel_ptr(), 512, (size_t)P.cells * 4) !=
-            hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
+    // Candidate rings of 512 words per wave, fewer where the histogram needs
+    // the LDS (the limit on cells keeps 128 possible); 8 waves per workgroup,
+    // or 16 when the LDS would leave a CU a single 8-wave workgroup.
+    const auto fit = [&](int waves, int &qcap, size_t &lds, int &per_cu) -> bool {
+        hipFuncAttributes fa{};
+        if (hipFuncGetAttributes(&fa, sqt_fast_kernel_ptr(waves)) != hipSuccess) return false;
+        for (qcap = waves == 8 ? 512 : 256; qcap >= 128; qcap /= 2) {
+            lds = sqt_fast_lds_bytes((int)P.cells, waves, qcap);
+            if (fa.sharedSizeBytes + lds <= 160 * 1024) break;
+        }
+        if (qcap < 128) return false;
+        per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sqt_fast_kernel_ptr(waves), waves * 64, lds) !=
+            hipSuccess)
+            per_cu = 0;
+        return per_cu >= 1;
+    };
+    int waves = 8, qcap = 0, per_cu = 0;
+    size_t lds = 0;
+    const bool ok8 = fit(8, qcap, lds, per_cu);
+    if (!ok8 || per_cu < 2) {
+        int q16 = 0, pc16 = 0;
+        size_t l16 = 0;
+        if (fit(16, q16, l16, pc16)) {
+            waves = 16;
+            qcap = q16;
+            per_cu = pc16;
+        } else if (!ok8) {
+            (void)hipGetLastError();
+            return set_error(TPCF_EINVAL, "(s,q,theta) histogram of %lld cells does not fit the FAST kernel's LDS",
+                             (long long)P.cells);
+        }
+    }
+    A.qcap = qcap;
     HIP_TRY(hipEventRecord(ws.ev[3], st));
-    HIP_TRY(launch_sqt_fast(A, ctx->num_cu * per_cu, st));
+    HIP_TRY(launch_sqt_fast(A, waves, ctx->num_cu * per_cu, st));
     return TPCF_OK;
 }
 
@@ -730,7 +763,7 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     // Kernels with an LDS histogram raise their dynamic LDS limit once, to
     // what their static LDS leaves of the 160 KiB.
     std::vector<const void *> lds_kernels{sqt_plain_kernel_ptr(kSqt), sqt_plain_kernel_ptr(kJbj),
-                                          sqt_fast_kernel_ptr()};
+                                          sqt_fast_kernel_ptr(8), sqt_fast_kernel_ptr(16)};
     for (int w : {2, 4, 8, 16})
         for (int k : {1, 2, 4, 8})
             if (const void *fn = r123_triplet_kernel_ptr(w, k)) lds_kernels.push_back(fn);

@@ -95,14 +95,21 @@ struct SqtPairJob {
     const uint8_t *blk_count;
     uint8_t *codes;  // 4 KiB per tile
     float *dists;    // 16 KiB per tile: (float)sqrt(s) of the FP64 squared side
+    // 2 KiB per tile, kMaskWords words: bit j of row i near / with an s bin,
+    // bit i of column j near / with an s bin.
+    unsigned long long *masks;
     const double *Ts;
     int ns;
     double far_s;    // squared sides above this are kCodeFar
     int64_t npad, nblk;
 };
+// Per-tile mask words: row near, row short, column near, column short.
+constexpr int kMaskWords = 4 * 64;
+constexpr int kMaskRowNear = 0, kMaskRowShort = 64, kMaskColNear = 128, kMaskColShort = 192;
 struct SqtFastArgs {
     const uint8_t *codes;
     const float *dists;
+    const unsigned long long *masks;
     const double *u64;
     const double *Ts, *Eq, *Et;
     const TripletItem *items;
@@ -115,13 +122,17 @@ struct SqtFastArgs {
     // confirmed against the edge tables.
     float q_lo, q_inv, t_lo, t_inv;
     int q_log, t_log;
+    // Candidate ring per wave in dynamic LDS after the histogram (words,
+    // power of two, >= 128).
+    int qcap;
     int64_t npad, nblk;
     unsigned long long *hist;       // term t at hist + t*cells
     unsigned long long *rechecked;  // triangles recomputed in FP64
 };
 hipError_t launch_sqt_pairs(const SqtPairJob &J, hipStream_t st);
-hipError_t launch_sqt_fast(const SqtFastArgs &P, int grid, hipStream_t st);
-const void *sqt_fast_kernel_ptr();
+hipError_t launch_sqt_fast(const SqtFastArgs &P, int waves, int grid, hipStream_t st);
+size_t sqt_fast_lds_bytes(int cells, int waves, int qcap);
+const void *sqt_fast_kernel_ptr(int waves);  // 8 or 16 waves per workgroup
 
 // Launchers (kernels_r123.hip).
 hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st);

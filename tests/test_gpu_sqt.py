@@ -125,3 +125,17 @@ def test_sqt_fast_edges_on_data(ctx, mode):
         got, st = run(ctx, [d], [(0, 0, 0)], tp.SQT, axes, mode)
         assert np.array_equal(got, oracle([d], [(0, 0, 0)], orc.SQT, axes))
         assert int(got.sum()) > 0
+
+
+@pytest.mark.parametrize("mode", [tp.MODE_FAST, tp.MODE_FAST64])
+def test_sqt_fast_largest_histogram(ctx, mode):
+    """36864 cells, the largest (s,q,theta) histogram: the FAST kernel's
+    candidate rings shrink to fit the LDS left by the histogram."""
+    print(f"case sqt largest histogram mode={mode}", flush=True)
+    d = synth.neyman_scott(170, nclust=10, sigma=1.5, size=20.0, seed=41)
+    r = synth.uniform_cube(150, 20.0, seed=42)
+    axes = [(48, 1.0, 13.0, "lin"), (24, 0.9, 4.1, "lin"), (32, 0.0, 1.0, "lin")]
+    assert tp.hist_cells(tp.binspec(tp.SQT, [tp.axis(*a) for a in axes])) == 36864
+    got, _ = run(ctx, [d, r], tp.FOUR_TERMS, tp.SQT, axes, mode)
+    assert np.array_equal(got, oracle([d, r], tp.FOUR_TERMS, orc.SQT, axes))
+    assert int(got.sum()) > 0
