@@ -67,7 +67,7 @@ def cpu_baseline(d, r, seconds_hint: float = 20.0):
     from oracle import oracle as orc
 
     threads = max(1, min(16, os.cpu_count() or 1))
-    m = 1500
+    m = 1800
     segs = [d[:m], r[:m]]
     t0 = time.perf_counter()
     orc.clean_count(segs, tp.FOUR_TERMS, orc.R123, [orc.axis(20, 1.0, 350.0, "log")], nthreads=threads)
