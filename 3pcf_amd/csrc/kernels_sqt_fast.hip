@@ -154,14 +154,14 @@ __device__ inline int bracket_desc(float c, float d, const float *C, int n, int 
 // Returns the histogram cell, -1 for "out of range" (certain), and sets
 // *unsure when some value lies within its error bound of a deciding edge.
 __device__ inline int entry32(int sbin, float b, float ra, float num, float rden, float ssum, const float *fQ,
-                              const float *fA, const SqtFastArgs &P, bool *unsure) {
-    const int nq = P.nq, nt = P.nt;
+                              const float *fA, const SqtTerm &T, bool *unsure) {
+    const int nq = T.nq, nt = T.nt;
     const float q = b * ra;
-    const int mq = bracket_asc(q, 6.f * kU * q, fQ, nq, guess_bin(q, P.q_lo, P.q_inv, P.q_log, nq), unsure);
+    const int mq = bracket_asc(q, 6.f * kU * q, fQ, nq, guess_bin(q, T.q_lo, T.q_inv, T.q_log, nq), unsure);
     const float c = num * rden;
     const float dc = 5.f * kU * ssum * rden + 7.f * kU * fabsf(c) + 2.f * kU;
     const int mt =
-        bracket_desc(c, dc, fA, nt, guess_bin(acos_over_pi(c), P.t_lo, P.t_inv, P.t_log, nt), unsure);
+        bracket_desc(c, dc, fA, nt, guess_bin(acos_over_pi(c), T.t_lo, T.t_inv, T.t_log, nt), unsure);
     if (mq < 0 || mt < 0) return -1;
     return (sbin * nq + mq) * nt + mt;
 }
@@ -260,7 +260,7 @@ struct SqtPending {
 };
 
 // Starts loading the sides of ring entries head .. head+cnt-1, one per lane.
-__device__ inline SqtPending sqt_gather(const SqtFastArgs &P, const SqtItemCtx &X, const uint32_t *Q,
+__device__ inline SqtPending sqt_gather(const SqtTerm &T, const SqtItemCtx &X, const uint32_t *Q,
                                         uint32_t head, uint32_t qmask, int cnt, int lane) {
     SqtPending p;
     p.live = lane < cnt;
@@ -270,20 +270,20 @@ __device__ inline SqtPending sqt_gather(const SqtFastArgs &P, const SqtItemCtx &
     if (p.live) {
         const uint32_t e = p.e;
         const int i = e & 63, j = (e >> 6) & 63, k = (e >> 12) & 63, C = X.c0 + (int)(e >> 18);
-        const int64_t tAC = tile_index(X.A, C, P.nblk) * kTileBytes;
-        const int64_t tBC = tile_index(X.B, C, P.nblk) * kTileBytes;
+        const int64_t tAC = tile_index(X.A, C, T.nblk) * kTileBytes;
+        const int64_t tBC = tile_index(X.B, C, T.nblk) * kTileBytes;
         p.xc = X.cAB[i * kBlock + j];
-        p.yc = P.codes[tAC + i * kBlock + k];
-        p.zc = P.codes[tBC + j * kBlock + k];
+        p.yc = T.codes[tAC + i * kBlock + k];
+        p.zc = T.codes[tBC + j * kBlock + k];
         p.xd = X.dAB[i * kBlock + j];
-        p.yd = P.dists[tAC + i * kBlock + k];
-        p.zd = P.dists[tBC + j * kBlock + k];
+        p.yd = T.dists[tAC + i * kBlock + k];
+        p.zd = T.dists[tBC + j * kBlock + k];
     }
     return p;
 }
 
 // Histogram entries of a gathered candidate (FP32 with FP64 recheck).
-__device__ inline void sqt_entries(const SqtFastArgs &P, const SqtItemCtx &X, const SqtPending &p,
+__device__ inline void sqt_entries(const SqtTerm &T, const SqtItemCtx &X, const SqtPending &p,
                                    const double *sT, const double *sQ, const double *sA, const float *fQ,
                                    const float *fA, uint32_t *hist, unsigned long long &nre) {
     if (!p.live) return;
@@ -293,8 +293,8 @@ __device__ inline void sqt_entries(const SqtFastArgs &P, const SqtItemCtx &X, co
     if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
     if (l2 > l3) { float t = l2; l2 = l3; l3 = t; uint32_t u = k2; k2 = k3; k3 = u; }
     if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
-    const int nq = P.nq, nt = P.nt;
-    bool unsure = P.force_fp64 != 0;
+    const int nq = T.nq, nt = T.nt;
+    bool unsure = T.force_fp64 != 0;
     // Side order certain only when lengths differ by more than their FP32
     // rounding; lengths far outside [1e-18, 1e18] go to FP64 (no FP32
     // underflow or overflow below).
@@ -306,13 +306,13 @@ __device__ inline void sqt_entries(const SqtFastArgs &P, const SqtItemCtx &X, co
         if (k1 < kCodeNear) {
             const float r1 = __builtin_amdgcn_rcpf(l1);
             cell[0] = entry32((int)k1, l2, r1, (s1 + s2) - s3, __builtin_amdgcn_rcpf((2.f * l1) * l2), ssum, fQ, fA,
-                              P, &unsure);
+                              T, &unsure);
             cell[1] = entry32((int)k1, l3, r1, (s1 + s3) - s2, __builtin_amdgcn_rcpf((2.f * l1) * l3), ssum, fQ, fA,
-                              P, &unsure);
+                              T, &unsure);
         }
         if (k2 < kCodeNear)
             cell[2] = entry32((int)k2, l3, __builtin_amdgcn_rcpf(l2), (s2 + s3) - s1,
-                              __builtin_amdgcn_rcpf((2.f * l2) * l3), ssum, fQ, fA, P, &unsure);
+                              __builtin_amdgcn_rcpf((2.f * l2) * l3), ssum, fQ, fA, T, &unsure);
     }
     if (!unsure) {
 #pragma unroll
@@ -323,8 +323,8 @@ __device__ inline void sqt_entries(const SqtFastArgs &P, const SqtItemCtx &X, co
         const uint32_t e = p.e;
         const int i = e & 63, j = (e >> 6) & 63, k = (e >> 12) & 63, C = X.c0 + (int)(e >> 18);
         const int64_t gi = (int64_t)X.A * kBlock + i, gj = (int64_t)X.B * kBlock + j, gk = (int64_t)C * kBlock + k;
-        entries_fp64(sq64f(P.u64, P.npad, gi, gj), sq64f(P.u64, P.npad, gi, gk), sq64f(P.u64, P.npad, gj, gk), sT,
-                     sQ, sA, P.ns, nq, nt, hist);
+        entries_fp64(sq64f(T.u64, T.npad, gi, gj), sq64f(T.u64, T.npad, gi, gk), sq64f(T.u64, T.npad, gj, gk), sT,
+                     sQ, sA, T.ns, nq, nt, hist);
     }
 }
 
@@ -370,24 +370,14 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int ns = P.ns, nq = P.nq, nt = P.nt, cells = P.cells;
-    for (int m = threadIdx.x; m <= kMaxBins; m += blockDim.x) {
-        if (m <= ns) sT[m] = P.Ts[m];
-        if (m <= nq) {
-            sQ[m] = P.Eq[m];
-            fQ[m] = (float)P.Eq[m];
-        }
-        if (m <= nt) {
-            sA[m] = P.Et[m];
-            fA[m] = (float)P.Et[m];
-        }
-    }
-    for (int c = threadIdx.x; c < cells; c += blockDim.x) hist[c] = 0u;
+    for (int c = threadIdx.x; c < P.lds_cells; c += blockDim.x) hist[c] = 0u;
     const uint32_t qcap = (uint32_t)P.qcap, qmask = qcap - 1u;
-    uint32_t *Q = hist + ((cells + 3) & ~3) + wave * qcap;
+    uint32_t *Q = hist + ((P.lds_cells + 3) & ~3) + wave * qcap;
+    // The current term (its job's tables in LDS); flushing writes and clears
+    // the cells the term used, so the next term starts from zero.
     int cur = -1;
+    SqtTerm T{};
     unsigned long long since = 0, nre = 0;
-    const int64_t nblk = P.nblk;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
@@ -403,15 +393,33 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         const int c1 = __builtin_amdgcn_readfirstlane(I.c1);
         const unsigned long long add = 3ull * (unsigned long long)(c1 - X.c0) * (kBlock * kBlock * kBlock);
         if (term != cur || since + add > 0xC0000000ull) {
-            if (cur >= 0) sqt_flush64(hist, cells, P.hist + (size_t)cur * cells);
+            if (cur >= 0) sqt_flush64(hist, T.cells, T.hist);
+            if (term != cur) {
+                if (nre) atomicAdd(T.rechecked, nre);
+                nre = 0;
+                T = P.terms[term];
+                for (int m = threadIdx.x; m <= kMaxBins; m += blockDim.x) {
+                    if (m <= T.ns) sT[m] = T.Ts[m];
+                    if (m <= T.nq) {
+                        sQ[m] = T.Eq[m];
+                        fQ[m] = (float)T.Eq[m];
+                    }
+                    if (m <= T.nt) {
+                        sA[m] = T.Et[m];
+                        fA[m] = (float)T.Et[m];
+                    }
+                }
+                __syncthreads();
+            }
             cur = term;
             since = 0;
         }
         since += add;
+        const int64_t nblk = T.nblk;
         const int64_t tAB = tile_index(X.A, X.B, nblk) * kTileBytes;
-        X.cAB = P.codes + tAB;
-        X.dAB = P.dists + tAB;
-        const unsigned long long *mAB = P.masks + tile_index(X.A, X.B, nblk) * kMaskWords;
+        X.cAB = T.codes + tAB;
+        X.dAB = T.dists + tAB;
+        const unsigned long long *mAB = T.masks + tile_index(X.A, X.B, nblk) * kMaskWords;
         for (int i = threadIdx.x; i < kBlock; i += blockDim.x) {
             xnear[i] = mAB[kMaskRowNear + i];
             xshort[i] = mAB[kMaskRowShort + i];
@@ -428,18 +436,18 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         auto drain_ready = [&]() {
             while (qtail - qhead >= (uint32_t)kBlock) {
                 wave_sync();
-                sqt_entries(P, X, pend, sT, sQ, sA, fQ, fA, hist, nre);
-                pend = sqt_gather(P, X, Q, qhead, qmask, kBlock, lane);
+                sqt_entries(T, X, pend, sT, sQ, sA, fQ, fA, hist, nre);
+                pend = sqt_gather(T, X, Q, qhead, qmask, kBlock, lane);
                 qhead += kBlock;
             }
         };
         for (int C = X.c0 + wave; C < c1; C += WAVES) {
             // Lane k's column of tiles (B,C) and (A,C): bits j of side (j,k)
             // and bits i of side (i,k).
-            const unsigned long long *mBC = P.masks + tile_index(X.B, C, nblk) * kMaskWords;
+            const unsigned long long *mBC = T.masks + tile_index(X.B, C, nblk) * kMaskWords;
             const unsigned long long zn = mBC[kMaskColNear + lane], zs = mBC[kMaskColShort + lane];
             if (__ballot(zn != 0ull) == 0ull) continue;
-            const unsigned long long *mAC = P.masks + tile_index(X.A, C, nblk) * kMaskWords;
+            const unsigned long long *mAC = T.masks + tile_index(X.A, C, nblk) * kMaskWords;
             const unsigned long long yn = mAC[kMaskColNear + lane], ys = mAC[kMaskColShort + lane];
             const uint32_t coff = (uint32_t)(C - X.c0) << 18;
             for (int i = 0; i < kBlock; ++i) {
@@ -484,16 +492,18 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
                 drain_ready();
             }
         }
-        sqt_entries(P, X, pend, sT, sQ, sA, fQ, fA, hist, nre);
+        sqt_entries(T, X, pend, sT, sQ, sA, fQ, fA, hist, nre);
         if (qtail != qhead) {
             wave_sync();
-            sqt_entries(P, X, sqt_gather(P, X, Q, qhead, qmask, (int)(qtail - qhead), lane), sT, sQ, sA, fQ, fA,
+            sqt_entries(T, X, sqt_gather(T, X, Q, qhead, qmask, (int)(qtail - qhead), lane), sT, sQ, sA, fQ, fA,
                         hist, nre);
             wave_sync();
         }
     }
-    if (cur >= 0) sqt_flush64(hist, cells, P.hist + (size_t)cur * cells);
-    if (nre) atomicAdd(P.rechecked, nre);
+    if (cur >= 0) {
+        sqt_flush64(hist, T.cells, T.hist);
+        if (nre) atomicAdd(T.rechecked, nre);
+    }
 }
 
 hipError_t launch_sqt_pairs(const SqtPairJob &J, hipStream_t st) {
@@ -513,7 +523,7 @@ size_t sqt_fast_lds_bytes(int cells, int waves, int qcap) {
 
 hipError_t launch_sqt_fast(const SqtFastArgs &P, int waves, int grid, hipStream_t st) {
     if (P.qcap < 128 || (P.qcap & (P.qcap - 1))) return hipErrorInvalidValue;
-    const size_t lds = sqt_fast_lds_bytes(P.cells, waves, P.qcap);
+    const size_t lds = sqt_fast_lds_bytes(P.lds_cells, waves, P.qcap);
     if (waves == 8)
         hipLaunchKernelGGL(sqt_fast_kernel<8>, dim3(grid), dim3(512), lds, st, P);
     else if (waves == 16)

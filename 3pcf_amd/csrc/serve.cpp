@@ -8,10 +8,11 @@
 // idle worker takes every queued job that fits the memory left in the budget
 // (first fit, in arrival order; a job larger than the whole budget fails with
 // TPCF_EBUSY) and runs them as one batch: pair stages back to back, then one
-// persistent triplet launch over all their work items.  Jobs that arrive
-// while a batch is on the GPU are picked up by another worker as soon as
-// memory allows, so their kernels overlap with the running batch on the
-// device.  PLAIN-mode jobs run one per launch (the FP64 baseline).
+// persistent triplet launch over all their work items (one for the
+// (r1,r2,r3) jobs, one for the (s,q,theta) jobs).  Jobs that arrive while a
+// batch is on the GPU are picked up by another worker as soon as memory
+// allows, so their kernels overlap with the running batch on the device.
+// PLAIN-mode jobs (the FP64 baseline) and JBJ jobs run one per launch.
 #include <chrono>
 #include <cstring>
 
@@ -33,7 +34,7 @@ struct Ticket {
 };
 
 // Jobs that do not join a batched persistent launch.
-static bool solo(const Ticket &t) { return t.mode == TPCF_MODE_PLAIN || t.spec.param != TPCF_R123; }
+static bool solo(const Ticket &t) { return t.mode == TPCF_MODE_PLAIN || t.spec.param == TPCF_JBJ; }
 
 static constexpr int kWorkers = 3;
 static constexpr int kMaxBatch = 64;
@@ -51,8 +52,8 @@ static void worker_main(tpcf_ctx *ctx) {
             std::unique_lock<std::mutex> lk(ctx->mu);
             for (;;) {
                 if (ctx->stopping && ctx->pending.empty()) break;
-                // First fit in arrival order; PLAIN jobs and triangle
-                // parametrizations run alone.
+                // First fit in arrival order; PLAIN and JBJ jobs run
+                // alone.
                 for (auto it = ctx->pending.begin(); it != ctx->pending.end() && (int)batch.size() < kMaxBatch;) {
                     auto t = ctx->tickets[*it];
                     const bool plain = solo(*t);

@@ -302,8 +302,9 @@ static PairJob pair_job(const JobPlan &P, char *base, int shift) {
 // ---------------------------------------------------------------------------
 
 struct Workspace {
-    DevBuf region;   // job regions
-    DevBuf queue;    // items + terms + counter
+    DevBuf region;     // job regions
+    DevBuf queue;      // items + terms + counter of the (r1,r2,r3) kernel
+    DevBuf sqt_queue;  // items + terms + counter of the (s,q,theta) kernel
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     ~Workspace() {
         for (auto &e : ev)
@@ -480,62 +481,84 @@ static int launch_plain(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tp
 // FAST (s,q,theta): FP64 pair codes and FP32 side lengths, then the
 // persistent candidate kernel over the items of every term (FP32 entries,
 // FP64 recheck near edges; FAST64 sends every candidate through FP64).
-static int launch_sqt_fast_job(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tpcf_job &job, char *base,
-                               int mode, hipStream_t st) {
-    SqtPairJob J{};
-    J.u64 = reinterpret_cast<const double *>(base + P.o_u64);
-    J.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
-    J.codes = reinterpret_cast<uint8_t *>(base + P.o_tiles);
-    J.dists = reinterpret_cast<float *>(base + P.o_dists);
-    J.masks = reinterpret_cast<unsigned long long *>(base + P.o_masks);
-    J.Ts = reinterpret_cast<const double *>(base + P.o_T64);
-    J.ns = P.n;
-    J.far_s = P.far_s;
-    J.npad = P.L.npad;
-    J.nblk = P.L.nblk;
-    HIP_TRY(launch_sqt_pairs(J, st));
-    HIP_TRY(hipEventRecord(ws.ev[1], st));
+// FAST (s,q,theta) for several jobs on stream st: every job's pair stage,
+// then one persistent triplet launch over the items of every job and term
+// (a workgroup reloads the edge tables and flushes its histogram when the
+// term changes).  bases[j] is job j's staged device region.
+static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<const JobPlan *> &plans,
+                                const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases, int mode,
+                                hipStream_t st) {
+    std::vector<SqtTerm> terms;
     std::vector<TripletItem> items;
-    for (int t = 0; t < P.nterms; ++t) {
-        const size_t before = items.size();
-        term_items(P.L, job.term[t], t, items);
-        std::stable_sort(items.begin() + before, items.end(), [](const TripletItem &a, const TripletItem &b) {
-            return (a.c1 - a.c0) > (b.c1 - b.c0);
-        });
+    int lds_cells = 1;
+    for (size_t j = 0; j < plans.size(); ++j) {
+        const JobPlan &P = *plans[j];
+        char *base = bases[j];
+        if (!P.L.nblk) continue;
+        SqtPairJob J{};
+        J.u64 = reinterpret_cast<const double *>(base + P.o_u64);
+        J.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
+        J.codes = reinterpret_cast<uint8_t *>(base + P.o_tiles);
+        J.dists = reinterpret_cast<float *>(base + P.o_dists);
+        J.masks = reinterpret_cast<unsigned long long *>(base + P.o_masks);
+        J.Ts = reinterpret_cast<const double *>(base + P.o_T64);
+        J.ns = P.n;
+        J.far_s = P.far_s;
+        J.npad = P.L.npad;
+        J.nblk = P.L.nblk;
+        HIP_TRY(launch_sqt_pairs(J, st));
+        SqtTerm T{};
+        T.codes = J.codes;
+        T.dists = J.dists;
+        T.masks = J.masks;
+        T.u64 = J.u64;
+        T.Ts = J.Ts;
+        T.Eq = reinterpret_cast<const double *>(base + P.o_Eq);
+        T.Et = reinterpret_cast<const double *>(base + P.o_Et);
+        T.rechecked = reinterpret_cast<unsigned long long *>(base + P.o_cnt);
+        T.npad = P.L.npad;
+        T.nblk = P.L.nblk;
+        T.ns = P.n;
+        T.nq = P.nq;
+        T.nt = P.nt;
+        T.cells = (int)P.cells;
+        T.force_fp64 = mode == TPCF_MODE_FAST64;
+        const auto guess = [](const tpcf_axis &ax, float &lo, float &inv, int &lg) {
+            lg = ax.scale == TPCF_LOG;
+            lo = (float)(lg ? std::log2(ax.lo) : ax.lo);
+            inv = (float)(ax.n / (lg ? std::log2(ax.hi / ax.lo) : ax.hi - ax.lo));
+        };
+        guess(P.axq, T.q_lo, T.q_inv, T.q_log);
+        guess(P.axt, T.t_lo, T.t_inv, T.t_log);
+        lds_cells = std::max(lds_cells, T.cells);
+        for (int t = 0; t < P.nterms; ++t) {
+            T.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist) + (size_t)t * P.cells;
+            const size_t before = items.size();
+            term_items(P.L, jobs[j]->term[t], (int)terms.size(), items);
+            std::stable_sort(items.begin() + before, items.end(), [](const TripletItem &a, const TripletItem &b) {
+                return (a.c1 - a.c0) > (b.c1 - b.c0);
+            });
+            terms.push_back(T);
+        }
     }
-    if (items.empty()) return TPCF_OK;
-    const size_t o_ctr = align256(sizeof(TripletItem) * items.size());
-    if (int e = ws.queue.reserve(o_ctr + 256)) return e;
-    char *q = reinterpret_cast<char *>(ws.queue.p);
+    HIP_TRY(hipEventRecord(ws.ev[1], st));
+    if (items.empty()) {
+        HIP_TRY(hipEventRecord(ws.ev[3], st));
+        return TPCF_OK;
+    }
+    const size_t o_terms = align256(sizeof(TripletItem) * items.size());
+    const size_t o_ctr = align256(o_terms + sizeof(SqtTerm) * terms.size());
+    if (int e = ws.sqt_queue.reserve(o_ctr + 256)) return e;
+    char *q = reinterpret_cast<char *>(ws.sqt_queue.p);
     HIP_TRY(hipMemcpyAsync(q, items.data(), sizeof(TripletItem) * items.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(q + o_terms, terms.data(), sizeof(SqtTerm) * terms.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(q + o_ctr), 1, 0u, st));
     SqtFastArgs A{};
-    A.codes = J.codes;
-    A.dists = J.dists;
-    A.masks = J.masks;
-    A.u64 = J.u64;
-    A.Ts = J.Ts;
-    A.Eq = reinterpret_cast<const double *>(base + P.o_Eq);
-    A.Et = reinterpret_cast<const double *>(base + P.o_Et);
+    A.terms = reinterpret_cast<const SqtTerm *>(q + o_terms);
     A.items = reinterpret_cast<const TripletItem *>(q);
     A.nitems = (int)items.size();
     A.counter = reinterpret_cast<int *>(q + o_ctr);
-    A.ns = P.n;
-    A.nq = P.nq;
-    A.nt = P.nt;
-    const auto guess = [](const tpcf_axis &ax, float &lo, float &inv, int &lg) {
-        lg = ax.scale == TPCF_LOG;
-        lo = (float)(lg ? std::log2(ax.lo) : ax.lo);
-        inv = (float)(ax.n / (lg ? std::log2(ax.hi / ax.lo) : ax.hi - ax.lo));
-    };
-    guess(P.axq, A.q_lo, A.q_inv, A.q_log);
-    guess(P.axt, A.t_lo, A.t_inv, A.t_log);
-    A.cells = (int)P.cells;
-    A.force_fp64 = mode == TPCF_MODE_FAST64;
-    A.npad = P.L.npad;
-    A.nblk = P.L.nblk;
-    A.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist);
-    A.rechecked = reinterpret_cast<unsigned long long *>(base + P.o_cnt);
+    A.lds_cells = lds_cells;
     // Candidate rings of 512 words per wave, fewer where the histogram needs
     // the LDS (the limit on cells keeps 128 possible); 8 waves per workgroup,
     // or 16 when the LDS would leave a CU a single 8-wave workgroup.
@@ -543,7 +566,7 @@ static int launch_sqt_fast_job(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, c
         hipFuncAttributes fa{};
         if (hipFuncGetAttributes(&fa, sqt_fast_kernel_ptr(waves)) != hipSuccess) return false;
         for (qcap = waves == 8 ? 512 : 256; qcap >= 128; qcap /= 2) {
-            lds = sqt_fast_lds_bytes((int)P.cells, waves, qcap);
+            lds = sqt_fast_lds_bytes(lds_cells, waves, qcap);
             if (fa.sharedSizeBytes + lds <= 160 * 1024) break;
         }
         if (qcap < 128) return false;
@@ -565,8 +588,8 @@ static int launch_sqt_fast_job(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, c
             per_cu = pc16;
         } else if (!ok8) {
             (void)hipGetLastError();
-            return set_error(TPCF_EINVAL, "(s,q,theta) histogram of %lld cells does not fit the FAST kernel's LDS",
-                             (long long)P.cells);
+            return set_error(TPCF_EINVAL, "(s,q,theta) histogram of %d cells does not fit the FAST kernel's LDS",
+                             lds_cells);
         }
     }
     A.qcap = qcap;
@@ -603,7 +626,10 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
             if (int e = launch_plain(ctx, ws, P, job, base, st)) return e;
         }
     } else if (P.param == TPCF_SQT && P.L.nblk) {
-        if (int e = launch_sqt_fast_job(ctx, ws, P, job, base, mode, st)) return e;
+        std::vector<const JobPlan *> plans{&P};
+        std::vector<const tpcf_job *> jobs{&job};
+        std::vector<char *> bases{base};
+        if (int e = launch_sqt_fast_jobs(ctx, ws, plans, jobs, bases, mode, st)) return e;
     } else if (P.L.nblk) {
         HIP_TRY(launch_pair_tiles(pair_job(P, base, P.n <= kMaxLdsR123Bins ? 2 : 0), mode == TPCF_MODE_FAST, st));
         HIP_TRY(hipEventRecord(ws.ev[1], st));
@@ -645,8 +671,8 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     size_t bytes = 0;
     std::vector<size_t> off(njobs);
     for (int j = 0; j < njobs; ++j) {
-        if (specs[j].param != TPCF_R123)
-            return set_error(TPCF_EINVAL, "batches hold (r1,r2,r3) jobs; count triangle parametrizations one by one");
+        if (specs[j].param == TPCF_JBJ)
+            return set_error(TPCF_EINVAL, "batches hold (r1,r2,r3) and (s,q,theta) jobs; count JBJ one by one");
         if (int e = plan_job(jobs[j], specs[j], mode, plans[j])) return e;
         off[j] = bytes;
         bytes = align256(bytes + plans[j].bytes);
@@ -658,21 +684,37 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
         if (int e = stage_job(jobs[j], plans[j], base + off[j], st)) return e;
     HIP_TRY(hipEventRecord(ws.ev[0], st));
     HIP_TRY(hipEventRecord(ws.ev[3], st));
+    // (s,q,theta) jobs: one persistent launch of their own.
+    std::vector<const JobPlan *> sp;
+    std::vector<const tpcf_job *> sj;
+    std::vector<char *> sb;
+    for (int j = 0; j < njobs; ++j)
+        if (plans[j].param == TPCF_SQT) {
+            sp.push_back(&plans[j]);
+            sj.push_back(&jobs[j]);
+            sb.push_back(base + off[j]);
+        }
+    if (!sp.empty())
+        if (int e = launch_sqt_fast_jobs(ctx, ws, sp, sj, sb, mode, st)) return e;
+    // (r1,r2,r3) jobs: pair stages, then one persistent launch.
     int max_n = 1;
-    for (int j = 0; j < njobs; ++j) max_n = std::max(max_n, plans[j].n);
+    for (int j = 0; j < njobs; ++j)
+        if (plans[j].param == TPCF_R123) max_n = std::max(max_n, plans[j].n);
     const int shift = max_n <= kMaxLdsR123Bins ? 2 : 0;
     std::vector<const JobPlan *> pp;
     std::vector<const tpcf_job *> jj;
     std::vector<char *> bb;
     for (int j = 0; j < njobs; ++j) {
-        if (!plans[j].L.nblk) continue;
+        if (plans[j].param != TPCF_R123 || !plans[j].L.nblk) continue;
         HIP_TRY(launch_pair_tiles(pair_job(plans[j], base + off[j], shift), mode == TPCF_MODE_FAST, st));
         pp.push_back(&plans[j]);
         jj.push_back(&jobs[j]);
         bb.push_back(base + off[j]);
     }
-    HIP_TRY(hipEventRecord(ws.ev[1], st));
-    if (int e = launch_triplets(ctx, ws, pp, jj, bb, st)) return e;
+    if (!pp.empty() || sp.empty()) {
+        HIP_TRY(hipEventRecord(ws.ev[1], st));
+        if (int e = launch_triplets(ctx, ws, pp, jj, bb, st)) return e;
+    }
     HIP_TRY(hipEventRecord(ws.ev[2], st));
     std::vector<uint64_t> rech(njobs, 0);
     for (int j = 0; j < njobs; ++j) {

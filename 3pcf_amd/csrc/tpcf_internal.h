@@ -106,15 +106,16 @@ struct SqtPairJob {
 // Per-tile mask words: row near, row short, column near, column short.
 constexpr int kMaskWords = 4 * 64;
 constexpr int kMaskRowNear = 0, kMaskRowShort = 64, kMaskColNear = 128, kMaskColShort = 192;
-struct SqtFastArgs {
+// One (job, term) of a FAST (s,q,theta) launch.
+struct SqtTerm {
     const uint8_t *codes;
     const float *dists;
     const unsigned long long *masks;
     const double *u64;
     const double *Ts, *Eq, *Et;
-    const TripletItem *items;
-    int nitems;
-    int *counter;
+    unsigned long long *hist;       // this term's ns*nq*nt cells
+    unsigned long long *rechecked;  // the job's count of triangles recomputed in FP64
+    int64_t npad, nblk;
     int ns, nq, nt, cells;
     int force_fp64;  // every candidate triangle through the FP64 definition
     // Bin guesses of the q and theta/pi axes: lin (v - lo) * inv, log
@@ -122,12 +123,16 @@ struct SqtFastArgs {
     // confirmed against the edge tables.
     float q_lo, q_inv, t_lo, t_inv;
     int q_log, t_log;
+};
+struct SqtFastArgs {
+    const SqtTerm *terms;
+    const TripletItem *items;  // sorted by term
+    int nitems;
+    int *counter;
+    int lds_cells;  // histogram words in LDS: the largest term's cells
     // Candidate ring per wave in dynamic LDS after the histogram (words,
     // power of two, >= 128).
     int qcap;
-    int64_t npad, nblk;
-    unsigned long long *hist;       // term t at hist + t*cells
-    unsigned long long *rechecked;  // triangles recomputed in FP64
 };
 hipError_t launch_sqt_pairs(const SqtPairJob &J, hipStream_t st);
 hipError_t launch_sqt_fast(const SqtFastArgs &P, int waves, int grid, hipStream_t st);

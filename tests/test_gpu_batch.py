@@ -100,3 +100,77 @@ def test_serving_admission_rejects_oversized(ctx):
     t, out = ctx.submit(job, spec)
     ctx.wait(t)
     assert int(out.sum()) > 0
+
+
+def _sqt_spec(rng):
+    nb = [int(rng.integers(6, 20)) for _ in range(3)]
+    scale = "log" if rng.random() < 0.5 else "lin"
+    axes = [(nb[0], 1.0, 12.0, scale), (nb[1], 1.0, 3.0, "lin"), (nb[2], 0.0, 1.0, "lin")]
+    return tp.binspec(tp.SQT, [tp.axis(*a) for a in axes]), [orc.axis(*a) for a in axes]
+
+
+def test_count_batch_mixed_parametrizations(ctx):
+    """(r1,r2,r3) and (s,q,theta) jobs of different sizes and histogram sizes
+    in one call: one persistent launch per parametrization, every histogram
+    equal to the oracle."""
+    print("case count_batch mixed r123/sqt", flush=True)
+    rng = np.random.Generator(np.random.PCG64(11))
+    jobs, specs, wants, keep = [], [], [], []
+    for k in range(7):
+        n = int(rng.integers(30, 160))
+        d = synth.neyman_scott(n, nclust=6, sigma=1.5, size=25.0, seed=800 + k)
+        r = synth.uniform_cube(n, 25.0, seed=900 + k)
+        cd, cr = ctx.catalog(d), ctx.catalog(r)
+        keep += [cd, cr]
+        terms = [t for t in tp.FOUR_TERMS if rng.random() < 0.7] or [tp.FOUR_TERMS[3]]
+        jobs.append(tp.make_job([(cd, 0, n), (cr, 0, n)], terms))
+        if k % 2:
+            spec, oaxes = _sqt_spec(rng)
+            wants.append(orc.clean_count([d, r], terms, orc.SQT, oaxes))
+        else:
+            spec = tp.r123_spec(int(rng.integers(8, 25)), 0.5, 40.0, "log")
+            wants.append(orc.clean_count([d, r], terms, orc.R123, [orc.axis(spec.ax[0].n, 0.5, 40.0, "log")]))
+        specs.append(spec)
+    for mode in (tp.MODE_FAST, tp.MODE_FAST64):
+        outs, st = ctx.count_batch(jobs, specs, mode)
+        for k, (o, w) in enumerate(zip(outs, wants)):
+            assert np.array_equal(o, w), (mode, k)
+
+
+def test_count_batch_refuses_jbj(ctx):
+    d = synth.uniform_cube(50, 10.0, seed=950)
+    cd = ctx.catalog(d)
+    job = tp.make_job([(cd, 0, cd.n)], [(0, 0, 0)])
+    spec = tp.binspec(tp.JBJ, [tp.axis(10, 1.0, 10.0, "log"), tp.axis(5, 0.0, 1.0), tp.axis(6, 0.0, 1.2)])
+    with pytest.raises(tp.TpcfError):
+        ctx.count_batch([job], [spec])
+
+
+def test_serving_mix_with_sqt_alone_equals_mix(ctx):
+    """(s,q,theta) and (r1,r2,r3) requests arriving together are batched by
+    parametrization; each result equals the job run alone and the oracle."""
+    print("case serving mix r123/sqt", flush=True)
+    rng = np.random.Generator(np.random.PCG64(13))
+    jobs, specs, wants, keep = [], [], [], []
+    for k in range(10):
+        n = int(rng.integers(40, 200))
+        d = synth.uniform_cube(n, 30.0, seed=1000 + k)
+        r = synth.uniform_cube(n, 30.0, seed=1100 + k)
+        cd, cr = ctx.catalog(d), ctx.catalog(r)
+        keep += [cd, cr]
+        terms = [t for t in tp.FOUR_TERMS if rng.random() < 0.7] or [tp.FOUR_TERMS[0]]
+        jobs.append(tp.make_job([(cd, 0, n), (cr, 0, n)], terms))
+        if k % 3 != 2:
+            spec, oaxes = _sqt_spec(rng)
+            wants.append(orc.clean_count([d, r], terms, orc.SQT, oaxes))
+        else:
+            spec = tp.r123_spec(16, 0.0, 52.0, "lin")
+            wants.append(orc.clean_count([d, r], terms, orc.R123, [orc.axis(16, 0.0, 52.0, "lin")]))
+        specs.append(spec)
+    alone = [ctx.count(j, s)[0] for j, s in zip(jobs, specs)]
+    for a, w in zip(alone, wants):
+        assert np.array_equal(a, w)
+    tickets = [ctx.submit(j, s) for j, s in zip(jobs, specs)]
+    for (t, out), a in zip(tickets, alone):
+        ctx.wait(t)
+        assert np.array_equal(out, a)
