@@ -24,6 +24,11 @@ namespace tpcf {
 
 namespace {
 
+// Term descriptors are read through the constant address space: invariant
+// scalar loads the compiler may repeat where needed instead of keeping every
+// field in registers.
+typedef const SqtTerm __attribute__((address_space(4))) CTerm;
+
 __device__ inline double sq64f(const double *u, int64_t npad, int64_t a, int64_t b) {
     const double dx = u[a] - u[b];
     const double dy = u[npad + a] - u[npad + b];
@@ -154,7 +159,7 @@ __device__ inline int bracket_desc(float c, float d, const float *C, int n, int 
 // Returns the histogram cell, -1 for "out of range" (certain), and sets
 // *unsure when some value lies within its error bound of a deciding edge.
 __device__ inline int entry32(int sbin, float b, float ra, float num, float rden, float ssum, const float *fQ,
-                              const float *fA, const SqtTerm &T, bool *unsure) {
+                              const float *fA, CTerm &T, bool *unsure) {
     const int nq = T.nq, nt = T.nt;
     const float q = b * ra;
     const int mq = bracket_asc(q, 6.f * kU * q, fQ, nq, guess_bin(q, T.q_lo, T.q_inv, T.q_log, nq), unsure);
@@ -260,7 +265,7 @@ struct SqtPending {
 };
 
 // Starts loading the sides of ring entries head .. head+cnt-1, one per lane.
-__device__ inline SqtPending sqt_gather(const SqtTerm &T, const SqtItemCtx &X, const uint32_t *Q,
+__device__ inline SqtPending sqt_gather(CTerm &T, const SqtItemCtx &X, const uint32_t *Q,
                                         uint32_t head, uint32_t qmask, int cnt, int lane) {
     SqtPending p;
     p.live = lane < cnt;
@@ -283,7 +288,7 @@ __device__ inline SqtPending sqt_gather(const SqtTerm &T, const SqtItemCtx &X, c
 }
 
 // Histogram entries of a gathered candidate (FP32 with FP64 recheck).
-__device__ inline void sqt_entries(const SqtTerm &T, const SqtItemCtx &X, const SqtPending &p,
+__device__ inline void sqt_entries(CTerm &T, const SqtItemCtx &X, const SqtPending &p,
                                    const double *sT, const double *sQ, const double *sA, const float *fQ,
                                    const float *fA, uint32_t *hist, unsigned long long &nre) {
     if (!p.live) return;
@@ -376,7 +381,7 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
     // The current term (its job's tables in LDS); flushing writes and clears
     // the cells the term used, so the next term starts from zero.
     int cur = -1;
-    SqtTerm T{};
+    CTerm *Tp = (CTerm *)(P.terms);
     unsigned long long since = 0, nre = 0;
     for (;;) {
         __syncthreads();
@@ -393,11 +398,12 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         const int c1 = __builtin_amdgcn_readfirstlane(I.c1);
         const unsigned long long add = 3ull * (unsigned long long)(c1 - X.c0) * (kBlock * kBlock * kBlock);
         if (term != cur || since + add > 0xC0000000ull) {
-            if (cur >= 0) sqt_flush64(hist, T.cells, T.hist);
+            if (cur >= 0) sqt_flush64(hist, Tp->cells, Tp->hist);
             if (term != cur) {
-                if (nre) atomicAdd(T.rechecked, nre);
+                if (nre) atomicAdd(Tp->rechecked, nre);
                 nre = 0;
-                T = P.terms[term];
+                Tp = (CTerm *)(P.terms) + term;
+                CTerm &T = *Tp;
                 for (int m = threadIdx.x; m <= kMaxBins; m += blockDim.x) {
                     if (m <= T.ns) sT[m] = T.Ts[m];
                     if (m <= T.nq) {
@@ -415,6 +421,7 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
             since = 0;
         }
         since += add;
+        CTerm &T = *Tp;
         const int64_t nblk = T.nblk;
         const int64_t tAB = tile_index(X.A, X.B, nblk) * kTileBytes;
         X.cAB = T.codes + tAB;
@@ -501,8 +508,8 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         }
     }
     if (cur >= 0) {
-        sqt_flush64(hist, T.cells, T.hist);
-        if (nre) atomicAdd(T.rechecked, nre);
+        sqt_flush64(hist, Tp->cells, Tp->hist);
+        if (nre) atomicAdd(Tp->rechecked, nre);
     }
 }
 

@@ -6,8 +6,9 @@
 // job (examples/build_voxel_submission_scripts.py:9-41).  Here a context owns
 // a small pool of workers, each with its own HIP stream and workspace.  An
 // idle worker takes every queued job that fits the memory left in the budget
-// (first fit, in arrival order; a job larger than the whole budget fails with
-// TPCF_EBUSY) and runs them as one batch: pair stages back to back, then one
+// (first fit, in arrival order, up to its share of the queued work, so that
+// the other workers get jobs too; a job larger than the whole budget fails
+// with TPCF_EBUSY) and runs them as one batch: pair stages back to back, then one
 // persistent triplet launch over all their work items (one for the
 // (r1,r2,r3) jobs, one for the (s,q,theta) jobs).  Jobs that arrive while a
 // batch is on the GPU are picked up by another worker as soon as memory
@@ -26,6 +27,7 @@ struct Ticket {
     int mode = TPCF_MODE_FAST;
     uint64_t *hist = nullptr;
     int64_t footprint = 0;
+    uint64_t work = 0;  // unique triplets
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
     int rc = TPCF_OK;
     std::string err;
@@ -52,9 +54,14 @@ static void worker_main(tpcf_ctx *ctx) {
             std::unique_lock<std::mutex> lk(ctx->mu);
             for (;;) {
                 if (ctx->stopping && ctx->pending.empty()) break;
+                // This worker's share of the queued work (triplets).
+                uint64_t queued = 0, taken = 0;
+                for (int64_t id : ctx->pending) queued += ctx->tickets[id]->work;
+                const uint64_t share = queued / kWorkers;
                 // First fit in arrival order; PLAIN and JBJ jobs run
                 // alone.
-                for (auto it = ctx->pending.begin(); it != ctx->pending.end() && (int)batch.size() < kMaxBatch;) {
+                for (auto it = ctx->pending.begin();
+                     it != ctx->pending.end() && (int)batch.size() < kMaxBatch && (batch.empty() || taken < share);) {
                     auto t = ctx->tickets[*it];
                     const bool plain = solo(*t);
                     if (plain && !batch.empty()) { ++it; continue; }
@@ -70,6 +77,7 @@ static void worker_main(tpcf_ctx *ctx) {
                     }
                     if (ctx->in_use + reserved + (uint64_t)t->footprint <= ctx->budget) {
                         reserved += t->footprint;
+                        taken += t->work;
                         t->state = 1;
                         batch.push_back(t);
                         ids.push_back(*it);
@@ -137,7 +145,8 @@ int tpcf_submit(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, in
                 int64_t *ticket) {
     clear_error();
     if (!ctx || !job || !spec || !hist || !ticket) return set_error(TPCF_EINVAL, "NULL argument");
-    const int64_t fp = plan_bytes(*job, *spec, mode);
+    uint64_t work = 0;
+    const int64_t fp = plan_bytes(*job, *spec, mode, &work);
     if (fp < 0) return (int)fp;
     auto t = std::make_shared<Ticket>();
     t->job = *job;
@@ -145,6 +154,7 @@ int tpcf_submit(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, in
     t->mode = mode;
     t->hist = hist;
     t->footprint = fp;
+    t->work = work;
     t->t_submit = std::chrono::steady_clock::now();
     {
         std::lock_guard<std::mutex> g(ctx->mu);

@@ -757,9 +757,10 @@ int run_single(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf_bin
     return run_job(ctx, ws, job, spec, mode, hist, stats, st);
 }
 
-int64_t plan_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode) {
+int64_t plan_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode, uint64_t *triplets) {
     JobPlan P;
     if (int e = plan_job(job, spec, mode, P)) return e;
+    if (triplets) *triplets = P.triplets;
     return (int64_t)P.bytes;
 }
 

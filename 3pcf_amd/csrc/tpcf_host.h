@@ -70,7 +70,9 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
               uint64_t *const *hist, tpcf_stats *per, tpcf_stats *total, hipStream_t st);
 int run_single(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf_binspec &spec, int mode, uint64_t *hist,
                tpcf_stats *stats, hipStream_t st);
-int64_t plan_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode);
+// Device workspace bytes of a job (negative: error); *triplets (if given)
+// receives its unique triplet count.
+int64_t plan_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode, uint64_t *triplets = nullptr);
 Workspace *new_workspace();
 void delete_workspace(Workspace *w);
 struct WorkspaceDeleter {
