@@ -128,7 +128,7 @@ int main(int argc, char **argv) {
         spec.ax[1] = {16, 0.9, 4.1, TPCF_LIN};
         spec.ax[2] = {32, 0.0, 1.0, TPCF_LIN};
     }
-    int mode = -1;  // default: FAST, PLAIN (FP64) for JBJ
+    int mode = -1;  // default: FAST
     int format = triprog ? 0 : 1;
     bool have_r123_range = false;
     double r_lo = 0, r_hi = 0;
@@ -191,7 +191,7 @@ int main(int argc, char **argv) {
         default: break;
         }
     }
-    if (mode < 0) mode = spec.param == TPCF_JBJ ? TPCF_MODE_PLAIN : TPCF_MODE_FAST;
+    if (mode < 0) mode = TPCF_MODE_FAST;
     if (have_r123_range) {
         if (r_n > 0) spec.ax[0].n = r_n;
         if (r_hi > r_lo) {

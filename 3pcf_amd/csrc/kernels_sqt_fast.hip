@@ -88,6 +88,25 @@ __device__ inline void entries_fp64(double s1, double s2, double s3, const doubl
     }
 }
 
+// FP64 definition of one JBJ triangle (the plain kernel's arithmetic): the
+// shortest side's s bin, q = l1/l2 and (l3 - l2)/l1 on ascending axes.
+__device__ inline void entry_jbj_fp64(double s1, double s2, double s3, const double *sT, const double *sQ,
+                                      const double *sV, int ns, int nq, int nt, uint32_t *hist) {
+    if (s1 == 0.0 || s2 == 0.0 || s3 == 0.0) return;
+    double t;
+    if (s1 > s2) { t = s1; s1 = s2; s2 = t; }
+    if (s2 > s3) { t = s2; s2 = s3; s3 = t; }
+    if (s1 > s2) { t = s1; s1 = s2; s2 = t; }
+    const double l1 = sqrt(s1), l2 = sqrt(s2), l3 = sqrt(s3);
+    const int is = bin_asc64(s1, sT, ns);
+    if (is < 0) return;
+    const int iq = bin_asc64(l1 / l2, sQ, nq);
+    if (iq < 0) return;
+    const int iv = bin_asc64((l3 - l2) / l1, sV, nt);
+    if (iv < 0) return;
+    inc(hist, (is * nq + iq) * nt + iv);
+}
+
 constexpr float kU = 5.9604644775390625e-8f;  // 2^-24, FP32 unit roundoff
 
 // Bin hint on an axis: lin (v - lo) * inv, log (log2(v) - lo) * inv; -1 below,
@@ -122,15 +141,15 @@ __device__ inline float acos_over_pi(float c) {
 // that decides it.  A wrong hint always fails the test.
 __device__ inline int bracket_asc(float v, float d, const float *e, int n, int m, bool *unsure) {
     if (m < 0) {
-        if (!(e[0] - v > d + kU * e[0])) *unsure = true;
+        if (!(e[0] - v > d + kU * fabsf(e[0]))) *unsure = true;
         return -1;
     }
     if (m >= n) {
-        if (!(v - e[n] > d + kU * e[n])) *unsure = true;
+        if (!(v - e[n] > d + kU * fabsf(e[n]))) *unsure = true;
         return -1;
     }
     const float a = e[m], b = e[m + 1];
-    if (!(v - a > d + kU * a) || !(b - v > d + kU * b)) *unsure = true;
+    if (!(v - a > d + kU * fabsf(a)) || !(b - v > d + kU * fabsf(b))) *unsure = true;
     return m;
 }
 
@@ -305,7 +324,24 @@ __device__ inline void sqt_entries(CTerm &T, const SqtItemCtx &X, const SqtPendi
     // underflow or overflow below).
     if (!(l2 - l1 > 4.f * kU * l2) || !(l3 - l2 > 4.f * kU * l3) || !(l1 > 1e-18f) || !(l3 < 1e18f)) unsure = true;
     int cell[3] = {-1, -1, -1};
-    if (!unsure) {
+    if (!unsure && T.param == kJbj) {
+        // One entry: the shortest side's s bin, q = l1/l2 and v =
+        // (l3 - l2)/l1.  With every FP32 side within u of the FP64 one:
+        // q = l1 * (1/l2) is within (2u + 3u) q (6u q used); the FP32
+        // difference l3 - l2 is within u (l2 + l3) + u |l3 - l2| of the
+        // FP64 one, and v within u (l2 + l3)/l1 + 5u |v| (2u (l2 + l3)/l1
+        // and 6u |v| used); each edge's own FP32 rounding is added by the
+        // bracket test.
+        if (k1 < kCodeNear) {
+            const float r1 = __builtin_amdgcn_rcpf(l1);
+            const float q = l1 * __builtin_amdgcn_rcpf(l2);
+            const int mq = bracket_asc(q, 6.f * kU * q, fQ, nq, guess_bin(q, T.q_lo, T.q_inv, T.q_log, nq), &unsure);
+            const float v = (l3 - l2) * r1;
+            const float dv = 2.f * kU * (l2 + l3) * r1 + 6.f * kU * fabsf(v);
+            const int mv = bracket_asc(v, dv, fA, nt, guess_bin(v, T.t_lo, T.t_inv, T.t_log, nt), &unsure);
+            if (mq >= 0 && mv >= 0) cell[0] = ((int)k1 * nq + mq) * nt + mv;
+        }
+    } else if (!unsure) {
         const float s1 = l1 * l1, s2 = l2 * l2, s3 = l3 * l3;
         const float ssum = (s1 + s2) + s3;
         if (k1 < kCodeNear) {
@@ -328,8 +364,12 @@ __device__ inline void sqt_entries(CTerm &T, const SqtItemCtx &X, const SqtPendi
         const uint32_t e = p.e;
         const int i = e & 63, j = (e >> 6) & 63, k = (e >> 12) & 63, C = X.c0 + (int)(e >> 18);
         const int64_t gi = (int64_t)X.A * kBlock + i, gj = (int64_t)X.B * kBlock + j, gk = (int64_t)C * kBlock + k;
-        entries_fp64(sq64f(T.u64, T.npad, gi, gj), sq64f(T.u64, T.npad, gi, gk), sq64f(T.u64, T.npad, gj, gk), sT,
-                     sQ, sA, T.ns, nq, nt, hist);
+        const double sij = sq64f(T.u64, T.npad, gi, gj), sik = sq64f(T.u64, T.npad, gi, gk),
+                     sjk = sq64f(T.u64, T.npad, gj, gk);
+        if (T.param == kJbj)
+            entry_jbj_fp64(sij, sik, sjk, sT, sQ, sA, T.ns, nq, nt, hist);
+        else
+            entries_fp64(sij, sik, sjk, sT, sQ, sA, T.ns, nq, nt, hist);
     }
 }
 

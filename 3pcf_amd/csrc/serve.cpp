@@ -10,10 +10,11 @@
 // the other workers get jobs too; a job larger than the whole budget fails
 // with TPCF_EBUSY) and runs them as one batch: pair stages back to back, then one
 // persistent triplet launch over all their work items (one for the
-// (r1,r2,r3) jobs, one for the (s,q,theta) jobs).  Jobs that arrive while a
+// (r1,r2,r3) jobs, one for the triangle parametrizations).  Jobs that arrive
+// while a
 // batch is on the GPU are picked up by another worker as soon as memory
 // allows, so their kernels overlap with the running batch on the device.
-// PLAIN-mode jobs (the FP64 baseline) and JBJ jobs run one per launch.
+// PLAIN-mode jobs (the FP64 baseline) run one per launch.
 #include <chrono>
 #include <cstring>
 
@@ -36,7 +37,7 @@ struct Ticket {
 };
 
 // Jobs that do not join a batched persistent launch.
-static bool solo(const Ticket &t) { return t.mode == TPCF_MODE_PLAIN || t.spec.param == TPCF_JBJ; }
+static bool solo(const Ticket &t) { return t.mode == TPCF_MODE_PLAIN; }
 
 static constexpr int kWorkers = 3;
 static constexpr int kMaxBatch = 64;
@@ -58,8 +59,7 @@ static void worker_main(tpcf_ctx *ctx) {
                 uint64_t queued = 0, taken = 0;
                 for (int64_t id : ctx->pending) queued += ctx->tickets[id]->work;
                 const uint64_t share = queued / kWorkers;
-                // First fit in arrival order; PLAIN and JBJ jobs run
-                // alone.
+                // First fit in arrival order; PLAIN jobs run alone.
                 for (auto it = ctx->pending.begin();
                      it != ctx->pending.end() && (int)batch.size() < kMaxBatch && (batch.empty() || taken < share);) {
                     auto t = ctx->tickets[*it];

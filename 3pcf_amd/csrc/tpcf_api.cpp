@@ -170,9 +170,7 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     if (int e = binspec_check(spec)) return e;
     if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_PLAIN && mode != TPCF_MODE_FAST64)
         return set_error(TPCF_EINVAL, "unknown mode %d", mode);
-    if (spec.param == TPCF_JBJ && mode != TPCF_MODE_PLAIN)
-        return set_error(TPCF_EINVAL, "JBJ histograms run in TPCF_MODE_PLAIN");
-    if (spec.param == TPCF_SQT && mode != TPCF_MODE_PLAIN && spec.ax[0].n > kMaxFastSqtBins)
+    if (spec.param != TPCF_R123 && mode != TPCF_MODE_PLAIN && spec.ax[0].n > kMaxFastSqtBins)
         return set_error(TPCF_EINVAL, "the FAST (s,q,theta) kernel takes up to %d s bins", kMaxFastSqtBins);
     P.L = make_layout(job);
     P.param = spec.param;
@@ -238,16 +236,23 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     P.o_cnt = o; o = align256(o + 64);
     P.o_hist = o; o = align256(o + sizeof(uint64_t) * P.cells * P.nterms);
     P.o_tiles = o; o = align256(o + (P.need_tiles ? (size_t)tile_count(P.L.nblk) * kTileBytes : 0));
-    const bool sqt_fast = spec.param == TPCF_SQT && mode != TPCF_MODE_PLAIN;
+    const bool sqt_fast = spec.param != TPCF_R123 && mode != TPCF_MODE_PLAIN;
     P.o_dists = o; o = align256(o + (sqt_fast ? (size_t)tile_count(P.L.nblk) * kTileBytes * sizeof(float) : 0));
     P.o_masks = o;
     o = align256(o + (sqt_fast ? (size_t)tile_count(P.L.nblk) * kMaskWords * sizeof(unsigned long long) : 0));
-    if (sqt_fast) {
+    if (sqt_fast && spec.param == TPCF_SQT) {
         // No entry of a triangle with a side >= s_hi*(1+q_hi) can be in range
         // (its s side is < s_hi, the q side < q_hi*s_hi, the third side at
         // most their sum); a relative margin keeps the cut conservative.
         const double dmax = spec.ax[0].hi * (1.0 + std::max(spec.ax[1].hi, 1.0)) * (1.0 + 1e-6);
         P.far_s = dmax * dmax;
+    } else if (sqt_fast) {
+        // JBJ: the shortest side l1 < s_hi, l2 <= l1/q_lo and l3 < l2 +
+        // v_hi*l1, so no side of an entry reaches s_hi*(1/q_lo + v_hi); with
+        // q_lo = 0 there is no such bound and no pair is far.
+        const double qlo = spec.ax[1].lo, vhi = std::max(spec.ax[2].hi, 0.0);
+        const double dmax = spec.ax[0].hi * (1.0 / qlo + vhi) * (1.0 + 1e-6);
+        P.far_s = qlo > 0 ? dmax * dmax : HUGE_VAL;
     }
     P.bytes = o;
     return TPCF_OK;
@@ -518,6 +523,7 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
         T.rechecked = reinterpret_cast<unsigned long long *>(base + P.o_cnt);
         T.npad = P.L.npad;
         T.nblk = P.L.nblk;
+        T.param = P.param == TPCF_JBJ ? kJbj : kSqt;
         T.ns = P.n;
         T.nq = P.nq;
         T.nt = P.nt;
@@ -625,7 +631,7 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
         if (P.L.nblk) {
             if (int e = launch_plain(ctx, ws, P, job, base, st)) return e;
         }
-    } else if (P.param == TPCF_SQT && P.L.nblk) {
+    } else if (P.param != TPCF_R123 && P.L.nblk) {
         std::vector<const JobPlan *> plans{&P};
         std::vector<const tpcf_job *> jobs{&job};
         std::vector<char *> bases{base};
@@ -671,8 +677,6 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     size_t bytes = 0;
     std::vector<size_t> off(njobs);
     for (int j = 0; j < njobs; ++j) {
-        if (specs[j].param == TPCF_JBJ)
-            return set_error(TPCF_EINVAL, "batches hold (r1,r2,r3) and (s,q,theta) jobs; count JBJ one by one");
         if (int e = plan_job(jobs[j], specs[j], mode, plans[j])) return e;
         off[j] = bytes;
         bytes = align256(bytes + plans[j].bytes);
@@ -684,12 +688,12 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
         if (int e = stage_job(jobs[j], plans[j], base + off[j], st)) return e;
     HIP_TRY(hipEventRecord(ws.ev[0], st));
     HIP_TRY(hipEventRecord(ws.ev[3], st));
-    // (s,q,theta) jobs: one persistent launch of their own.
+    // (s,q,theta) and JBJ jobs: one persistent launch of their own.
     std::vector<const JobPlan *> sp;
     std::vector<const tpcf_job *> sj;
     std::vector<char *> sb;
     for (int j = 0; j < njobs; ++j)
-        if (plans[j].param == TPCF_SQT) {
+        if (plans[j].param != TPCF_R123) {
             sp.push_back(&plans[j]);
             sj.push_back(&jobs[j]);
             sb.push_back(base + off[j]);

@@ -116,6 +116,7 @@ struct SqtTerm {
     unsigned long long *hist;       // this term's ns*nq*nt cells
     unsigned long long *rechecked;  // the job's count of triangles recomputed in FP64
     int64_t npad, nblk;
+    int param;  // kSqt or kJbj
     int ns, nq, nt, cells;
     int force_fp64;  // every candidate triangle through the FP64 definition
     // Bin guesses of the q and theta/pi axes: lin (v - lo) * inv, log

@@ -73,8 +73,8 @@ enum { TPCF_LIN = 0, TPCF_LOG = 1 };
 /* Device implementations (identical histograms). */
 enum {
     TPCF_MODE_FAST = 0,   /* (r1,r2,r3): FP32 pair stage + FP64 recheck, integer triplet
-                             kernel; (s,q,theta): FP64 pair stage, FP32 entry arithmetic
-                             with FP64 recheck of triangles near an edge.  Not JBJ. */
+                             kernel; (s,q,theta) and JBJ: FP64 pair stage, FP32 entry
+                             arithmetic with FP64 recheck of triangles near an edge. */
     TPCF_MODE_PLAIN = 1,  /* FP64 distances recomputed per triplet, one job per launch */
     TPCF_MODE_FAST64 = 2  /* as FAST with every decision in FP64 */
 };
@@ -109,8 +109,8 @@ typedef struct {
 typedef struct {
     uint64_t triplets;         /* unique triplets classified (all terms) */
     uint64_t pairs;            /* pairs binned in the pair stage */
-    uint64_t pairs_rechecked;  /* FAST: pairs (r1,r2,r3) or triangles (s,q,theta)
-                                  recomputed in FP64 */
+    uint64_t pairs_rechecked;  /* FAST: pairs ((r1,r2,r3)) or triangles ((s,q,theta),
+                                  JBJ) recomputed in FP64 */
     double device_ms;          /* device time of the counting stages (HIP events) */
     double wall_ms;            /* host wall time of the call (serving: submit to done) */
     double pair_ms;            /* pair stage (FAST modes) */
@@ -169,8 +169,9 @@ int tpcf_count(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int
 /* Count many jobs (for instance every voxel and term of a survey) in one
  * persistent device launch.  hist[j] receives job j's histograms.  Replaces
  * the per-voxel process fan-out of examples/build_voxel_submission_scripts.py
- * :9-41 and examples/run_CPU_calculation_each_voxels.csh:60-67.  (r1,r2,r3)
- * jobs; mode must be TPCF_MODE_FAST or TPCF_MODE_FAST64. */
+ * :9-41 and examples/run_CPU_calculation_each_voxels.csh:60-67.  Jobs of any
+ * parametrization (one persistent launch per family); mode must be
+ * TPCF_MODE_FAST or TPCF_MODE_FAST64. */
 int tpcf_count_batch(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs,
                      int mode, uint64_t *const *hist, tpcf_stats *stats);
 

@@ -137,13 +137,24 @@ def test_count_batch_mixed_parametrizations(ctx):
             assert np.array_equal(o, w), (mode, k)
 
 
-def test_count_batch_refuses_jbj(ctx):
-    d = synth.uniform_cube(50, 10.0, seed=950)
-    cd = ctx.catalog(d)
-    job = tp.make_job([(cd, 0, cd.n)], [(0, 0, 0)])
-    spec = tp.binspec(tp.JBJ, [tp.axis(10, 1.0, 10.0, "log"), tp.axis(5, 0.0, 1.0), tp.axis(6, 0.0, 1.2)])
-    with pytest.raises(tp.TpcfError):
-        ctx.count_batch([job], [spec])
+def test_count_batch_with_jbj_and_sqt(ctx):
+    """JBJ and (s,q,theta) jobs share one persistent launch (the term
+    descriptors carry the parametrization)."""
+    print("case count_batch jbj+sqt", flush=True)
+    jaxes = [(10, 1.0, 10.0, "log"), (5, 0.0, 1.0, "lin"), (6, 0.0, 1.2, "lin")]
+    saxes = [(8, 1.0, 10.0, "log"), (6, 1.0, 3.0, "lin"), (9, 0.0, 1.0, "lin")]
+    jobs, specs, wants, keep = [], [], [], []
+    for k in range(4):
+        d = synth.uniform_cube(60 + 20 * k, 10.0, seed=950 + k)
+        cd = ctx.catalog(d)
+        keep.append(cd)
+        jobs.append(tp.make_job([(cd, 0, cd.n)], [(0, 0, 0)]))
+        axes, param, oparam = (jaxes, tp.JBJ, orc.JBJ) if k % 2 == 0 else (saxes, tp.SQT, orc.SQT)
+        specs.append(tp.binspec(param, [tp.axis(*a) for a in axes]))
+        wants.append(orc.clean_count([d], [(0, 0, 0)], oparam, [orc.axis(*a) for a in axes]))
+    outs, _ = ctx.count_batch(jobs, specs)
+    for o, w in zip(outs, wants):
+        assert np.array_equal(o, w)
 
 
 def test_serving_mix_with_sqt_alone_equals_mix(ctx):

@@ -45,12 +45,33 @@ def test_sqt_four_terms(ctx, axes, mode):
     assert int(got.sum()) > 0
 
 
-def test_jbj(ctx):
-    print("case jbj", flush=True)
+@pytest.mark.parametrize("mode", SQT_MODES)
+def test_jbj(ctx, mode):
+    print(f"case jbj mode={mode}", flush=True)
     d = synth.uniform_cube(200, 20.0, seed=23)
-    got, _ = run(ctx, [d], [(0, 0, 0)], tp.JBJ, JBJ_SPEC)
+    got, _ = run(ctx, [d], [(0, 0, 0)], tp.JBJ, JBJ_SPEC, mode)
     assert np.array_equal(got, oracle([d], [(0, 0, 0)], orc.JBJ, JBJ_SPEC))
     assert int(got.sum()) > 0
+
+
+# q = l1/l2 bounded below (a far cut exists), and a lin v axis starting below 0.
+JBJ_SPEC2 = [(12, 1.0, 15.0, "log"), (8, 0.3, 1.0, "lin"), (10, -0.2, 1.5, "lin")]
+
+
+@pytest.mark.parametrize("mode", SQT_MODES)
+def test_jbj_four_terms_lattice_and_far_cut(ctx, mode):
+    print(f"case jbj four terms mode={mode}", flush=True)
+    d = synth.neyman_scott(170, nclust=8, sigma=2.0, size=40.0, seed=61)
+    r = synth.uniform_cube(150, 40.0, seed=62)
+    for spec in (JBJ_SPEC, JBJ_SPEC2):
+        got, _ = run(ctx, [d, r], tp.FOUR_TERMS, tp.JBJ, spec, mode)
+        assert np.array_equal(got, oracle([d, r], tp.FOUR_TERMS, orc.JBJ, spec))
+    g = np.stack(np.meshgrid(np.arange(4), np.arange(4), np.arange(3), indexing="ij"), -1).reshape(-1, 3) * 2.0
+    axes = [(6, 1.0, 7.0, "lin"), (4, 0.25, 1.0, "lin"), (4, 0.0, 1.0, "lin")]
+    got, st = run(ctx, [g.astype(np.float64)], [(0, 0, 0)], tp.JBJ, axes, mode)
+    assert np.array_equal(got, oracle([g.astype(np.float64)], [(0, 0, 0)], orc.JBJ, axes))
+    if mode == tp.MODE_FAST:
+        assert st.pairs_rechecked > 0
 
 
 @pytest.mark.parametrize("mode", SQT_MODES)
@@ -93,10 +114,13 @@ def test_sqt_fast_far_from_origin_and_ragged(ctx):
     assert np.array_equal(got, oracle([a, b], terms, orc.SQT, axes))
 
 
-def test_jbj_fast_mode_refused(ctx):
+def test_sqt_fast_rejects_too_many_s_bins(ctx):
     d = synth.uniform_cube(20, 8.0, seed=25)
+    axes = [(254, 1.0, 8.0, "lin"), (2, 1.0, 3.0, "lin"), (2, 0.0, 1.0, "lin")]
     with pytest.raises(tp.TpcfError):
-        run(ctx, [d], [(0, 0, 0)], tp.JBJ, JBJ_SPEC, mode=tp.MODE_FAST)
+        run(ctx, [d], [(0, 0, 0)], tp.SQT, axes, mode=tp.MODE_FAST)
+    got, _ = run(ctx, [d], [(0, 0, 0)], tp.SQT, axes, mode=tp.MODE_PLAIN)
+    assert np.array_equal(got, oracle([d], [(0, 0, 0)], orc.SQT, axes))
 
 
 def _tri_q_theta(p, i, j, k):

@@ -1,4 +1,5 @@
-"""The FP32 error bounds of the FAST (s,q,theta) entries, checked on the CPU.
+"""The FP32 error bounds of the FAST (s,q,theta) and JBJ entries, checked on
+the CPU.
 
 The device kernel (3pcf_amd/csrc/kernels_sqt_fast.hip, entry32 / bracket_asc /
 bracket_desc / sqt_entries) decides an entry in FP32 only when every value is
@@ -10,8 +11,8 @@ the exact value, the 1-ulp accuracy of v_rcp_f32, chosen at random) and
 checks, over random and deliberately near-edge triangles, that every entry
 the FP32 path claims to be certain of equals the FP64 definition (the clean64
 oracle's arithmetic: src/3pcf_C_version_triangle_parametrization.cc:188-236
-restated with sorted squared sides, q = l2/l1 and the cosine rule).  A bound
-derived too tight fails here.
+restated with sorted squared sides, q = l2/l1 and the cosine rule; for JBJ
+q = l1/l2 and (l3 - l2)/l1).  A bound derived too tight fails here.
 """
 import math
 
@@ -57,11 +58,11 @@ def _bracket_asc32(v, d, e32, m, scale=1):
     mid = ~below & ~above
     e0, en = e32[0], e32[n]
     g = F(scale)
-    certain &= ~below | ((e0 - v) > g * (d + U * e0))
-    certain &= ~above | ((v - en) > g * (d + U * en))
+    certain &= ~below | ((e0 - v) > g * (d + U * np.abs(e0)))
+    certain &= ~above | ((v - en) > g * (d + U * np.abs(en)))
     mm = np.clip(m, 0, n - 1)
     a, b = e32[mm], e32[mm + 1]
-    certain &= ~mid | (((v - a) > g * (d + U * a)) & ((b - v) > g * (d + U * b)))
+    certain &= ~mid | (((v - a) > g * (d + U * np.abs(a))) & ((b - v) > g * (d + U * np.abs(b))))
     return np.where(mid, m, -1), certain
 
 
@@ -224,3 +225,91 @@ def test_check_detects_a_bound_too_tight():
     assert _mismatches(P, axes, rng)[0].size == 0
     assert _mismatches(P, axes, rng, qscale=0)[0].size > 0
     assert _mismatches(P, axes, rng, cscale=0)[0].size > 0
+
+
+def _jbj_entries(points, axes, rng, qscale=1, vscale=1):
+    """FP64 cell and FP32-path (cell, certain) of the JBJ entry of every
+    triangle: the shortest side's s bin, q = l1/l2 and v = (l3 - l2)/l1."""
+    ea = [orc.edges(orc.axis(*a)) for a in axes]
+    T = orc.sq_thresholds(ea[0])
+    Eq, Ev = ea[1], ea[2]
+    nq, nt = axes[1][0], axes[2][0]
+    P = points
+    def sq(a, b):
+        d = P[:, a] - P[:, b]
+        return (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    s = np.stack([sq(0, 1), sq(0, 2), sq(1, 2)], axis=1)
+    sbin = _bin_asc(s, T)
+    s64 = np.sort(s, axis=1)
+    L = np.sqrt(s64)
+    b1 = _bin_asc(s64[:, 0], T)
+    iq = _bin_asc(L[:, 0] / L[:, 1], Eq)
+    iv = _bin_asc((L[:, 2] - L[:, 1]) / L[:, 0], Ev)
+    ok = (b1 >= 0) & (iq >= 0) & (iv >= 0)
+    ref = np.where(ok, (b1 * nq + iq) * nt + iv, -1)
+    l32 = np.sqrt(s).astype(np.float32)
+    o32 = np.argsort(l32, axis=1, kind="stable")
+    l = np.take_along_axis(l32, o32, 1)
+    k1 = np.take_along_axis(sbin, o32, 1)[:, 0]
+    l1, l2, l3 = l[:, 0], l[:, 1], l[:, 2]
+    certain = ((l2 - l1) > F(4) * U * l2) & ((l3 - l2) > F(4) * U * l3) & (l1 > F(1e-18)) & (l3 < F(1e18))
+    fQ, fV = Eq.astype(np.float32), Ev.astype(np.float32)
+    r1 = _faithful_rcp(l1, rng)
+    q = l1 * _faithful_rcp(l2, rng)
+    mq, okq = _bracket_asc32(q, F(6) * U * q, fQ, _fp32_hint_asc(q, fQ), qscale)
+    v = (l3 - l2) * r1
+    dv = F(2) * U * (l2 + l3) * r1 + F(6) * U * np.abs(v)
+    mv, okv = _bracket_asc32(v, dv, fV, _fp32_hint_asc(v, fV), vscale)
+    has = k1 >= 0
+    fast = np.where(has & (mq >= 0) & (mv >= 0), (k1 * nq + mq) * nt + mv, -1)
+    certain &= ~has | (okq & okv)
+    return ref, fast, certain
+
+
+JBJ_AXES = [
+    [(20, 1.0, 30.0, "log"), (10, 0.0, 1.0, "lin"), (12, 0.0, 1.2, "lin")],
+    [(12, 1.0, 15.0, "log"), (8, 0.3, 1.0, "lin"), (10, -0.2, 1.5, "lin")],
+    [(10, 2.0, 12.0, "lin"), (6, 0.2, 1.0, "log"), (7, 0.01, 1.0, "log")],
+]
+
+
+def _jbj_near_edge(axes, n, rng, offset):
+    """Triangles with q = l1/l2 or v = (l3 - l2)/l1 within about 1e-7
+    (relative) of an edge, built from their three side lengths."""
+    ea = [orc.edges(orc.axis(*a)) for a in axes]
+    l1 = rng.uniform(2.5, 9.0, n)
+    qe = ea[1][(ea[1] > 0.05) & (ea[1] < 0.999)]
+    ve = ea[2][(ea[2] > 0.01) & (ea[2] < 0.95)]
+    pick = rng.random(n) < 0.5
+    q = np.where(pick, rng.choice(qe, n) * (1.0 + rng.uniform(-3e-7, 3e-7, n)), rng.uniform(0.35, 0.95, n))
+    v = np.where(pick, rng.uniform(0.05, 0.9, n), rng.choice(ve, n) * (1.0 + rng.uniform(-3e-7, 3e-7, n)))
+    l2 = l1 / q
+    l3 = l2 + v * l1
+    x = (l1 * l1 + l2 * l2 - l3 * l3) / (2.0 * l1)
+    y = np.sqrt(np.maximum(l2 * l2 - x * x, 0.0))
+    pts = np.stack([np.zeros((n, 3)), np.stack([l1, np.zeros(n), np.zeros(n)], 1),
+                    np.stack([x, y, np.zeros(n)], 1)], axis=1)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    R = np.stack([np.stack([np.cos(phi), -np.sin(phi), np.zeros(n)], 1),
+                  np.stack([np.sin(phi), np.cos(phi), np.zeros(n)], 1),
+                  np.stack([np.zeros(n), np.zeros(n), np.ones(n)], 1)], 1)
+    return pts @ np.transpose(R, (0, 2, 1)) + offset + rng.random((n, 1, 3))
+
+
+def test_jbj_certain_entries_match_fp64():
+    rng = np.random.default_rng(17)
+    for axes in JBJ_AXES:
+        for offset in (0.0, 600.0):
+            for P in (rng.random((200000, 3, 3)) * 25.0 + offset, _jbj_near_edge(axes, 200000, rng, offset)):
+                ref, fast, certain = _jbj_entries(P, axes, rng)
+                bad = np.nonzero(certain & (ref != fast))[0]
+                assert bad.size == 0, (axes, offset, bad.size)
+
+
+def test_jbj_check_detects_a_bound_too_tight():
+    rng = np.random.default_rng(19)
+    axes = JBJ_AXES[1]
+    P = _jbj_near_edge(axes, 200000, rng, 600.0)
+    for margins in ({"qscale": 0}, {"vscale": 0}):
+        ref, fast, certain = _jbj_entries(P, axes, rng, **margins)
+        assert np.any(certain & (ref != fast)), margins
