@@ -12,6 +12,7 @@
 #   _ref/trij32     3pcf_C_version_triangle_parametrization_JBJ
 #   _ref/plain      3pcf_C_version                              (64 bins, as shipped)
 #   _ref/plain16    3pcf_C_version with DEFAULT_NBINS 16
+#   _ref/two        2pcf_C_version                              (700 bins on [0,70), as shipped)
 set -euo pipefail
 SRC=${REF_SRC:-/root/reference/src}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -30,4 +31,5 @@ $CXX $FLAGS -o "$OUT/trij32" "$SRC/3pcf_C_version_triangle_parametrization_JBJ.c
 $CXX $FLAGS -o "$OUT/plain" "$SRC/3pcf_C_version.cc" -lm
 sed 's/^#define DEFAULT_NBINS 64/#define DEFAULT_NBINS 16/' "$SRC/3pcf_C_version.cc" \
     | $CXX $FLAGS -x c++ -o "$OUT/plain16" - -lm
+$CXX $FLAGS -o "$OUT/two" "$SRC/2pcf_C_version.cc" -lm
 echo "built: $(ls "$OUT" | tr '\n' ' ')"

@@ -75,6 +75,11 @@ def load() -> C.CDLL:
         lib.orc_ref_which.restype = C.c_int
         lib.orc_ref_vox2gal.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
         lib.orc_ref_vox2gal.restype = C.c_int
+        lib.orc_clean_pairs.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(OrcAxis),
+                                        C.c_void_p, C.c_int]
+        lib.orc_clean_pairs.restype = C.c_int
+        lib.orc_ref_two.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        lib.orc_ref_two.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -137,3 +142,30 @@ def ref_tri(cats: Sequence[np.ndarray], which: int, variant: int, axes, kmin_leg
     numcalc = C.c_uint64()
     lib.orc_ref_tri(ptrs, ns, which, variant, kmin_legacy, ax, h.ctypes.data, C.byref(numcalc), nthreads)
     return h, numcalc.value
+
+
+def clean_pairs(segments: Sequence[np.ndarray], terms, ax: OrcAxis, nthreads: int = 8) -> np.ndarray:
+    """clean64 pair-count histograms, shape (nterms, ax.n), uint64; a term
+    (a, b) counts the unique pairs of segment a (a == b) or every pair of
+    segments a and b."""
+    lib = load()
+    soas = [np.ascontiguousarray(np.asarray(p, dtype=np.float64).T) for p in segments]
+    ptrs = (C.c_void_p * 2)(*([s.ctypes.data for s in soas] + [None] * (2 - len(soas))))
+    ns = (C.c_int64 * 2)(*([s.shape[1] for s in soas] + [0] * (2 - len(soas))))
+    terms = list(terms)
+    tarr = (C.c_int * (2 * len(terms)))(*[v for t in terms for v in t[:2]])
+    out = np.zeros((len(terms), ax.n), dtype=np.uint64)
+    if lib.orc_clean_pairs(ptrs, ns, len(segments), tarr, len(terms), C.byref(ax), out.ctypes.data, nthreads):
+        raise ValueError("bad oracle pair job")
+    return out
+
+
+def ref_two(cats: Sequence[np.ndarray], which: int, nthreads: int = 8) -> np.ndarray:
+    """refcompat32 pair counts of the reference's 2pcf program (700 uint32)."""
+    lib = load()
+    soas = [np.ascontiguousarray(np.asarray(c, dtype=np.float32).T) for c in cats]
+    ptrs = (C.c_void_p * 2)(*[s.ctypes.data for s in soas])
+    ns = (C.c_int * 2)(*[s.shape[1] for s in soas])
+    h = np.zeros(700, dtype=np.uint32)
+    lib.orc_ref_two(ptrs, ns, which, h.ctypes.data, nthreads)
+    return h

@@ -9,6 +9,8 @@
  *   orc_cli ref-jbj   [opts] f0 f1 f2   src/3pcf_C_version_triangle_parametrization_JBJ.cc
  *   orc_cli ref-plain [opts] f0 f1 f2   src/3pcf_C_version.cc:147-466
  *   orc_cli clean-r123|clean-sqt|clean-jbj [opts] f0 f1 f2   (clean64 semantics)
+ *   orc_cli ref-two   [opts] f0 f1      src/2pcf_C_version.cc:124-411
+ *   orc_cli clean-two [opts] f0 f1      pair counts, clean64 semantics
  *
  * Options: -o out, -X div, -x AAABBBCCC, -s, -m, --threads N, --kmin-legacy,
  *   --axis IDX,N,LO,HI[,log]   bin spec of axis IDX (0..2),
@@ -39,6 +41,94 @@ static int parse_axis(const char *arg, int *idx, int *n, double *lo, double *hi,
     *hi = atof(tok[3]);
     *lg = nt == 5 && strcmp(tok[4], "log") == 0;
     return (*idx < 0 || *idx > 2) ? -1 : 0;
+}
+
+/* Pair counts: two catalogs, -X/-x decode two voxel indices
+ * (src/2pcf_C_version.cc:187-193). */
+static int two_main(const char *mode, int npath, char **paths, const char *out, int vdiv, int vtemp, int nthreads,
+                    int an, double alo, double ahi, int alog)
+{
+    if (npath < 2) {
+        fprintf(stderr, "need two catalog paths\n");
+        return 2;
+    }
+    const char *path[2] = {paths[0], paths[1]};
+    int vidx[2] = {0, 0};
+    if (vdiv > 0) {
+        vidx[0] = vtemp / 1000000;
+        vidx[1] = (vtemp / 1000) % 1000;
+    }
+    const int is_ref = strcmp(mode, "ref-two") == 0;
+    void *cat[2];
+    int64_t ng[2];
+    for (int i = 0; i < 2; i++) {
+        ng[i] = orc_read_7col(path[i], !is_ref, &cat[i]);
+        if (ng[i] < 0) {
+            fprintf(stderr, "cannot read %s\n", path[i]);
+            return 1;
+        }
+    }
+    int lo[2], hi[2];
+    for (int i = 0; i < 2; i++) {
+        lo[i] = 0;
+        hi[i] = (int)ng[i];
+        if (vdiv > 0 && vtemp >= 0) {
+            int lh[2];
+            if (orc_ref_vox2gal(vdiv, vidx[i], (int)ng[i], lh)) return 1;
+            lo[i] = lh[0];
+            hi[i] = lh[1];
+        }
+    }
+    if (is_ref) {
+        const float *p[2];
+        int n[2];
+        for (int i = 0; i < 2; i++) {
+            const float *P = cat[i];
+            int m = hi[i] - lo[i];
+            float *Q = malloc(sizeof(float) * 3 * (size_t)(m > 0 ? m : 1));
+            for (int t = 0; t < m; t++) {
+                Q[t] = P[lo[i] + t];
+                Q[m + t] = P[ng[i] + lo[i] + t];
+                Q[2 * m + t] = P[2 * ng[i] + lo[i] + t];
+            }
+            p[i] = Q;
+            n[i] = m;
+        }
+        uint32_t h32[700];
+        orc_ref_two(p, n, strcmp(path[0], path[1]) == 0 ? 0 : 1, h32, nthreads);
+        uint64_t h[700];
+        for (int b = 0; b < 700; b++) h[b] = h32[b];
+        return orc_write_two(out, ng, 700, 0.0, 70.0, (70.0 - 0.0) / 700, h) ? 1 : 0;
+    }
+    orc_axis ax = {700, 0.0, 70.0, ORC_LIN};
+    if (an > 0) ax = (orc_axis){an, alo, ahi, alog ? ORC_LOG : ORC_LIN};
+    char *rp0 = realpath(path[0], NULL), *rp1 = realpath(path[1], NULL);
+    if (!rp0 || !rp1) return 1;
+    const int same_file = strcmp(rp0, rp1) == 0;
+    const int same = same_file && lo[0] == lo[1];
+    const int empty = same_file && lo[0] > lo[1];
+    double *seg[2];
+    int64_t nseg[2];
+    for (int i = 0; i < 2; i++) {
+        int m = hi[i] - lo[i];
+        const double *P = cat[i];
+        seg[i] = malloc(sizeof(double) * 3 * (size_t)(m > 0 ? m : 1));
+        for (int t = 0; t < m; t++) {
+            seg[i][t] = P[lo[i] + t];
+            seg[i][m + t] = P[ng[i] + lo[i] + t];
+            seg[i][2 * m + t] = P[2 * ng[i] + lo[i] + t];
+        }
+        nseg[i] = m;
+    }
+    uint64_t *h = calloc((size_t)ax.n, 8);
+    const int term[1][2] = {{0, same ? 0 : 1}};
+    const double *sp[2] = {seg[0], seg[1]};
+    if (!empty && orc_clean_pairs(sp, nseg, same ? 1 : 2, term, 1, &ax, h, nthreads)) {
+        fprintf(stderr, "bad job\n");
+        return 1;
+    }
+    const double w = ax.scale == ORC_LOG ? log(ax.hi / ax.lo) / ax.n : (ax.hi - ax.lo) / ax.n;
+    return orc_write_two(out, ng, ax.n, ax.lo, ax.hi, w, h) ? 1 : 0;
 }
 
 int main(int argc, char **argv)
@@ -81,6 +171,9 @@ int main(int argc, char **argv)
         default: break;
         }
     }
+    if (strcmp(mode, "ref-two") == 0 || strcmp(mode, "clean-two") == 0)
+        return two_main(mode, argc - optind, argv + optind, out, vdiv, vtemp, nthreads, ax_n[0], ax_lo[0],
+                        ax_hi[0], ax_log[0]);
     if (argc - optind < 3) {
         fprintf(stderr, "need three catalog paths\n");
         return 2;

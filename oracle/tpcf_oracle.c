@@ -424,6 +424,69 @@ int orc_ref_plain(const float *const pts[3], const int n[3], int nbins, float hi
     return 0;
 }
 
+/* ---- pair counts: src/2pcf_C_version.cc ------------------------------- */
+
+typedef struct {
+    const float *p[2];
+    int n[2];
+    int which;
+    uint32_t **th_hist;
+} ref_two_ctx;
+
+/* distance_to_bin (:58-85) with the program's S_LO 0, S_HI 70, S_NBINS 700
+ * and flag 0: float width, -999 outside [lo, hi), int((d - lo)/w). */
+static int ref_two_bin(float d)
+{
+    const float lo = 0.0f, hi = 70.0f;
+    const int nbins = 700;
+    const float w = (hi - lo) / nbins;
+    if (d < lo) return -999;
+    if (d >= hi) return -999;
+    return (int)((d - lo) / w);
+}
+
+static void ref_two_body(void *vctx, int tid, long long ilo, long long ihi)
+{
+    ref_two_ctx *c = (ref_two_ctx *)vctx;
+    uint32_t *h = c->th_hist[tid];
+    const float *A = c->p[0], *B = c->p[1];
+    const int na = c->n[0], nb = c->n[1];
+    for (long long i = ilo; i < ihi; i++) {
+        /* :358-364: same file -> j from i (self pairs included); else 0 */
+        int jmin = c->which == 0 ? (int)i : 0;
+        for (int j = jmin; j < nb; j++) {
+            float xd = A[i] - B[j];
+            float yd = A[na + i] - B[nb + j];
+            float zd = A[2 * na + i] - B[2 * nb + j];
+            float d = sqrtf(xd * xd + yd * yd + zd * zd);
+            int b = ref_two_bin(d);
+            if (b >= 0) h[b]++;
+        }
+    }
+}
+
+int orc_ref_two(const float *const pts[2], const int n[2], int which, uint32_t hist[700], int nthreads)
+{
+    ref_two_ctx c;
+    memset(&c, 0, sizeof c);
+    for (int i = 0; i < 2; i++) {
+        c.p[i] = pts[i];
+        c.n[i] = n[i];
+    }
+    c.which = which;
+    if (nthreads < 1) nthreads = 1;
+    c.th_hist = (uint32_t **)calloc((size_t)nthreads, sizeof(uint32_t *));
+    for (int t = 0; t < nthreads; t++) c.th_hist[t] = (uint32_t *)calloc(700, 4);
+    run_pool(0, n[0], 1, nthreads, &c, ref_two_body);
+    memset(hist, 0, 700 * 4);
+    for (int t = 0; t < nthreads; t++) {
+        for (int b = 0; b < 700; b++) hist[b] += c.th_hist[t][b];
+        free(c.th_hist[t]);
+    }
+    free(c.th_hist);
+    return 0;
+}
+
 /* ======================================================================== */
 /* clean64                                                                  */
 /* ======================================================================== */
@@ -431,7 +494,7 @@ int orc_ref_plain(const float *const pts[3], const int n[3], int nbins, float hi
 int orc_edges(const orc_axis *ax, double *e)
 {
     int n = ax->n;
-    if (n < 1 || n > 255 || !(ax->hi > ax->lo)) return -1;
+    if (n < 1 || n > 4096 || !(ax->hi > ax->lo)) return -1;
     if (ax->scale == ORC_LOG) {
         if (!(ax->lo > 0)) return -1;
         double a = log(ax->lo), b = log(ax->hi);
@@ -615,7 +678,7 @@ int orc_clean_count(const orc_job *job, uint64_t *hist, int nthreads)
     const int naxes = job->param == ORC_R123 ? 1 : 3;
     double e[3][256];
     for (int a = 0; a < naxes; a++)
-        if (orc_edges(&job->ax[a], e[a])) return -1;
+        if (job->ax[a].n > 255 || orc_edges(&job->ax[a], e[a])) return -1;
     int64_t cells = orc_hist_cells(job);
     if (nthreads < 1) nthreads = 1;
     clean_ctx *c = (clean_ctx *)calloc(1, sizeof(clean_ctx));
@@ -664,6 +727,74 @@ int orc_clean_count(const orc_job *job, uint64_t *hist, int nthreads)
     return 0;
 }
 
+typedef struct {
+    const double *A, *B;
+    int64_t na, nb;
+    int same;
+    const double *T;
+    int n;
+    uint64_t **th_hist;
+} clean_pairs_ctx;
+
+static void clean_pairs_body(void *vctx, int tid, long long ilo, long long ihi)
+{
+    clean_pairs_ctx *c = (clean_pairs_ctx *)vctx;
+    uint64_t *h = c->th_hist[tid];
+    for (long long i = ilo; i < ihi; i++) {
+        int64_t j0 = c->same ? i + 1 : 0;
+        for (int64_t j = j0; j < c->nb; j++) {
+            int b = orc_bin_sq(sqdist(c->A, c->na, i, c->B, c->nb, j), c->T, c->n);
+            if (b >= 0) h[b]++;
+        }
+    }
+}
+
+int orc_clean_pairs(const double *const seg[2], const int64_t nseg_pts[2], int nseg, const int term[][2],
+                    int nterms, const orc_axis *ax, uint64_t *hist, int nthreads)
+{
+    if (nseg < 1 || nseg > 2 || nterms < 1 || nterms > 3) return -1;
+    const int n = ax->n;
+    double *e = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *T = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    if (!e || !T || orc_edges(ax, e)) {
+        free(e);
+        free(T);
+        return -1;
+    }
+    orc_sq_thresholds(e, n, T);
+    if (nthreads < 1) nthreads = 1;
+    clean_pairs_ctx c;
+    memset(&c, 0, sizeof c);
+    c.T = T;
+    c.n = n;
+    c.th_hist = (uint64_t **)calloc((size_t)nthreads, sizeof(uint64_t *));
+    for (int t = 0; t < nthreads; t++) c.th_hist[t] = (uint64_t *)calloc((size_t)n, 8);
+    int rc = 0;
+    for (int tm = 0; tm < nterms && rc == 0; tm++) {
+        const int a = term[tm][0], b = term[tm][1];
+        if (a < 0 || b < a || b >= nseg) {
+            rc = -1;
+            break;
+        }
+        c.A = seg[a];
+        c.B = seg[b];
+        c.na = nseg_pts[a];
+        c.nb = nseg_pts[b];
+        c.same = a == b;
+        for (int t = 0; t < nthreads; t++) memset(c.th_hist[t], 0, (size_t)n * 8);
+        run_pool(0, c.na, 1, nthreads, &c, clean_pairs_body);
+        uint64_t *out = hist + (int64_t)tm * n;
+        memset(out, 0, (size_t)n * 8);
+        for (int t = 0; t < nthreads; t++)
+            for (int x = 0; x < n; x++) out[x] += c.th_hist[t][x];
+    }
+    for (int t = 0; t < nthreads; t++) free(c.th_hist[t]);
+    free(c.th_hist);
+    free(e);
+    free(T);
+    return rc;
+}
+
 /* ======================================================================== */
 /* Histogram writers                                                        */
 /* ======================================================================== */
@@ -705,5 +836,17 @@ int orc_write_plain(const char *path, int n, const uint64_t *hist)
             fprintf(f, "\n");
         }
     }
+    return fclose(f) ? -1 : 0;
+}
+
+/* Pair-count format of src/2pcf_C_version.cc:383-397. */
+int orc_write_two(const char *path, const int64_t ngal[2], int nbins, double lo, double hi, double width,
+                  const uint64_t *hist)
+{
+    FILE *f = fopen(path, "w");
+    if (!f) return -1;
+    fprintf(f, "%lld\n%lld\n%d\n", (long long)ngal[0], (long long)ngal[1], nbins);
+    fprintf(f, "%-6.3f %-6.3f %-6.3f\n", lo, hi, width);
+    for (int b = 0; b < nbins; b++) fprintf(f, "%llu\n", (unsigned long long)hist[b]);
     return fclose(f) ? -1 : 0;
 }

@@ -16,6 +16,7 @@
  *                 - (s,q,theta)  src/3pcf_C_version_triangle_parametrization.cc
  *                 - JBJ variant  src/3pcf_C_version_triangle_parametrization_JBJ.cc
  *                 - (r1,r2,r3)   src/3pcf_C_version.cc
+ *                 - pair counts  src/2pcf_C_version.cc
  *
  *  clean64      The FP64 semantics the GPU kernels must reproduce bit-exactly:
  *               unique triplets, runtime bin specs shared through edge tables,
@@ -96,6 +97,16 @@ int orc_ref_tri(const float *const pts[3], const int n[3], int which, int varian
 int orc_ref_plain(const float *const pts[3], const int n[3], int nbins, float hist_min,
                   float hist_max, uint64_t *hist, uint64_t *oob, int nthreads);
 
+/* Pair counts of src/2pcf_C_version.cc (main loop :346-378): FP32
+ * distances sqrt(dx*dx + dy*dy + dz*dz), 700 linear bins on [0,70) with the
+ * float width 0.1 and int truncation (:58-85), out of range dropped; same
+ * file (which = 0, strcmp of the two paths, :213-224): j from i, so each
+ * point's self pair lands in bin 0; different files (which = 1): every
+ * pair.  Counters are the reference's unsigned int (wrap).  Pinned: the
+ * reference compiled from source (oracle/_ref/two) on sample catalogs,
+ * byte for byte through orc_write_two. */
+int orc_ref_two(const float *const pts[2], const int n[2], int which, uint32_t hist[700], int nthreads);
+
 /* ------------------------------------------------------------------------ */
 /* clean64                                                                   */
 /* ------------------------------------------------------------------------ */
@@ -103,7 +114,7 @@ int orc_ref_plain(const float *const pts[3], const int n[3], int nbins, float hi
 enum { ORC_LIN = 0, ORC_LOG = 1 };
 
 typedef struct {
-    int n;       /* 1..255 */
+    int n;       /* 1..255 for triplets, 1..4096 for pair counts */
     double lo, hi;
     int scale;   /* ORC_LIN or ORC_LOG */
 } orc_axis;
@@ -150,6 +161,16 @@ int orc_clean_count(const orc_job *job, uint64_t *hist, int nthreads);
  * range. */
 int orc_bin_sq(double s, const double *T, int n);
 
+/* Pair counts in clean64 semantics: terms are pairs of segment labels
+ * (a <= b); (a,a) counts the unique pairs i<j of segment a, (a,b) every pair
+ * of the product.  FP64 squared distances against the squared thresholds of
+ * ax (n up to 4096), out of range dropped, uint64 counters; hist receives
+ * nterms histograms of ax->n cells.  Pinned: closed-form totals C(N,2) and
+ * N_a N_b, the union identity DD+DR+RR = pairs of the union, a numpy brute
+ * force, exact lattice distances (tests/test_oracle_pairs.py). */
+int orc_clean_pairs(const double *const seg[2], const int64_t nseg_pts[2], int nseg, const int term[][2],
+                    int nterms, const orc_axis *ax, uint64_t *hist, int nthreads);
+
 /* ------------------------------------------------------------------------ */
 /* Writers                                                                   */
 /* ------------------------------------------------------------------------ */
@@ -163,6 +184,12 @@ int orc_write_tri(const char *path, const int64_t ngal[3], const int nb[3],
 /* (r1,r2,r3) cube format, src/3pcf_C_version.cc:440-460.  Pinned: byte
  * equality with the oracle/_ref build of the reference. */
 int orc_write_plain(const char *path, int n, const uint64_t *hist);
+
+/* Pair-count format of src/2pcf_C_version.cc:383-397: the two catalog
+ * sizes, the bin count, one "%-6.3f" line (lo, hi, width) and one "%lu"
+ * line per bin.  Pinned: byte equality with the oracle/_ref build. */
+int orc_write_two(const char *path, const int64_t ngal[2], int nbins, double lo, double hi, double width,
+                  const uint64_t *hist);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,9 @@ Fixtures are data only:
                  DDD_voxel4 files, and the Debbie LRG DDD file;
   ref_outputs/   outputs of the reference programs compiled from source
                  (oracle/_ref) on the small slices: (s,q,theta) and JBJ for
-                 every file-order case, and the (r1,r2,r3) cube program;
+                 every file-order case, the (r1,r2,r3) cube program, and the
+                 pair-count program (2pcf_C_version) on the slices and on the
+                 full 1k catalogs;
   MANIFEST.json  sha256 of every fixture and the command that produced each
                  ref_outputs file.
 """
@@ -91,6 +93,14 @@ def main():
                         "ABB": ("a60.cat", "b50.cat", "b50.cat")}.items():
         runs.append((f"plain_{name}.dat", "plain", list(files), []))
         runs.append((f"plain16_{name}.dat", "plain16", list(files), []))
+    two = {"DD": ("d200.dat", "d200.dat"), "DR": ("d200.dat", "r180.dat"), "RR": ("r180.dat", "r180.dat"),
+           "DD1k": ("MICE_5degx5deg_1k_randomized.dat",) * 2,
+           "DR1k": ("MICE_5degx5deg_1k_randomized.dat", "flat_MICE_5degx5deg_1k.dat"),
+           "RR1k": ("flat_MICE_5degx5deg_1k.dat",) * 2}
+    for name, files in two.items():
+        runs.append((f"two_{name}.dat", "two", list(files), []))
+    runs.append(("two_DD_vox2_001000.dat", "two", list(two["DD"]), ["-X", "2", "-x", "001000000"]))
+    runs.append(("two_DD_vox2_000001.dat", "two", list(two["DD"]), ["-X", "2", "-x", "000001000"]))
     for out, prog, files, extra in runs:
         cmd = [os.path.join(REF_BIN, prog)] + files + extra + ["-o", os.path.join(out_dir, out)]
         subprocess.run(cmd, cwd=inp, check=True, stdout=subprocess.DEVNULL)
