@@ -11,7 +11,8 @@ present, the calls raise instead of falling back to anything else.
 Reference interface being replaced: the command-line programs
 ``src/3pcf_C_version_triangle_parametrization.cc`` (main at :245-571) and
 ``src/3pcf_C_version.cc`` (main at :147-466): three catalogs in, one histogram
-out.  :func:`count_files` mirrors that call; :class:`Context` exposes the
+out; and ``src/2pcf_C_version.cc`` (main at :124-411) for pair counts
+(:meth:`Context.count_pairs`).  :func:`count_files` mirrors that call; :class:`Context` exposes the
 library's catalogs, jobs, batches and the serving queue.
 """
 from __future__ import annotations
@@ -35,6 +36,11 @@ MODES = {"fast": MODE_FAST, "plain": MODE_PLAIN, "fast64": MODE_FAST64}
 # DDD, DDR, DRR, RRR over segment 0 = data, segment 1 = randoms.
 FOUR_TERMS = ((0, 0, 0), (0, 0, 1), (0, 1, 1), (1, 1, 1))
 TERM_NAMES = ("DDD", "DDR", "DRR", "RRR")
+# DD, DR, RR pair terms (the third label of a pair term is ignored).
+PAIR_TERMS = ((0, 0), (0, 1), (1, 1))
+PAIR_TERM_NAMES = ("DD", "DR", "RR")
+# Bins of src/2pcf_C_version.cc:14-16: 700 linear bins on [0, 70).
+TWO_PCF_AXIS = (700, 0.0, 70.0, "lin")
 
 
 class TpcfError(RuntimeError):
@@ -113,6 +119,11 @@ _SIGNATURES = {
     "tpcf_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(Stats)]),
     "tpcf_batch_info": (C.c_char_p, [C.c_void_p]),
     "tpcf_batch_destroy": (C.c_int, [C.c_void_p]),
+    "tpcf_count_pairs": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(Axis), C.c_int, C.c_void_p, C.POINTER(Stats)]),
+    "tpcf_count_pairs_batch": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(Job), C.POINTER(Axis), C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(Stats)],
+    ),
 }
 
 BATCH_AUTOTUNE, BATCH_GRAPH = 1, 2
@@ -220,7 +231,7 @@ def make_job(segments: Sequence[tuple], terms: Iterable[tuple]) -> Job:
     job.nterms = len(terms)
     for t, tr in enumerate(terms):
         for k in range(3):
-            job.term[t][k] = tr[k]
+            job.term[t][k] = tr[k] if k < len(tr) else 0
     return job
 
 
@@ -265,6 +276,23 @@ class Context:
         ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
         st = Stats()
         _check(self.lib.tpcf_count_batch(self.handle, JA, SA, n, mode, ptrs, C.byref(st)))
+        return outs, st
+
+    def count_pairs(self, job: Job, ax: Axis, mode: int = MODE_FAST):
+        """Pair-count histograms of every pair term of job: (nterms, ax.n) uint64, Stats."""
+        out = np.zeros((job.nterms, ax.n), dtype=np.uint64)
+        st = Stats()
+        _check(self.lib.tpcf_count_pairs(self.handle, C.byref(job), C.byref(ax), mode, out.ctypes.data, C.byref(st)))
+        return out, st
+
+    def count_pairs_batch(self, jobs: Sequence[Job], axes: Sequence[Axis], mode: int = MODE_FAST):
+        n = len(jobs)
+        JA = (Job * n)(*jobs)
+        AA = (Axis * n)(*axes)
+        outs = [np.zeros((j.nterms, a.n), dtype=np.uint64) for j, a in zip(jobs, axes)]
+        ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
+        st = Stats()
+        _check(self.lib.tpcf_count_pairs_batch(self.handle, JA, AA, n, mode, ptrs, C.byref(st)))
         return outs, st
 
     def batch(self, jobs: Sequence[Job], specs: Sequence[BinSpec], mode: int = MODE_FAST,
@@ -330,3 +358,8 @@ class Batch:
 def four_term_job(data: Catalog, randoms: Catalog) -> Job:
     """DDD, DDR, DRR, RRR of one data and one random catalog."""
     return make_job([(data, 0, data.n), (randoms, 0, randoms.n)], FOUR_TERMS)
+
+
+def pair_job(data: Catalog, randoms: Catalog) -> Job:
+    """DD, DR, RR pair terms of one data and one random catalog."""
+    return make_job([(data, 0, data.n), (randoms, 0, randoms.n)], PAIR_TERMS)

@@ -13,9 +13,9 @@
 
 namespace tpcf {
 
-int axis_edges(const tpcf_axis &ax, double *e) {
+int axis_edges(const tpcf_axis &ax, double *e, int max_bins) {
     const int n = ax.n;
-    if (n < 1 || n > kMaxBins) return set_error(TPCF_EINVAL, "axis needs 1..255 bins, got %d", n);
+    if (n < 1 || n > max_bins) return set_error(TPCF_EINVAL, "axis needs 1..%d bins, got %d", max_bins, n);
     if (!(ax.hi > ax.lo) || !std::isfinite(ax.lo) || !std::isfinite(ax.hi))
         return set_error(TPCF_EINVAL, "axis range [%g, %g) is empty", ax.lo, ax.hi);
     if (ax.scale == TPCF_LOG) {

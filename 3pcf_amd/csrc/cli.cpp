@@ -1,11 +1,12 @@
 // cli.cpp -- drop-in command-line programs over libtpcf.
 //
-// One source, three programs (selected at build time with TPCF_PROGRAM):
+// One source, four programs (selected at build time with TPCF_PROGRAM):
 //   0  3pcf_C_version_triangle_parametrization      (s,q,theta), TRI format
 //        argv of src/3pcf_C_version_triangle_parametrization.cc:269-313
 //   1  3pcf_C_version_triangle_parametrization_JBJ  JBJ variant, TRI format
 //   2  3pcf_C_version                               sorted (r1,r2,r3), cube format
 //        argv of src/3pcf_C_version.cc:181-217
+//   3  2pcf_C_version                               pair counts (see two_main)
 // Usage is the reference's: three catalog paths plus getopt flags, one
 // histogram file out (default default_out.dat).  Long options add runtime
 // bin specs and the device path:
@@ -107,10 +108,157 @@ bool write_plain(const char *path, int n, const uint64_t *h) {
     return std::fclose(f) == 0;
 }
 
+// Pair-count file (src/2pcf_C_version.cc:391-404): both catalog sizes, the
+// bin count, one range line, then one count per line.
+bool write_two(const char *path, const int64_t ng[2], const tpcf_axis &ax, const uint64_t *h) {
+    FILE *f = std::fopen(path, "w");
+    if (!f) return false;
+    const double w = ax.scale == TPCF_LOG ? std::log(ax.hi / ax.lo) / ax.n : (ax.hi - ax.lo) / ax.n;
+    std::fprintf(f, "%lld\n%lld\n%d\n", (long long)ng[0], (long long)ng[1], ax.n);
+    std::fprintf(f, "%-6.3f %-6.3f %-6.3f\n", ax.lo, ax.hi, w);
+    for (int b = 0; b < ax.n; ++b) std::fprintf(f, "%llu\n", (unsigned long long)h[b]);
+    return std::fclose(f) == 0;
+}
+
+// 2pcf_C_version f0 f1 [-o out] [-l flag] [-X div -x AAABBBCCC] [-a]: the
+// argv of src/2pcf_C_version.cc:143-210 (-l is parsed and, as there, has no
+// effect).  Two 7-column catalogs; voxel chunk AAA of f0 against chunk BBB of
+// f1.  Default axis: 700 linear bins on [0, 70) (src/2pcf_C_version.cc:14-16);
+// --bins N --lo X --hi X [--log] change it.  --mode fast|plain|fast64,
+// --device N, --quiet, --stats as for the triplet programs.  The catalogs are
+// compared as files (realpath): one file gives the C(N,2) unique pairs (i < j,
+// no self pairs); a chunk pair (p, q) of one file counts chunk p against
+// chunk q for p < q, the unique pairs of chunk p for p == q, and nothing for
+// p > q, so the chunk pairs of a voxel run add up to the full run.
+int two_main(int argc, char **argv) {
+    const char *outfile = "default_out_2pcf.dat";
+    int vdiv = -999, vtemp = -999, device = 0, quiet = 0, want_stats = 0, mode = TPCF_MODE_FAST;
+    tpcf_axis ax{700, 0.0, 70.0, TPCF_LIN};
+    enum { O_BINS = 1000, O_LO, O_HI, O_LOG, O_MODE, O_DEVICE, O_QUIET, O_STATS };
+    static option longopts[] = {{"bins", required_argument, nullptr, O_BINS},
+                                {"lo", required_argument, nullptr, O_LO},
+                                {"hi", required_argument, nullptr, O_HI},
+                                {"log", no_argument, nullptr, O_LOG},
+                                {"mode", required_argument, nullptr, O_MODE},
+                                {"device", required_argument, nullptr, O_DEVICE},
+                                {"quiet", no_argument, nullptr, O_QUIET},
+                                {"stats", no_argument, nullptr, O_STATS},
+                                {nullptr, 0, nullptr, 0}};
+    int c;
+    while ((c = getopt_long(argc, argv, "ao:l:x:X:", longopts, nullptr)) != -1) {
+        switch (c) {
+        case 'o': outfile = optarg; break;
+        case 'X': vdiv = std::atoi(optarg); break;
+        case 'x': vtemp = std::atoi(optarg); break;
+        case 'a': case 'l': break;
+        case O_BINS: ax.n = std::atoi(optarg); break;
+        case O_LO: ax.lo = std::atof(optarg); break;
+        case O_HI: ax.hi = std::atof(optarg); break;
+        case O_LOG: ax.scale = TPCF_LOG; break;
+        case O_MODE:
+            if (!std::strcmp(optarg, "fast")) mode = TPCF_MODE_FAST;
+            else if (!std::strcmp(optarg, "plain")) mode = TPCF_MODE_PLAIN;
+            else if (!std::strcmp(optarg, "fast64")) mode = TPCF_MODE_FAST64;
+            else { std::fprintf(stderr, "error: --mode fast|plain|fast64\n"); return 2; }
+            break;
+        case O_DEVICE: device = std::atoi(optarg); break;
+        case O_QUIET: quiet = 1; break;
+        case O_STATS: want_stats = 1; break;
+        case '?': std::printf("unknown arg %c\n", optopt); break;  // the reference continues
+        default: break;
+        }
+    }
+    if (argc - optind < 2) {
+        std::printf("\nMust pass in at least two input files on command line!\n");
+        return 1;
+    }
+    std::vector<double> e(ax.n > 0 && ax.n <= 4096 ? ax.n + 1 : 1);
+    if (tpcf_axis_edges(&ax, e.data()) != TPCF_OK) return die(2, "bad axis");
+    Catalog cat[2];
+    for (int i = 0; i < 2; ++i) {
+        cat[i].path = argv[optind + i];
+        char rp[PATH_MAX];
+        if (!realpath(cat[i].path.c_str(), rp)) {
+            std::fprintf(stderr, "error: cannot open %s\n", cat[i].path.c_str());
+            return 1;
+        }
+        cat[i].real = rp;
+    }
+    const bool same_file = cat[0].real == cat[1].real;
+    for (int i = 0; i < 2; ++i) {
+        if (i == 1 && same_file) {
+            cat[1].xyz = cat[0].xyz;
+            cat[1].n = cat[0].n;
+            break;
+        }
+        cat[i].n = tpcf_read_catalog(cat[i].path.c_str(), 7, 1.0, &cat[i].xyz);
+        if (cat[i].n < 0) return die(1, "cannot read catalog");
+        if (!quiet) std::printf("Opening input file %d: %s (%lld points)\n", i, cat[i].path.c_str(), (long long)cat[i].n);
+    }
+    int64_t lo[2], hi[2];
+    int vidx[2] = {0, 0};
+    if (vdiv > 0) {
+        vidx[0] = vtemp / 1000000;
+        vidx[1] = (vtemp / 1000) % 1000;
+    }
+    for (int i = 0; i < 2; ++i) {
+        lo[i] = 0;
+        hi[i] = cat[i].n;
+        if (vdiv > 0 && vtemp >= 0 && !vox_chunk(vdiv, vidx[i], cat[i].n, lo[i], hi[i])) {
+            std::printf("Voxel chunks are bigger than the number of galaxies!");
+            return 255;
+        }
+    }
+    const bool one_seg = same_file && lo[0] == lo[1];
+    const bool empty = same_file && lo[0] > lo[1];
+
+    tpcf_ctx *ctx = nullptr;
+    if (tpcf_ctx_create(device, &ctx) != TPCF_OK) return die(3, "no usable GPU");
+    tpcf_catalog *dev[2] = {nullptr, nullptr};
+    for (int i = 0; i < 2; ++i) {
+        if (i == 1 && same_file) {
+            dev[1] = dev[0];
+            break;
+        }
+        if (tpcf_catalog_create(ctx, cat[i].xyz, cat[i].n, &dev[i]) != TPCF_OK) return die(3, "catalog upload failed");
+    }
+    tpcf_job job{};
+    job.nseg = one_seg ? 1 : 2;
+    for (int s = 0; s < job.nseg; ++s) {
+        job.seg[s] = dev[s];
+        job.seg_lo[s] = lo[s];
+        job.seg_hi[s] = hi[s];
+    }
+    job.nterms = 1;
+    job.term[0][0] = 0;
+    job.term[0][1] = one_seg ? 0 : 1;
+    std::vector<uint64_t> h(ax.n, 0);
+    tpcf_stats st{};
+    if (!empty && tpcf_count_pairs(ctx, &job, &ax, mode, h.data(), &st) != TPCF_OK) return die(4, "counting failed");
+    uint64_t total = 0;
+    for (auto v : h) total += v;
+    const int64_t ng[2] = {cat[0].n, cat[1].n};
+    if (!write_two(outfile, ng, ax, h.data())) return die(1, "cannot write the histogram file");
+    if (!quiet) {
+        std::printf("Output filename is %s\n", outfile);
+        std::printf("Total: %llu\n", (unsigned long long)total);
+    }
+    if (want_stats)
+        std::printf("{\"pairs\": %llu, \"device_ms\": %.3f, \"wall_ms\": %.3f}\n", (unsigned long long)st.pairs,
+                    st.device_ms, st.wall_ms);
+    tpcf_catalog_destroy(dev[0]);
+    if (!same_file) tpcf_catalog_destroy(dev[1]);
+    tpcf_free(cat[0].xyz);
+    if (!same_file) tpcf_free(cat[1].xyz);
+    tpcf_ctx_destroy(ctx);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
     const int program = TPCF_PROGRAM;
+    if (program == 3) return two_main(argc, argv);
     const bool triprog = program != 2;
     const char *outfile = "default_out.dat";
     int vdiv = -999, vtemp = -999, device = 0, quiet = 0, want_stats = 0;

@@ -37,14 +37,6 @@ int set_error(int code, const char *fmt, ...) {
 
 void clear_error() { g_err.clear(); }
 
-#define HIP_TRY(expr)                                                                                   \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess)                                                                           \
-            return set_error(e_ == hipErrorOutOfMemory ? TPCF_ENOMEM : TPCF_EDEVICE, "%s: %s (%s:%d)", \
-                             #expr, hipGetErrorString(e_), __FILE__, __LINE__);                         \
-    } while (0)
-
 DevBuf::~DevBuf() {
     if (p) (void)hipFree(p);
 }
@@ -810,7 +802,8 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     // Kernels with an LDS histogram raise their dynamic LDS limit once, to
     // what their static LDS leaves of the 160 KiB.
     std::vector<const void *> lds_kernels{sqt_plain_kernel_ptr(kSqt), sqt_plain_kernel_ptr(kJbj),
-                                          sqt_fast_kernel_ptr(8), sqt_fast_kernel_ptr(16)};
+                                          sqt_fast_kernel_ptr(8), sqt_fast_kernel_ptr(16),
+                                          pair_hist_kernel_ptr(8), pair_hist_kernel_ptr(16)};
     for (int w : {2, 4, 8, 16})
         for (int k : {1, 2, 4, 8})
             if (const void *fn = r123_triplet_kernel_ptr(w, k)) lds_kernels.push_back(fn);
@@ -858,6 +851,10 @@ int tpcf_ctx_set_memory_budget(tpcf_ctx *ctx, uint64_t bytes) {
 int tpcf_catalog_create(tpcf_ctx *ctx, const double *xyz, int64_t n, tpcf_catalog **out) {
     clear_error();
     if (!ctx || !out || n < 0 || (n > 0 && !xyz)) return set_error(TPCF_EINVAL, "bad catalog arguments");
+    for (int64_t i = 0; i < 3 * n; ++i)
+        if (!(std::fabs(xyz[i]) <= kMaxCoordinate))
+            return set_error(TPCF_EINVAL, "coordinate %lld of the catalog is not finite or exceeds %g in magnitude",
+                             (long long)i, kMaxCoordinate);
     HIP_TRY(hipSetDevice(ctx->device));
     auto *c = new tpcf_catalog();
     c->ctx = ctx;
@@ -895,7 +892,7 @@ int64_t tpcf_hist_cells(const tpcf_binspec *spec) {
 int tpcf_axis_edges(const tpcf_axis *ax, double *edges) {
     clear_error();
     if (!ax || !edges) return set_error(TPCF_EINVAL, "NULL argument");
-    return axis_edges(*ax, edges);
+    return axis_edges(*ax, edges, kMaxPairBins);
 }
 
 int tpcf_binspec_check(const tpcf_binspec *spec) {

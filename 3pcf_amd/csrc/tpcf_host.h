@@ -16,12 +16,25 @@
 namespace tpcf {
 
 constexpr int kMaxSqtCells = 36 * 1024;  // LDS histogram limit of the (s,q,theta) kernels
+// Catalog coordinates are finite and at most this large in magnitude, so
+// every squared distance is finite in FP64.
+constexpr double kMaxCoordinate = 1e100;
 
 // Record a message for tpcf_last_error() and return code.
 int set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
 
-int axis_edges(const tpcf_axis &ax, double *e);
+// Return the HIP error of expr as TPCF_ENOMEM / TPCF_EDEVICE with a message.
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return set_error(e_ == hipErrorOutOfMemory ? TPCF_ENOMEM : TPCF_EDEVICE, "%s: %s (%s:%d)", \
+                             #expr, hipGetErrorString(e_), __FILE__, __LINE__);                         \
+    } while (0)
+
+// Edges e[0..n] of an axis of 1..max_bins bins (triplet axes: kMaxBins).
+int axis_edges(const tpcf_axis &ax, double *e, int max_bins = kMaxBins);
 void sq_thresholds(const double *e, int n, double *T);
 int binspec_check(const tpcf_binspec &s);
 int64_t hist_cells(const tpcf_binspec &s);
@@ -42,6 +55,13 @@ struct Layout {
     int64_t seg_blk0[3] = {0, 0, 0};
     int64_t seg_nblk[3] = {0, 0, 0};
     int64_t npad = 0, nblk = 0;
+};
+
+// Device buffer and events of the pair-count calls (pairs_api.cpp).
+struct PairWorkspace {
+    DevBuf buf;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~PairWorkspace();
 };
 
 // Tuned launch shape of the triplet kernels.
@@ -88,6 +108,7 @@ struct tpcf_ctx {
     std::mutex call_mu;                    // serializes synchronous calls
     std::unique_ptr<tpcf::Workspace, tpcf::WorkspaceDeleter> ws;  // workspace of synchronous calls
     tpcf::LaunchShape shape;
+    tpcf::PairWorkspace pairs;             // workspace of tpcf_count_pairs*
     // serving
     uint64_t budget = 0, in_use = 0;
     std::mutex mu;

@@ -140,6 +140,40 @@ hipError_t launch_sqt_fast(const SqtFastArgs &P, int waves, int grid, hipStream_
 size_t sqt_fast_lds_bytes(int cells, int waves, int qcap);
 const void *sqt_fast_kernel_ptr(int waves);  // 8 or 16 waves per workgroup
 
+// Pair counts (kernels_pairs.hip).  Bins per axis up to kMaxPairBins.
+constexpr int kMaxPairBins = 4096;
+constexpr int kPairIpl = 2;  // i points per lane in the FAST pair kernel
+// One (job, term) of a pair-count launch: segment a = (ax, ay, az)[0..na),
+// segment b = (bx, by, bz)[0..nb).
+struct PairHistTerm {
+    const double *ax, *ay, *az, *bx, *by, *bz;
+    const double *D;           // n+3 doubles: -inf, squared thresholds T[0..n], +inf
+    unsigned long long *hist;  // n uint64
+    int na, nb, n;
+    int same;                  // a and b are one segment: count i < j
+    int log;                   // bin guess from log2 (log axis) or sqrt (linear)
+    float h_mul, h_add;        // guess k = floor(g * h_mul + h_add)
+};
+// Work item: i in [i0, i0 + block) of segment a against j in [j0, j1) of b.
+struct PairHistItem {
+    int term, i0, j0, j1;
+};
+struct PairHistArgs {
+    const PairHistTerm *terms;
+    const PairHistItem *items;
+    int nitems;
+    int *counter;
+    int max_n;       // largest n of the launch's terms (LDS table size)
+    int copies;      // privatized LDS histogram copies
+    int stride;      // words between copies
+    int hist_words;  // copies * stride, rounded up to an even count
+};
+hipError_t launch_pair_hist(const PairHistArgs &P, int waves, int grid, hipStream_t st);
+hipError_t launch_pair_plain(const PairHistArgs &P, hipStream_t st);
+size_t pair_hist_lds_bytes(int hist_words, int max_n);
+const void *pair_hist_kernel_ptr(int waves);  // 8 or 16 waves per workgroup
+const void *pair_plain_kernel_ptr();
+
 // Launchers (kernels_r123.hip).
 hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st);
 hipError_t launch_r123_triplets(const TripletItem *items, int nitems, int *counter, const TripletTerm *terms,

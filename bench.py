@@ -365,6 +365,92 @@ def bench_sqt(args) -> int:
     return 0
 
 
+PAIR_OPS = 8.0  # FP64 lane-ops of one exact squared distance: 3 sub, 3 mul, 2 add
+
+
+def bench_pairs(args) -> int:
+    """Pair counts (DD, DR, RR) on the reference's binning (700 linear bins
+    on [0, 70) Mpc/h): clustered data + uniform randoms in a 400 Mpc/h cube,
+    FAST (FP32 bin guess, FP64 decision) against the plain FP64 kernel, and
+    the oracle on the host cores over a sample, all exact."""
+    n = args.pairs_n
+    d = synth.neyman_scott(n, nclust=2000, sigma=3.0, size=400.0, seed=0)
+    r = synth.uniform_cube(n, 400.0, seed=1)
+    ax = tp.axis(*tp.TWO_PCF_AXIS)
+    reps = max(1, args.steps)
+    with tp.Context(0) as ctx:
+        cd, cr = ctx.catalog(d), ctx.catalog(r)
+        job = tp.pair_job(cd, cr)
+        for _ in range(args.warmup):
+            ctx.count_pairs(job, ax)
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out, st = ctx.count_pairs(job, ax)
+            stats.append(st)
+        dt = time.perf_counter() - t0
+        plain = dtp = None
+        if not args.no_plain:
+            t1 = time.perf_counter()
+            plain, stp = ctx.count_pairs(job, ax, tp.MODE_PLAIN)
+            dtp = time.perf_counter() - t1
+    pairs = stats[0].pairs
+    kern_s = sum(x.pair_ms for x in stats) / len(stats) / 1e3
+    achieved = PAIR_OPS * pairs / kern_s
+    line = {
+        "metric": "2PCF DD/DR/RR pairs/s, FAST vs plain FP64, exact",
+        "value": pairs * reps / dt,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / reps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 distances, f32 bin guess, u32/u64 counts",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{n // 1000}k Neyman-Scott data (2000 clusters, sigma 3 Mpc/h) + {n // 1000}k uniform "
+                        "randoms in a 400 Mpc/h cube, DD/DR/RR, 700 linear bins on [0,70) Mpc/h",
+            "pairs_per_step": pairs,
+            "pairs_in_range": int(out.sum()),
+            "pair_kernel_ms": stats[0].pair_ms,
+            "plain_fp64_s": dtp,
+            "plain_kernel_ms": stp.pair_ms if dtp else None,
+            "speedup_vs_plain_kernel": stp.pair_ms / stats[0].pair_ms if dtp else None,
+            "fast_equals_plain": bool(np.array_equal(out, plain)) if plain is not None else None,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved / 1e12,
+            "peak": VALU_PEAK / 1e12,
+            "unit": "TOP/s (FP64 VALU lane-ops, 8 per pair for the squared distance; binning and the LDS "
+                    "atomic are on top)",
+            "frac": achieved / VALU_PEAK,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+
+        threads = max(1, min(16, os.cpu_count() or 1))
+        m = min(n, 40000)
+        t1 = time.perf_counter()
+        orc.clean_pairs([d[:m], r[:m]], tp.PAIR_TERMS, orc.axis(*tp.TWO_PCF_AXIS), nthreads=threads)
+        dc = time.perf_counter() - t1
+        cpairs = m * (m - 1) + m * m
+        line["cpu_baseline"] = {
+            "value": cpairs / dc,
+            "unit": "pairs/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {m} data + {m} randoms, DD/DR/RR ({cpairs} pairs), clean64 oracle, {dc:.1f} s",
+        }
+    print(json.dumps(line))
+    return 0
+
+
 def bench_precision(args) -> int:
     """Precision form: 20k clustered data + 20k uniform randoms in a
     10x10 deg wedge (r 500-800 Mpc/h), 20 log bins on [1,50) Mpc/h, recheck
@@ -438,7 +524,8 @@ def main(argv=None) -> int:
     ap.add_argument("--nr", type=int, default=50000)
     ap.add_argument("--mode", choices=sorted(tp.MODES), default="fast")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["headline", "batch", "serve", "sqt", "precision"], default="headline",
+    ap.add_argument("--workload", choices=["headline", "batch", "serve", "sqt", "precision", "pairs"],
+                    default="headline",
                     help="headline: 50k+50k four terms (default); batch: 64 angular voxels of 150k+150k in one "
                          "prepared, graph-replayed persistent launch vs one FP64 plain job at a time; serve: a "
                          "Poisson request stream through the serving queue vs plain one-at-a-time; sqt: 10k+10k "
@@ -448,6 +535,7 @@ def main(argv=None) -> int:
                     help="sqt: skip the FAST64 and FP64 plain comparison runs (for profiler runs)")
     ap.add_argument("--sqt-n", type=int, default=10000, help="sqt: data (and randoms) catalog size")
     ap.add_argument("--precision-n", type=int, default=20000, help="precision: data (and randoms) catalog size")
+    ap.add_argument("--pairs-n", type=int, default=500000, help="pairs: data (and randoms) catalog size")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--n-max", type=int, default=8000, help="serve: largest catalog size of a request")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -475,6 +563,8 @@ def main(argv=None) -> int:
         return bench_serve(args)
     if args.workload == "sqt":
         return bench_sqt(args)
+    if args.workload == "pairs":
+        return bench_pairs(args)
     if args.workload == "precision":
         return bench_precision(args)
 

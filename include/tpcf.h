@@ -83,7 +83,7 @@ typedef struct tpcf_ctx tpcf_ctx;
 typedef struct tpcf_catalog tpcf_catalog;
 
 typedef struct {
-    int n;        /* bins, 1..255 (R123 kernels: 1..32) */
+    int n;        /* bins, 1..255 (pair counts: 1..4096) */
     double lo, hi;
     int scale;    /* TPCF_LIN / TPCF_LOG */
 } tpcf_axis;
@@ -133,8 +133,10 @@ int tpcf_ctx_set_memory_budget(tpcf_ctx *ctx, uint64_t bytes);
 
 /* ---- catalogs ------------------------------------------------------------ */
 
-/* Upload n points (FP64 SoA) to the device.  Replaces the float host arrays
- * filled at src/3pcf_C_version_triangle_parametrization.cc:366-439. */
+/* Upload n points (FP64 SoA) to the device; every coordinate must be finite
+ * and at most 1e100 in magnitude (TPCF_EINVAL otherwise).  Replaces the
+ * float host arrays filled at
+ * src/3pcf_C_version_triangle_parametrization.cc:366-439. */
 int tpcf_catalog_create(tpcf_ctx *ctx, const double *xyz_soa, int64_t n, tpcf_catalog **out);
 int tpcf_catalog_destroy(tpcf_catalog *cat);
 int64_t tpcf_catalog_size(const tpcf_catalog *cat);
@@ -152,7 +154,8 @@ void tpcf_free(void *p);
 
 /* Cells of one term histogram: n^3 (R123) or n0*n1*n2. */
 int64_t tpcf_hist_cells(const tpcf_binspec *spec);
-/* Edge table e[0..n] of an axis (the host table every kernel uses). */
+/* Edge table e[0..n] of an axis of 1..4096 bins (the host table every
+ * kernel uses). */
 int tpcf_axis_edges(const tpcf_axis *ax, double *edges);
 /* Validate a spec (returns TPCF_OK or TPCF_EINVAL with a message). */
 int tpcf_binspec_check(const tpcf_binspec *spec);
@@ -205,6 +208,31 @@ int tpcf_wait(tpcf_ctx *ctx, int64_t ticket, tpcf_stats *stats);
 
 /* Device bytes the job's workspace needs (admission footprint). */
 int64_t tpcf_job_footprint(const tpcf_job *job, const tpcf_binspec *spec, int mode);
+
+/* ---- pair counts (two-point function) ---------------------------------- */
+
+/* Pair-count histograms of one job over a distance axis of 1..4096 bins
+ * (lin or log, lo >= 0, hi <= 1e100).  A pair term uses term[t][0] <=
+ * term[t][1] (term[t][2] is ignored): (a,a) counts the C(n_a,2) pairs i<j of
+ * segment a (no self pairs), (a,b) the n_a*n_b pairs of the product, so DD,
+ * DR, RR of data segment 0 and random segment 1 are (0,0), (0,1), (1,1).  A
+ * pair is in bin m iff e[m] <= d < e[m+1] with d = sqrt((dx*dx + dy*dy) +
+ * dz*dz) in FP64, the edges as for the triplet axes; others are dropped.
+ * hist: nterms * ax->n uint64, term-major.  Modes: TPCF_MODE_FAST (and
+ * TPCF_MODE_FAST64, the same kernel: its FP32 arithmetic only guesses which
+ * FP64 comparison to make) and TPCF_MODE_PLAIN (binary search per pair).
+ * stats->pairs counts the pairs classified.  Replaces the pair loop,
+ * distance() and distance_to_bin() of src/2pcf_C_version.cc:358-387, :89-113
+ * and :57-86 (700 FP32 bins on [0, 70), self pairs counted in bin 0). */
+int tpcf_count_pairs(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_axis *ax, int mode, uint64_t *hist,
+                     tpcf_stats *stats);
+
+/* Many pair-count jobs (for instance every voxel pair of a survey) in one
+ * persistent launch: job j over axis axes[j] into hist[j].  Replaces one
+ * src/2pcf_C_version.cc process per voxel pair and term
+ * (examples/run_CPU_calculation_2pcf.csh:36-53). */
+int tpcf_count_pairs_batch(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_axis *axes, int njobs, int mode,
+                           uint64_t *const *hist, tpcf_stats *stats);
 
 #ifdef __cplusplus
 }
