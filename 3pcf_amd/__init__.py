@@ -124,6 +124,17 @@ _SIGNATURES = {
         C.c_int,
         [C.c_void_p, C.POINTER(Job), C.POINTER(Axis), C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(Stats)],
     ),
+    "tpcf_triplet_norm": (C.c_int, [C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_double)]),
+    "tpcf_pair_norm": (C.c_int, [C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_double)]),
+    "tpcf_zeta": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double),
+                            C.c_void_p]),
+    "tpcf_xi": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.c_void_p]),
+    "tpcf_arange": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int64]),
+    "tpcf_reduced_q": (
+        C.c_int,
+        [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int,
+         C.c_void_p, C.c_void_p, C.c_void_p],
+    ),
 }
 
 BATCH_AUTOTUNE, BATCH_GRAPH = 1, 2
@@ -353,6 +364,57 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---- estimators (host arithmetic in libtpcf; no device needed) -------------
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def triplet_norm(ngal: Sequence[int], term: int) -> float:
+    """Triplets term 0..3 (DDD, DDR, DRR, RRR) is normalized by, from the
+    three catalog sizes of its histogram file."""
+    out = C.c_double()
+    _check(load_library().tpcf_triplet_norm((C.c_int64 * 3)(*ngal), term, C.byref(out)))
+    return out.value
+
+
+def pair_norm(ngal: Sequence[int], term: int) -> float:
+    """Pairs term 0..2 (DD, DR, RR) is normalized by, from the two catalog
+    sizes of its pair-count file."""
+    out = C.c_double()
+    _check(load_library().tpcf_pair_norm((C.c_int64 * 2)(*ngal), term, C.byref(out)))
+    return out.value
+
+
+def zeta(ddd, ddr, drr, rrr, norms: Sequence[float]) -> np.ndarray:
+    """(DDD - 3 DDR + 3 DRR - RRR) / RRR of normalized histograms, any shape."""
+    arrs = [_f64(a) for a in (ddd, ddr, drr, rrr)]
+    out = np.zeros(arrs[0].shape, dtype=np.float64)
+    _check(load_library().tpcf_zeta(*[a.ctypes.data for a in arrs], arrs[0].size, (C.c_double * 4)(*norms),
+                                    out.ctypes.data))
+    return out
+
+
+def xi(dd, dr, rr, norms: Sequence[float]) -> np.ndarray:
+    """(DD - 2 DR + RR) / RR of normalized pair counts."""
+    arrs = [_f64(a) for a in (dd, dr, rr)]
+    out = np.zeros(arrs[0].shape, dtype=np.float64)
+    _check(load_library().tpcf_xi(*[a.ctypes.data for a in arrs], arrs[0].size, (C.c_double * 3)(*norms),
+                                  out.ctypes.data))
+    return out
+
+
+def reduced_q(zeta_theta, s: float, q: float, theta_lo: float, theta_width: float, xs, ys):
+    """(theta/pi, denominator, Q) along theta for one (s, q) from a 2PCF table."""
+    z, x, y = _f64(zeta_theta), _f64(xs), _f64(ys)
+    n = z.size
+    th, den, qr = np.zeros(n), np.zeros(n), np.zeros(n)
+    _check(load_library().tpcf_reduced_q(z.ctypes.data, n, s, q, theta_lo, theta_width, x.ctypes.data,
+                                         y.ctypes.data, x.size, th.ctypes.data, den.ctypes.data, qr.ctypes.data))
+    return th, den, qr
 
 
 def four_term_job(data: Catalog, randoms: Catalog) -> Job:

@@ -234,6 +234,35 @@ int tpcf_count_pairs(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_axis *ax, in
 int tpcf_count_pairs_batch(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_axis *axes, int njobs, int mode,
                            uint64_t *const *hist, tpcf_stats *stats);
 
+/* ---- estimators (host arithmetic, FP64) ---------------------------------- */
+
+/* Number of triplets a term is normalized by, from the three catalog sizes
+ * written in its histogram file: term 0 DDD and 3 RRR: N0 (N1-1) (N2-2) / 6,
+ * 1 DDR: N0 (N1-1) N2 / 2, 2 DRR: N0 N1 (N2-1) / 2, the integer product
+ * rounded once to FP64 (examples/plot_MICE_3pcf.py:96-107). */
+int tpcf_triplet_norm(const int64_t ngal[3], int term, double *norm);
+/* Pairs a term is normalized by: term 0 DD and 2 RR: (N0^2 - N0) / 2, 1 DR:
+ * N0 N1 (examples/plot_MICE_2pcf.py:43-50). */
+int tpcf_pair_norm(const int64_t ngal[2], int term, double *norm);
+/* zeta = (DDD/n0 - 3 DDR/n1 + 3 DRR/n2 - RRR/n3) / (RRR/n3) per cell, in
+ * that order of operations; NaN and +inf give 0
+ * (examples/plot_MICE_3pcf.py:143-156). */
+int tpcf_zeta(const double *ddd, const double *ddr, const double *drr, const double *rrr, int64_t cells,
+              const double norm[4], double *zeta);
+/* xi = (DD/n0 - 2 DR/n1 + RR/n2) / (RR/n2) per bin; NaN and +inf give 0
+ * (examples/plot_MICE_2pcf.py:63-69). */
+int tpcf_xi(const double *dd, const double *dr, const double *rr, int64_t n, const double norm[3], double *xi);
+/* numpy's arange(lo, hi, step) (ceil((hi-lo)/step) values lo, lo+step,
+ * lo + i*((lo+step)-lo)) into out[cap]; returns the count or an error. */
+int tpcf_arange(double lo, double hi, double step, double *out, int64_t cap);
+/* Reduced amplitude along theta for one (s, q): theta[t] = the t-th theta/pi
+ * low edge, sides s, s q, s sqrt(1 + q^2 - 2 q cos(pi (theta + width/2))),
+ * xi of each side from the table (xs, ys) at the nearest abscissa,
+ * denom = xi12 xi23 + xi23 xi31 + xi31 xi12, qred = zeta / denom
+ * (examples/plot_MICE_3pcf.py:9-51, :196-221). */
+int tpcf_reduced_q(const double *zeta_theta, int ntheta, double s, double q, double theta_lo, double theta_width,
+                   const double *xs, const double *ys, int nxi, double *theta, double *denom, double *qred);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,9 @@ Fixtures are data only:
   sandbox/       histogram files copied from the reference's examples/sandbox/
                  (outputs of the reference's own CPU program, verified
                  byte-reproducible): all 8 DDD_voxel2 files, 8 of the 64
-                 DDD_voxel4 files, and the Debbie LRG DDD file;
+                 DDD_voxel4 files, the Debbie LRG DDD, DDR, DRR and RRR
+                 files, and examples/master_2pcf_reference.dat (the 2PCF
+                 table of the reduced-amplitude plot);
   ref_outputs/   outputs of the reference programs compiled from source
                  (oracle/_ref) on the small slices: (s,q,theta) and JBJ for
                  every file-order case, the (r1,r2,r3) cube program, and the
@@ -35,7 +37,9 @@ SANDBOX = [f"DDD_voxel2_{a}{b}{c}_evenbinning_CPU_5degx5deg_LRG_binning_1k.dat"
 SANDBOX += [f"DDD_voxel4_{v}_evenbinning_CPU_5degx5deg_LRG_binning_1k.dat"
             for v in ("000000000", "000001002", "001001001", "001002003", "002001000", "002002003",
                       "003000002", "003003003")]
-SANDBOX += ["DDD_Debbie_LRG_CPU_10degx10deg_LRG_binning_SLAC_1k.dat"]
+SANDBOX += [f"{t}_Debbie_LRG_CPU_10degx10deg_LRG_binning_SLAC_1k.dat" for t in ("DDD", "DDR", "DRR", "RRR")]
+# The 2PCF table the reduced-amplitude plot reads (examples/plot_MICE_3pcf.py:176).
+EXAMPLES = ["master_2pcf_reference.dat"]
 INPUTS = ["MICE_5degx5deg_1k_randomized.dat", "MICE_LRGs_10degx10deg.dat", "flat_MICE_5degx5deg_1k.dat",
           "input_1k_0.cat", "input_1k_1.cat"]
 
@@ -72,6 +76,10 @@ def main():
     for f in SANDBOX:
         dst = os.path.join(HERE, "sandbox", f)
         shutil.copyfile(os.path.join(REF, "examples", "sandbox", f), dst)
+        manifest["sandbox"][f] = sha(dst)
+    for f in EXAMPLES:
+        dst = os.path.join(HERE, "sandbox", f)
+        shutil.copyfile(os.path.join(REF, "examples", f), dst)
         manifest["sandbox"][f] = sha(dst)
     inp = os.path.join(HERE, "inputs")
     head(os.path.join(inp, "MICE_5degx5deg_1k_randomized.dat"), os.path.join(inp, "d200.dat"), 200)
