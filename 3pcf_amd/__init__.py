@@ -124,6 +124,8 @@ _SIGNATURES = {
         C.c_int,
         [C.c_void_p, C.POINTER(Job), C.POINTER(Axis), C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(Stats)],
     ),
+    "tpcf_radecz_to_xyz": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "tpcf_read_radecz": (C.c_int64, [C.c_char_p, C.c_void_p, C.POINTER(C.POINTER(C.c_double))]),
     "tpcf_triplet_norm": (C.c_int, [C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_double)]),
     "tpcf_pair_norm": (C.c_int, [C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_double)]),
     "tpcf_zeta": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double),
@@ -198,6 +200,32 @@ def read_catalog(path: str, fmt: int = 7, scale: float = 1.0) -> np.ndarray:
     lib = load_library()
     ptr = C.POINTER(C.c_double)()
     n = _check(lib.tpcf_read_catalog(path.encode(), fmt, scale, C.byref(ptr)))
+    try:
+        soa = np.ctypeslib.as_array(ptr, shape=(3 * n,)).copy() if n else np.zeros(0)
+    finally:
+        lib.tpcf_free(ptr)
+    return soa.reshape(3, n).T.copy() if n else np.zeros((0, 3))
+
+
+def radecz_to_xyz(ra, dec, zred, poly: Sequence[float] | None = None) -> np.ndarray:
+    """(n, 3) Cartesian Mpc/h from RA, Dec (degrees) and redshift (library
+    conversion; poly None = the MICE distance fit)."""
+    ra, dec, zred = (np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in (ra, dec, zred))
+    n = ra.size
+    soa = np.zeros(3 * n)
+    pp = (C.c_double * 3)(*poly) if poly is not None else None
+    _check(load_library().tpcf_radecz_to_xyz(ra.ctypes.data, dec.ctypes.data, zred.ctypes.data, n,
+                                             C.cast(pp, C.c_void_p) if pp is not None else None, soa.ctypes.data))
+    return soa.reshape(3, n).T.copy()
+
+
+def read_radecz(path: str, poly: Sequence[float] | None = None) -> np.ndarray:
+    """Read an "id ra dec z ..." catalog and convert it to (n, 3) Cartesian."""
+    lib = load_library()
+    ptr = C.POINTER(C.c_double)()
+    pp = (C.c_double * 3)(*poly) if poly is not None else None
+    n = _check(lib.tpcf_read_radecz(path.encode(), C.cast(pp, C.c_void_p) if pp is not None else None,
+                                    C.byref(ptr)))
     try:
         soa = np.ctypeslib.as_array(ptr, shape=(3 * n,)).copy() if n else np.zeros(0)
     finally:

@@ -150,6 +150,17 @@ int64_t tpcf_catalog_size(const tpcf_catalog *cat);
 int64_t tpcf_read_catalog(const char *path, int format, double scale, double **out);
 void tpcf_free(void *p);
 
+/* RA, Dec (degrees) and redshift to Cartesian Mpc/h (FP64 SoA out, x|y|z):
+ * distance D = p0 + p1 z + p2 z^2 with poly = {p0, p1, p2} (NULL: the MICE fit
+ * -1.07199554, 3013.13296, -638.827998), x = D sin(90-dec) sin(ra),
+ * y = D sin(90-dec) cos(ra), z = D cos(90-dec).  Replaces convert() of
+ * sample_data/gen_flat_MICE_data.py:49-99 (same order of operations). */
+int tpcf_radecz_to_xyz(const double *ra, const double *dec, const double *zred, int64_t n, const double poly[3],
+                       double *xyz_soa);
+/* Read "id ra dec z ..." rows (the MICE catalogs of sample_data/) and
+ * convert them as tpcf_radecz_to_xyz does; *out as for tpcf_read_catalog. */
+int64_t tpcf_read_radecz(const char *path, const double poly[3], double **out);
+
 /* ---- bin specs --------------------------------------------------------- */
 
 /* Cells of one term histogram: n^3 (R123) or n0*n1*n2. */

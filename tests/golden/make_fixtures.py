@@ -41,7 +41,7 @@ SANDBOX += [f"{t}_Debbie_LRG_CPU_10degx10deg_LRG_binning_SLAC_1k.dat" for t in (
 # The 2PCF table the reduced-amplitude plot reads (examples/plot_MICE_3pcf.py:176).
 EXAMPLES = ["master_2pcf_reference.dat"]
 INPUTS = ["MICE_5degx5deg_1k_randomized.dat", "MICE_LRGs_10degx10deg.dat", "flat_MICE_5degx5deg_1k.dat",
-          "input_1k_0.cat", "input_1k_1.cat"]
+          "flat_MICE_20degx20deg_1k.dat", "input_1k_0.cat", "input_1k_1.cat"]
 
 
 def sha(path):
