@@ -124,7 +124,7 @@ bool write_two(const char *path, const int64_t ng[2], const tpcf_axis &ax, const
 // argv of src/2pcf_C_version.cc:143-210 (-l is parsed and, as there, has no
 // effect).  Two 7-column catalogs; voxel chunk AAA of f0 against chunk BBB of
 // f1.  Default axis: 700 linear bins on [0, 70) (src/2pcf_C_version.cc:14-16);
-// --bins N --lo X --hi X [--log] change it.  --mode fast|plain|fast64,
+// --bins N --lo X --hi X [--log] change it.  --mode fast|plain|fast64|cells,
 // --device N, --quiet, --stats as for the triplet programs.  The catalogs are
 // compared as files (realpath): one file gives the C(N,2) unique pairs (i < j,
 // no self pairs); a chunk pair (p, q) of one file counts chunk p against
@@ -159,7 +159,8 @@ int two_main(int argc, char **argv) {
             if (!std::strcmp(optarg, "fast")) mode = TPCF_MODE_FAST;
             else if (!std::strcmp(optarg, "plain")) mode = TPCF_MODE_PLAIN;
             else if (!std::strcmp(optarg, "fast64")) mode = TPCF_MODE_FAST64;
-            else { std::fprintf(stderr, "error: --mode fast|plain|fast64\n"); return 2; }
+            else if (!std::strcmp(optarg, "cells")) mode = TPCF_MODE_CELLS;
+            else { std::fprintf(stderr, "error: --mode fast|plain|fast64|cells\n"); return 2; }
             break;
         case O_DEVICE: device = std::atoi(optarg); break;
         case O_QUIET: quiet = 1; break;

@@ -1,0 +1,120 @@
+// kernels_cells.hip -- spatial ordering of a catalog segment for the culled
+// pair counts: Morton keys of the coordinates, a device radix sort (rocPRIM),
+// a gather into a sorted SoA copy, and the bounding box of every chunk of
+// consecutive sorted points.
+//
+// Reference idea: examples/test_predivision_of_distances.py:11-227 divides
+// the volume into cells no smaller than the maximum separation and pairs each
+// cell only with its neighbours.  Here the points are put in Morton order, so
+// that runs of consecutive points are compact, and the host skips every pair
+// of runs whose boxes are farther apart than the last bin edge.  Counts do not
+// depend on the order of the points, so the histograms are those of the
+// unsorted segment.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "tpcf_internal.h"
+
+namespace tpcf {
+namespace {
+
+constexpr int kMortonBits = 21;
+
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0x1FFFFFu;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t quantize(double c, double lo, double inv) {
+    const double q = (c - lo) * inv;
+    const double top = (double)((1u << kMortonBits) - 1);
+    return (uint32_t)(q < 0.0 ? 0.0 : (q > top ? top : q));
+}
+
+__global__ void morton_keys_kernel(const double *__restrict__ x, const double *__restrict__ y,
+                                   const double *__restrict__ z, int n, double lox, double loy, double loz,
+                                   double inv, uint64_t *__restrict__ keys, uint32_t *__restrict__ idx) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        keys[i] = spread3(quantize(x[i], lox, inv)) | (spread3(quantize(y[i], loy, inv)) << 1) |
+                  (spread3(quantize(z[i], loz, inv)) << 2);
+        idx[i] = (uint32_t)i;
+    }
+}
+
+__global__ void gather_kernel(const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ z, const uint32_t *__restrict__ idx, int n,
+                              double *__restrict__ out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t s = idx[i];
+        out[i] = x[s];
+        out[n + i] = y[s];
+        out[2 * n + i] = z[s];
+    }
+}
+
+// Box of chunk c (points [c*chunk, min(n, (c+1)*chunk))): lo x, y, z then
+// hi x, y, z.
+__global__ void chunk_bounds_kernel(const double *__restrict__ p, int n, int chunk, int nchunks,
+                                    double *__restrict__ box) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
+        const int b = c * chunk, e = min(n, b + chunk);
+        double lo[3] = {p[b], p[n + b], p[2 * n + b]}, hi[3] = {lo[0], lo[1], lo[2]};
+        for (int i = b + 1; i < e; ++i)
+            for (int a = 0; a < 3; ++a) {
+                const double v = p[a * n + i];
+                lo[a] = fmin(lo[a], v);
+                hi[a] = fmax(hi[a], v);
+            }
+        for (int a = 0; a < 3; ++a) {
+            box[6 * c + a] = lo[a];
+            box[6 * c + 3 + a] = hi[a];
+        }
+    }
+}
+
+int grid_for(int n) { return (int)std::min<int64_t>(4096, ((int64_t)n + 255) / 256); }
+
+}  // namespace
+
+size_t sort_segment_scratch_bytes(int n) {
+    size_t temp = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, temp, (uint64_t *)nullptr, (uint64_t *)nullptr, (uint32_t *)nullptr,
+                                    (uint32_t *)nullptr, (size_t)n, 0, 3 * kMortonBits);
+    const auto a = [](size_t v) { return (v + 255) & ~size_t(255); };
+    return a(temp) + 2 * a((size_t)n * 8) + 2 * a((size_t)n * 4);
+}
+
+hipError_t sort_segment(const double *x, const double *y, const double *z, int n, const double lo[3], double span,
+                        void *scratch, size_t scratch_bytes, double *sorted, int chunk, double *box,
+                        hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const auto a = [](size_t v) { return (v + 255) & ~size_t(255); };
+    size_t temp = 0;
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, temp, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                             (uint32_t *)nullptr, (uint32_t *)nullptr, (size_t)n, 0, 3 * kMortonBits);
+    if (e != hipSuccess) return e;
+    char *s = static_cast<char *>(scratch);
+    void *tmp = s;
+    uint64_t *k0 = reinterpret_cast<uint64_t *>(s + a(temp));
+    uint64_t *k1 = reinterpret_cast<uint64_t *>(s + a(temp) + a((size_t)n * 8));
+    uint32_t *v0 = reinterpret_cast<uint32_t *>(s + a(temp) + 2 * a((size_t)n * 8));
+    uint32_t *v1 = reinterpret_cast<uint32_t *>(s + a(temp) + 2 * a((size_t)n * 8) + a((size_t)n * 4));
+    if (a(temp) + 2 * a((size_t)n * 8) + 2 * a((size_t)n * 4) > scratch_bytes) return hipErrorInvalidValue;
+    const double inv = span > 0 ? (double)(1u << kMortonBits) / span : 0.0;
+    hipLaunchKernelGGL(morton_keys_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, y, z, n, lo[0], lo[1], lo[2], inv,
+                       k0, v0);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = rocprim::radix_sort_pairs(tmp, temp, k0, k1, v0, v1, (size_t)n, 0, 3 * kMortonBits, st)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, y, z, v1, n, sorted);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int nchunks = (n + chunk - 1) / chunk;
+    hipLaunchKernelGGL(chunk_bounds_kernel, dim3(grid_for(nchunks)), dim3(256), 0, st, sorted, n, chunk, nchunks, box);
+    return hipGetLastError();
+}
+
+}  // namespace tpcf

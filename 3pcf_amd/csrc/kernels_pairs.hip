@@ -8,7 +8,7 @@
 // axis, for any runtime axis of up to kMaxPairBins bins; (a, a) terms count
 // i < j (no self pairs), (a, b) terms the full product.
 //
-// FAST (pair_hist_kernel): persistent workgroups pull items -- an i block of
+// FAST (pair_hist_kernel<.., CULL = false>): persistent workgroups pull items -- an i block of
 // WAVES*64*IPL points of segment a against a j range of segment b -- from an
 // atomic counter.  Each lane keeps IPL i points in registers; the j point is
 // the same for the whole wave and comes through scalar loads.  The FP32 sqrt
@@ -19,6 +19,11 @@
 // Counts go to privatized 32-bit LDS copies of the histogram (copy = lane %
 // copies), flushed with 64-bit atomics on a term change, before any copy
 // could wrap, and at exit.
+//
+// CELLS (pair_hist_kernel<.., CULL = true>): the same over Morton-ordered
+// copies of the segments; each wave takes the box of its own points and
+// runs only the 64-point j runs whose box gap reaches below the last squared
+// threshold (a lower bound of every pair's squared distance).
 //
 // PLAIN (pair_plain_kernel): one item per workgroup of 256 lanes, one i point
 // per lane, a binary search of the FP64 thresholds in global memory per pair,
@@ -105,7 +110,7 @@ constexpr double kFarPoint = 1e150;
 // One item: the lane's IPL points of segment a against j in [j0, j1) of
 // segment b.  DIAG: a and b are one segment and the item holds pairs with
 // j <= i, which are skipped.
-template <int WAVES, int IPL, bool LOG, bool DIAG>
+template <int WAVES, int IPL, bool LOG, bool DIAG, bool CULL>
 __device__ __forceinline__ void pair_item(CPairTerm *T, int i0, int j0, int j1, const double *D, uint32_t *hk) {
     constexpr int BI = WAVES * 64;
     const int na = T->na, n = T->n;
@@ -127,21 +132,64 @@ __device__ __forceinline__ void pair_item(CPairTerm *T, int i0, int j0, int j1, 
     }
     const double *bx = T->bx, *by = T->by, *bz = T->bz;
     const float mul = T->h_mul, add = T->h_add;
-    int j = j0;
-    // kJBatch j points per batch of scalar loads (merged into wide loads).
-    for (; j + kJBatch <= j1; j += kJBatch) {
-        double X[kJBatch], Y[kJBatch], Z[kJBatch];
+    // The pairs of the lane's points with j in [ja, jb).
+    auto run = [&](int ja, int jb) {
+        int j = ja;
+        // kJBatch j points per batch of scalar loads (merged into wide loads).
+        for (; j + kJBatch <= jb; j += kJBatch) {
+            double X[kJBatch], Y[kJBatch], Z[kJBatch];
 #pragma unroll
-        for (int u = 0; u < kJBatch; ++u) {
-            X[u] = ld_uniform(bx, j + u);
-            Y[u] = ld_uniform(by, j + u);
-            Z[u] = ld_uniform(bz, j + u);
+            for (int u = 0; u < kJBatch; ++u) {
+                X[u] = ld_uniform(bx, j + u);
+                Y[u] = ld_uniform(by, j + u);
+                Z[u] = ld_uniform(bz, j + u);
+            }
+            pair_batch<IPL, kJBatch, LOG, DIAG>(xi, yi, zi, jmin, j, X, Y, Z, n, mul, add, D, hk);
         }
-        pair_batch<IPL, kJBatch, LOG, DIAG>(xi, yi, zi, jmin, j, X, Y, Z, n, mul, add, D, hk);
+        for (; j < jb; ++j) {
+            const double X[1] = {ld_uniform(bx, j)}, Y[1] = {ld_uniform(by, j)}, Z[1] = {ld_uniform(bz, j)};
+            pair_batch<IPL, 1, LOG, DIAG>(xi, yi, zi, jmin, j, X, Y, Z, n, mul, add, D, hk);
+        }
+    };
+    if (!CULL) {
+        run(j0, j1);
+        return;
     }
-    for (; j < j1; ++j) {
-        const double X[1] = {ld_uniform(bx, j)}, Y[1] = {ld_uniform(by, j)}, Z[1] = {ld_uniform(bz, j)};
-        pair_batch<IPL, 1, LOG, DIAG>(xi, yi, zi, jmin, j, X, Y, Z, n, mul, add, D, hk);
+    const double *jbox = T->jbox;
+    // Culled form: the box of the wave's points (uniform after the
+    // reductions), then only the j boxes it can reach.
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+#pragma unroll
+    for (int q = 0; q < IPL; ++q)
+        if (jmin[q] != INT_MAX) {
+            lo[0] = fmin(lo[0], xi[q]);
+            lo[1] = fmin(lo[1], yi[q]);
+            lo[2] = fmin(lo[2], zi[q]);
+            hi[0] = fmax(hi[0], xi[q]);
+            hi[1] = fmax(hi[1], yi[q]);
+            hi[2] = fmax(hi[2], zi[q]);
+        }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fmin(lo[a], __shfl_xor(lo[a], m));
+            hi[a] = fmax(hi[a], __shfl_xor(hi[a], m));
+        }
+    if (!(lo[0] <= hi[0])) return;  // no point of segment a in this wave
+    const double Tn = D[n + 1];
+    for (int j = j0; j < j1;) {
+        const int c = j / kPairBoxPoints;
+        const int je = min(j1, (c + 1) * kPairBoxPoints);
+        const double *b = jbox + 6 * c;
+        double g[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            g[a] = fmax(0.0, fmax(ld_uniform(b, a) - hi[a], lo[a] - ld_uniform(b, 3 + a)));
+        // The kernel's squared distance of every pair is at least this
+        // (monotone FP64 rounding of the same operations).
+        if (__builtin_amdgcn_readfirstlane((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2] < Tn ? 1 : 0)) run(j, je);
+        j = je;
     }
 }
 
@@ -159,7 +207,7 @@ __device__ void pair_flush(uint32_t *H, int copies, int stride, int n, unsigned 
     __syncthreads();
 }
 
-template <int WAVES, int IPL>
+template <int WAVES, int IPL, bool CULL>
 __global__ __launch_bounds__(WAVES * 64) void pair_hist_kernel(PairHistArgs P) {
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ int s_item;
@@ -201,11 +249,11 @@ __global__ __launch_bounds__(WAVES * 64) void pair_hist_kernel(PairHistArgs P) {
         // Pairs with j <= i exist in the item only if j0 <= the last i.
         const bool diag = T->same && j0 < i0 + WAVES * 64 * IPL;
         if (T->log) {
-            if (diag) pair_item<WAVES, IPL, true, true>(T, i0, j0, j1, D, hk);
-            else pair_item<WAVES, IPL, true, false>(T, i0, j0, j1, D, hk);
+            if (diag) pair_item<WAVES, IPL, true, true, CULL>(T, i0, j0, j1, D, hk);
+            else pair_item<WAVES, IPL, true, false, CULL>(T, i0, j0, j1, D, hk);
         } else {
-            if (diag) pair_item<WAVES, IPL, false, true>(T, i0, j0, j1, D, hk);
-            else pair_item<WAVES, IPL, false, false>(T, i0, j0, j1, D, hk);
+            if (diag) pair_item<WAVES, IPL, false, true, CULL>(T, i0, j0, j1, D, hk);
+            else pair_item<WAVES, IPL, false, false, CULL>(T, i0, j0, j1, D, hk);
         }
     }
     if (cur >= 0) pair_flush(H, P.copies, P.stride, cur_n, cur_hist);
@@ -252,19 +300,20 @@ size_t pair_hist_lds_bytes(int hist_words, int max_n) {
     return (size_t)hist_words * 4 + (size_t)(max_n + 3) * 8;
 }
 
-const void *pair_hist_kernel_ptr(int waves) {
-    if (waves == 8) return reinterpret_cast<const void *>(&pair_hist_kernel<8, kPairIpl>);
-    if (waves == 16) return reinterpret_cast<const void *>(&pair_hist_kernel<16, kPairIpl>);
+const void *pair_hist_kernel_ptr(int waves, bool cull) {
+    if (waves == 8)
+        return cull ? reinterpret_cast<const void *>(&pair_hist_kernel<8, kPairIpl, true>)
+                    : reinterpret_cast<const void *>(&pair_hist_kernel<8, kPairIpl, false>);
     return nullptr;
 }
 
 const void *pair_plain_kernel_ptr() { return reinterpret_cast<const void *>(&pair_plain_kernel); }
 
-hipError_t launch_pair_hist(const PairHistArgs &P, int waves, int grid, hipStream_t st) {
+hipError_t launch_pair_hist(const PairHistArgs &P, int waves, bool cull, int grid, hipStream_t st) {
     const size_t lds = pair_hist_lds_bytes(P.hist_words, P.max_n);
-    if (waves == 8) hipLaunchKernelGGL((pair_hist_kernel<8, kPairIpl>), dim3(grid), dim3(8 * 64), lds, st, P);
-    else if (waves == 16) hipLaunchKernelGGL((pair_hist_kernel<16, kPairIpl>), dim3(grid), dim3(16 * 64), lds, st, P);
-    else return hipErrorInvalidValue;
+    if (waves != 8) return hipErrorInvalidValue;
+    if (cull) hipLaunchKernelGGL((pair_hist_kernel<8, kPairIpl, true>), dim3(grid), dim3(8 * 64), lds, st, P);
+    else hipLaunchKernelGGL((pair_hist_kernel<8, kPairIpl, false>), dim3(grid), dim3(8 * 64), lds, st, P);
     return hipGetLastError();
 }
 

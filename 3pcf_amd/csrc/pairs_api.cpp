@@ -11,6 +11,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <array>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "tpcf_host.h"
@@ -29,6 +32,7 @@ constexpr int kPlainBlock = 256;       // PLAIN kernel: i points per item
 constexpr int kPlainChunk = 4096;      // PLAIN kernel: j points per item
 constexpr double kMaxPairHi = 1e100;   // far-point sentinel of the kernel stays beyond every edge
 constexpr int kMaxSegPoints = 0x7FFFFFFF - 65536;  // 32-bit point indices in the kernels
+constexpr int kCellChunk = 512;        // TPCF_MODE_CELLS: points per box in the host's item culling
 
 int check_pair_job(const tpcf_job &job) {
     if (job.nseg < 1 || job.nseg > 3) return set_error(TPCF_EINVAL, "a job has 1..3 segments, got %d", job.nseg);
@@ -103,6 +107,148 @@ PairShape pair_shape(const tpcf_ctx *ctx, int max_n) {
     return s;
 }
 
+// A segment as the kernel reads it: SoA base pointers and size.
+struct SegView {
+    const double *x, *y, *z;
+    int n;
+    int sorted;          // index of its Morton-ordered copy (TPCF_MODE_CELLS), or -1
+    const double *jbox;  // device boxes of its kPairBoxPoints runs (TPCF_MODE_CELLS)
+};
+
+// Squared FP64 gap between two boxes, in the kernel's order of operations.
+// Every pair (p in a, q in b) has |p_c - q_c| >= the gap on each axis, and
+// FP64 rounding is monotone, so the kernel's squared distance of every pair
+// is at least this value.
+double box_gap2(const double *a, const double *b) {
+    double g[3];
+    for (int c = 0; c < 3; ++c) {
+        const double u = b[c] - a[3 + c], v = a[c] - b[3 + c];
+        g[c] = std::max(0.0, std::max(u, v));
+    }
+    return (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2];
+}
+
+// Morton-ordered copies of the distinct segments of the jobs (on the
+// context's stream) and their chunk boxes (on the host).
+int sort_segments(tpcf_ctx *ctx, const tpcf_job *jobs, int njobs, std::vector<std::array<SegView, 3>> &views,
+                  std::vector<std::vector<double>> &boxes) {
+    std::map<std::tuple<const tpcf_catalog *, int64_t, int64_t>, int> ids;
+    std::vector<std::tuple<const tpcf_catalog *, int64_t, int64_t>> segs;
+    for (int j = 0; j < njobs; ++j)
+        for (int s = 0; s < jobs[j].nseg; ++s) {
+            const auto key = std::make_tuple((const tpcf_catalog *)jobs[j].seg[s], jobs[j].seg_lo[s], jobs[j].seg_hi[s]);
+            auto it = ids.find(key);
+            if (it == ids.end()) {
+                it = ids.emplace(key, (int)segs.size()).first;
+                segs.push_back(key);
+            }
+            views[j][s].sorted = it->second;
+        }
+    const size_t ns = segs.size();
+    std::vector<size_t> o_sorted(ns), o_box(ns);
+    size_t off = 0, scratch = 0;
+    for (size_t i = 0; i < ns; ++i) {
+        const int n = (int)(std::get<2>(segs[i]) - std::get<1>(segs[i]));
+        o_sorted[i] = off;
+        off = align256(off + (size_t)n * 24);
+        o_box[i] = off;
+        off = align256(off + (size_t)((n + kPairBoxPoints - 1) / kPairBoxPoints) * 48);
+        scratch = std::max(scratch, sort_segment_scratch_bytes(n));
+    }
+    const size_t o_scratch = off;
+    if (int e = ctx->pairs.cells.reserve(o_scratch + scratch + 256)) return e;
+    char *base = static_cast<char *>(ctx->pairs.cells.p);
+    boxes.assign(ns, {});
+    for (size_t i = 0; i < ns; ++i) {
+        const tpcf_catalog *c = std::get<0>(segs[i]);
+        const int64_t lo = std::get<1>(segs[i]);
+        const int n = (int)(std::get<2>(segs[i]) - lo);
+        if (n == 0) continue;
+        double blo[3], bhi[3];
+        for (int a = 0; a < 3; ++a) {
+            const double *v = c->h.data() + (size_t)a * c->n + lo;
+            blo[a] = *std::min_element(v, v + n);
+            bhi[a] = *std::max_element(v, v + n);
+        }
+        const double span = std::max({bhi[0] - blo[0], bhi[1] - blo[1], bhi[2] - blo[2]});
+        double *sorted = reinterpret_cast<double *>(base + o_sorted[i]);
+        double *box = reinterpret_cast<double *>(base + o_box[i]);
+        HIP_TRY(sort_segment(c->d + lo, c->d + c->n + lo, c->d + 2 * c->n + lo, n, blo, span, base + o_scratch,
+                             scratch, sorted, kPairBoxPoints, box, ctx->stream));
+        boxes[i].resize((size_t)((n + kPairBoxPoints - 1) / kPairBoxPoints) * 6);
+        HIP_TRY(hipMemcpyAsync(boxes[i].data(), box, boxes[i].size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // Host boxes of kCellChunk points: unions of the device boxes.
+    constexpr int per = kCellChunk / kPairBoxPoints;
+    for (auto &b : boxes) {
+        const size_t nf = b.size() / 6, nc = (nf + per - 1) / per;
+        std::vector<double> cb(nc * 6);
+        for (size_t c = 0; c < nc; ++c) {
+            for (int a = 0; a < 3; ++a) {
+                cb[6 * c + a] = HUGE_VAL;
+                cb[6 * c + 3 + a] = -HUGE_VAL;
+            }
+            for (size_t f = c * per; f < std::min(nf, (c + 1) * per); ++f)
+                for (int a = 0; a < 3; ++a) {
+                    cb[6 * c + a] = std::min(cb[6 * c + a], b[6 * f + a]);
+                    cb[6 * c + 3 + a] = std::max(cb[6 * c + 3 + a], b[6 * f + 3 + a]);
+                }
+        }
+        b.swap(cb);
+    }
+    for (int j = 0; j < njobs; ++j)
+        for (int s = 0; s < jobs[j].nseg; ++s) {
+            SegView &v = views[j][s];
+            const int i = v.sorted;
+            v.n = (int)(std::get<2>(segs[i]) - std::get<1>(segs[i]));
+            const double *p = reinterpret_cast<const double *>(base + o_sorted[i]);
+            v.jbox = reinterpret_cast<const double *>(base + o_box[i]);
+            v.x = p;
+            v.y = p + v.n;
+            v.z = p + 2 * (size_t)v.n;
+        }
+    return TPCF_OK;
+}
+
+// Items of one term: i blocks of `block` points of segment a against runs
+// of j of segment b (at most `chunk` long).  With boxes (TPCF_MODE_CELLS),
+// runs skip the chunks whose box gap to the i block's box reaches T[n].
+void term_pair_items(int slot, int na, int nb, bool same, int block, int chunk, const std::vector<double> *abox,
+                     const std::vector<double> *bbox, double Tn, std::vector<PairHistItem> &out) {
+    for (int64_t i0 = 0; i0 < na; i0 += block) {
+        const int64_t js = same ? i0 + 1 : 0;
+        if (js >= nb) continue;
+        if (!abox) {
+            for (int64_t j0 = js; j0 < nb; j0 += chunk)
+                out.push_back({slot, (int)i0, (int)j0, (int)std::min<int64_t>(nb, j0 + chunk)});
+            continue;
+        }
+        double ib[6] = {HUGE_VAL, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+        for (int64_t c = i0 / kCellChunk; c * kCellChunk < std::min<int64_t>(na, i0 + block); ++c)
+            for (int a = 0; a < 3; ++a) {
+                ib[a] = std::min(ib[a], (*abox)[6 * c + a]);
+                ib[3 + a] = std::max(ib[3 + a], (*abox)[6 * c + 3 + a]);
+            }
+        int64_t r0 = -1, r1 = -1;
+        for (int64_t c = js / kCellChunk; c * kCellChunk < nb; ++c) {
+            const int64_t j0 = std::max<int64_t>(js, c * kCellChunk), j1 = std::min<int64_t>(nb, (c + 1) * kCellChunk);
+            if (box_gap2(ib, bbox->data() + 6 * c) < Tn) {
+                if (r0 < 0) r0 = j0;
+                r1 = j1;
+                if (r1 - r0 >= chunk) {
+                    out.push_back({slot, (int)i0, (int)r0, (int)r1});
+                    r0 = -1;
+                }
+            } else if (r0 >= 0) {
+                out.push_back({slot, (int)i0, (int)r0, (int)r1});
+                r0 = -1;
+            }
+        }
+        if (r0 >= 0) out.push_back({slot, (int)i0, (int)r0, (int)r1});
+    }
+}
+
 }  // namespace
 
 // Count the pair terms of njobs jobs (axis axes[j] for job j) in one launch
@@ -110,9 +256,9 @@ PairShape pair_shape(const tpcf_ctx *ctx, int max_n) {
 static int run_pairs(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_axis *axes, int njobs, int mode,
                      uint64_t *const *hist, tpcf_stats *stats) {
     const auto t0 = std::chrono::steady_clock::now();
-    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_PLAIN && mode != TPCF_MODE_FAST64)
+    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_PLAIN && mode != TPCF_MODE_FAST64 && mode != TPCF_MODE_CELLS)
         return set_error(TPCF_EINVAL, "unknown mode %d", mode);
-    const bool plain = mode == TPCF_MODE_PLAIN;
+    const bool plain = mode == TPCF_MODE_PLAIN, cells = mode == TPCF_MODE_CELLS;
     // Host tables: per job its threshold table; per (job, term) a term record.
     std::vector<std::vector<double>> tables(njobs);
     std::vector<float> hmul(njobs), hadd(njobs);
@@ -145,72 +291,79 @@ static int run_pairs(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_axis *axes,
     if (!plain) {
         const double per_item = (double)pairs_all / (16.0 * shape.grid) / block;
         chunk = (int)std::min(16384.0, std::max(64.0, std::ceil(per_item / 8.0) * 8.0));
+        // Culled runs carry a fraction of the pairs: keep them short enough
+        // to balance the persistent workgroups.
+        if (cells) chunk = std::min(chunk, 2048);
     }
+
+    // Segments as the kernel reads them: the catalogs, or their
+    // Morton-ordered copies with chunk boxes.
+    std::vector<std::array<SegView, 3>> views(njobs);
+    std::vector<std::vector<double>> boxes;
+    if (cells) {
+        if (int e = sort_segments(ctx, jobs, njobs, views, boxes)) return e;
+    } else {
+        for (int j = 0; j < njobs; ++j)
+            for (int s = 0; s < jobs[j].nseg; ++s) {
+                const tpcf_catalog *c = jobs[j].seg[s];
+                const int64_t lo = jobs[j].seg_lo[s];
+                views[j][s] = {c->d + lo, c->d + c->n + lo, c->d + 2 * c->n + lo,
+                               (int)(jobs[j].seg_hi[s] - lo), -1, nullptr};
+            }
+    }
+    // Terms and items.
+    std::vector<PairHistTerm> terms;
+    std::vector<PairHistItem> items;
+    terms.reserve(nterms_all);
+    for (int j = 0; j < njobs; ++j) {
+        const tpcf_job &J = jobs[j];
+        const int n = axes[j].n;
+        for (int t = 0; t < J.nterms; ++t) {
+            const int a = J.term[t][0], b = J.term[t][1];
+            const SegView &A = views[j][a], &B = views[j][b];
+            PairHistTerm T{};
+            T.ax = A.x;
+            T.ay = A.y;
+            T.az = A.z;
+            T.bx = B.x;
+            T.by = B.y;
+            T.bz = B.z;
+            T.na = A.n;
+            T.nb = B.n;
+            T.n = n;
+            T.same = a == b;
+            T.log = axes[j].scale == TPCF_LOG;
+            T.h_mul = hmul[j];
+            T.h_add = hadd[j];
+            T.jbox = B.jbox;
+            const int slot = (int)terms.size();
+            terms.push_back(T);
+            term_pair_items(slot, A.n, B.n, a == b, block, chunk, cells ? &boxes[A.sorted] : nullptr,
+                            cells ? &boxes[B.sorted] : nullptr, tables[j][n + 1], items);
+        }
+    }
+    if (items.size() > 0x7FFFFFFF) return set_error(TPCF_EINVAL, "too many work items (%zu)", items.size());
+    const size_t nitems = items.size();
 
     // Layout of the device buffer.
     const size_t o_hist = 0;
     const size_t o_table = align256(o_hist + hist_words * 8);
     const size_t o_terms = align256(o_table + table_words * 8);
     const size_t o_items = align256(o_terms + (size_t)nterms_all * sizeof(PairHistTerm));
-    std::vector<PairHistTerm> terms;
-    std::vector<PairHistItem> items;
-    terms.reserve(nterms_all);
-    PairWorkspace &W = ctx->pairs;
-    // Items are built after the buffer is sized; term pointers need its base,
-    // so size with a first pass over the item counts.
-    size_t nitems = 0;
-    for (int j = 0; j < njobs; ++j)
-        for (int t = 0; t < jobs[j].nterms; ++t) {
-            const tpcf_job &J = jobs[j];
-            const int a = J.term[t][0], b = J.term[t][1];
-            const int64_t na = J.seg_hi[a] - J.seg_lo[a], nb = J.seg_hi[b] - J.seg_lo[b];
-            for (int64_t i0 = 0; i0 < na; i0 += block) {
-                const int64_t js = a == b ? i0 + 1 : 0;
-                if (js < nb) nitems += (size_t)((nb - js + chunk - 1) / chunk);
-            }
-        }
-    if (nitems > 0x7FFFFFFF) return set_error(TPCF_EINVAL, "too many work items (%zu)", nitems);
     const size_t o_counter = align256(o_items + nitems * sizeof(PairHistItem));
     const size_t bytes = o_counter + 256;
+    PairWorkspace &W = ctx->pairs;
     if (int e = W.buf.reserve(bytes)) return e;
     for (auto &e : W.ev)
         if (!e) HIP_TRY(hipEventCreate(&e));
     char *base = static_cast<char *>(W.buf.p);
     auto *dhist = reinterpret_cast<unsigned long long *>(base + o_hist);
     auto *dtable = reinterpret_cast<double *>(base + o_table);
-    items.reserve(nitems);
-    for (int j = 0; j < njobs; ++j) {
-        const tpcf_job &J = jobs[j];
-        const int n = axes[j].n;
-        for (int t = 0; t < J.nterms; ++t) {
-            const int a = J.term[t][0], b = J.term[t][1];
-            const tpcf_catalog *ca = J.seg[a], *cb = J.seg[b];
-            const int64_t na = J.seg_hi[a] - J.seg_lo[a], nb = J.seg_hi[b] - J.seg_lo[b];
-            PairHistTerm T{};
-            T.ax = ca->d + J.seg_lo[a];
-            T.ay = ca->d + ca->n + J.seg_lo[a];
-            T.az = ca->d + 2 * ca->n + J.seg_lo[a];
-            T.bx = cb->d + J.seg_lo[b];
-            T.by = cb->d + cb->n + J.seg_lo[b];
-            T.bz = cb->d + 2 * cb->n + J.seg_lo[b];
-            T.D = dtable + table_off[j];
-            T.hist = dhist + hist_off[j] + (size_t)t * n;
-            T.na = (int)na;
-            T.nb = (int)nb;
-            T.n = n;
-            T.same = a == b;
-            T.log = axes[j].scale == TPCF_LOG;
-            T.h_mul = hmul[j];
-            T.h_add = hadd[j];
-            const int slot = (int)terms.size();
-            terms.push_back(T);
-            for (int64_t i0 = 0; i0 < na; i0 += block) {
-                const int64_t js = a == b ? i0 + 1 : 0;
-                for (int64_t j0 = js; j0 < nb; j0 += chunk)
-                    items.push_back({slot, (int)i0, (int)j0, (int)std::min<int64_t>(nb, j0 + chunk)});
-            }
+    for (size_t t = 0, k = 0; t < (size_t)njobs; ++t)
+        for (int u = 0; u < jobs[t].nterms; ++u, ++k) {
+            terms[k].D = dtable + table_off[t];
+            terms[k].hist = dhist + hist_off[t] + (size_t)u * axes[t].n;
         }
-    }
 
     // Upload tables, terms and items; clear histograms and the counter.
     hipStream_t st = ctx->stream;
@@ -236,7 +389,7 @@ static int run_pairs(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_axis *axes,
     HIP_TRY(hipEventRecord(W.ev[0], st));
     if (A.nitems > 0) {
         if (plain) HIP_TRY(launch_pair_plain(A, st));
-        else HIP_TRY(launch_pair_hist(A, kPairWaves, shape.grid, st));
+        else HIP_TRY(launch_pair_hist(A, kPairWaves, cells, shape.grid, st));
     }
     HIP_TRY(hipEventRecord(W.ev[1], st));
     for (int j = 0; j < njobs; ++j)

@@ -60,6 +60,7 @@ struct Layout {
 // Device buffer and events of the pair-count calls (pairs_api.cpp).
 struct PairWorkspace {
     DevBuf buf;
+    DevBuf cells;  // sorted segment copies, chunk boxes and sort scratch (TPCF_MODE_CELLS)
     hipEvent_t ev[2] = {nullptr, nullptr};
     ~PairWorkspace();
 };

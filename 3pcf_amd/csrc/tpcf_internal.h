@@ -153,7 +153,12 @@ struct PairHistTerm {
     int same;                  // a and b are one segment: count i < j
     int log;                   // bin guess from log2 (log axis) or sqrt (linear)
     float h_mul, h_add;        // guess k = floor(g * h_mul + h_add)
+    // Culled form: box (lo xyz, hi xyz) of every kPairBoxPoints consecutive
+    // points of segment b; each wave skips the boxes whose gap to its own
+    // points' box reaches the last squared threshold.  nullptr: no skipping.
+    const double *jbox;
 };
+constexpr int kPairBoxPoints = 64;
 // Work item: i in [i0, i0 + block) of segment a against j in [j0, j1) of b.
 struct PairHistItem {
     int term, i0, j0, j1;
@@ -168,11 +173,18 @@ struct PairHistArgs {
     int stride;      // words between copies
     int hist_words;  // copies * stride, rounded up to an even count
 };
-hipError_t launch_pair_hist(const PairHistArgs &P, int waves, int grid, hipStream_t st);
+hipError_t launch_pair_hist(const PairHistArgs &P, int waves, bool cull, int grid, hipStream_t st);
 hipError_t launch_pair_plain(const PairHistArgs &P, hipStream_t st);
 size_t pair_hist_lds_bytes(int hist_words, int max_n);
-const void *pair_hist_kernel_ptr(int waves);  // 8 or 16 waves per workgroup
+const void *pair_hist_kernel_ptr(int waves, bool cull);  // 8 waves per workgroup
 const void *pair_plain_kernel_ptr();
+
+// Morton ordering of a segment (kernels_cells.hip): sorted SoA copy
+// (x[n] y[n] z[n]) and the box (lo xyz, hi xyz) of every chunk of consecutive
+// sorted points.  scratch: sort_segment_scratch_bytes(n) bytes.
+size_t sort_segment_scratch_bytes(int n);
+hipError_t sort_segment(const double *x, const double *y, const double *z, int n, const double lo[3], double span,
+                        void *scratch, size_t scratch_bytes, double *sorted, int chunk, double *box, hipStream_t st);
 
 // Launchers (kernels_r123.hip).
 hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st);

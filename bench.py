@@ -389,6 +389,14 @@ def bench_pairs(args) -> int:
             out, st = ctx.count_pairs(job, ax)
             stats.append(st)
         dt = time.perf_counter() - t0
+        # The culled form (Morton order + box skipping), same histograms.
+        ctx.count_pairs(job, ax, tp.MODE_CELLS)
+        cstats = []
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            cells, stc = ctx.count_pairs(job, ax, tp.MODE_CELLS)
+            cstats.append(stc)
+        dtc = time.perf_counter() - t1
         plain = dtp = None
         if not args.no_plain:
             t1 = time.perf_counter()
@@ -420,6 +428,10 @@ def bench_pairs(args) -> int:
             "plain_kernel_ms": stp.pair_ms if dtp else None,
             "speedup_vs_plain_kernel": stp.pair_ms / stats[0].pair_ms if dtp else None,
             "fast_equals_plain": bool(np.array_equal(out, plain)) if plain is not None else None,
+            "cells_ms_per_step": dtc * 1e3 / reps,
+            "cells_kernel_ms": sum(x.pair_ms for x in cstats) / len(cstats),
+            "cells_pairs_per_s": pairs * reps / dtc,
+            "cells_equals_fast": bool(np.array_equal(out, cells)),
         },
         "roofline": {
             "bound": "valu",

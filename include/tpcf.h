@@ -76,7 +76,9 @@ enum {
                              kernel; (s,q,theta) and JBJ: FP64 pair stage, FP32 entry
                              arithmetic with FP64 recheck of triangles near an edge. */
     TPCF_MODE_PLAIN = 1,  /* FP64 distances recomputed per triplet, one job per launch */
-    TPCF_MODE_FAST64 = 2  /* as FAST with every decision in FP64 */
+    TPCF_MODE_FAST64 = 2, /* as FAST with every decision in FP64 */
+    TPCF_MODE_CELLS = 3   /* pair counts only: FAST over Morton-ordered segments, skipping
+                             every pair of point runs whose boxes lie beyond the last edge */
 };
 
 typedef struct tpcf_ctx tpcf_ctx;
@@ -231,7 +233,12 @@ int64_t tpcf_job_footprint(const tpcf_job *job, const tpcf_binspec *spec, int mo
  * dz*dz) in FP64, the edges as for the triplet axes; others are dropped.
  * hist: nterms * ax->n uint64, term-major.  Modes: TPCF_MODE_FAST (and
  * TPCF_MODE_FAST64, the same kernel: its FP32 arithmetic only guesses which
- * FP64 comparison to make) and TPCF_MODE_PLAIN (binary search per pair).
+ * FP64 comparison to make), TPCF_MODE_PLAIN (binary search per pair) and
+ * TPCF_MODE_CELLS (the FAST kernel over Morton-ordered copies of the
+ * segments, skipping pairs of point runs whose FP64 box gap already reaches
+ * the last squared threshold: same histograms, work roughly proportional to
+ * the pairs within the last edge; the idea of
+ * examples/test_predivision_of_distances.py:11-227).
  * stats->pairs counts the pairs classified.  Replaces the pair loop,
  * distance() and distance_to_bin() of src/2pcf_C_version.cc:358-387, :89-113
  * and :57-86 (700 FP32 bins on [0, 70), self pairs counted in bin 0). */

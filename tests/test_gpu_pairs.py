@@ -21,7 +21,7 @@ tp = importlib.import_module("3pcf_amd")
 synth = importlib.import_module("3pcf_amd.synth")
 
 pytestmark = pytest.mark.gpu
-MODES = [tp.MODE_FAST, tp.MODE_FAST64, tp.MODE_PLAIN]
+MODES = [tp.MODE_FAST, tp.MODE_FAST64, tp.MODE_PLAIN, tp.MODE_CELLS]
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INP = os.path.join(ROOT, "tests", "golden", "inputs")
 TWO = os.path.join(ROOT, "3pcf_amd", "bin", "2pcf_C_version")
@@ -70,7 +70,7 @@ def test_bin_count_extremes(ctx, ax):
     print(f"case bins {ax}", flush=True)
     d = synth.neyman_scott(1500, nclust=15, sigma=4.0, size=120.0, seed=15)
     r = synth.uniform_cube(1300, 120.0, seed=16)
-    for mode in (tp.MODE_FAST, tp.MODE_PLAIN):
+    for mode in (tp.MODE_FAST, tp.MODE_PLAIN, tp.MODE_CELLS):
         got, _ = run(ctx, [d, r], tp.PAIR_TERMS, ax, mode)
         assert np.array_equal(got, want([d, r], tp.PAIR_TERMS, ax)), mode
 
@@ -104,7 +104,7 @@ def test_coincident_points_and_far_offsets(ctx):
     d = np.concatenate([base, base[:300]]) + 700.0
     r = synth.uniform_cube(800, 30.0, seed=20) + 700.0
     for ax in [(64, 0.0, 16.0, "lin"), (64, 0.5, 16.0, "log")]:
-        for mode in (tp.MODE_FAST, tp.MODE_PLAIN):
+        for mode in (tp.MODE_FAST, tp.MODE_PLAIN, tp.MODE_CELLS):
             got, _ = run(ctx, [d, r], tp.PAIR_TERMS, ax, mode)
             assert np.array_equal(got, want([d, r], tp.PAIR_TERMS, ax)), (ax, mode)
     got, _ = run(ctx, [d], [(0, 0)], (64, 0.0, 16.0, "lin"))
@@ -147,9 +147,11 @@ def test_batch_of_voxel_pairs_matches_single_calls(ctx):
     jobs.append(tp.make_job([(cat, 0, 4000)], [(0, 0)]))
     axes.append(tp.axis(30, 0.5, 60.0, "log"))
     outs, st = ctx.count_pairs_batch(jobs, axes)
-    for job, ax, out in zip(jobs, axes, outs):
+    culled, _ = ctx.count_pairs_batch(jobs, axes, tp.MODE_CELLS)
+    for job, ax, out, cul in zip(jobs, axes, outs, culled):
         single, _ = ctx.count_pairs(job, ax)
         assert np.array_equal(out, single)
+        assert np.array_equal(cul, single)
     full = want([d], [(0, 0)], (700, 0.0, 70.0, "lin"))[0]
     assert np.array_equal(sum(o[0] for o in outs[:-1]), full)
     assert np.array_equal(outs[-1], want([d], [(0, 0)], (30, 0.5, 60.0, "log")))
@@ -171,6 +173,11 @@ def test_large_catalog_properties(ctx):
     assert np.array_equal(parts.sum(axis=0), fast[0])
     cover, _ = run(ctx, [d], [(0, 0)], (8, 0.0, 1000.0, "lin"))
     assert int(cover.sum()) == c2(300000)
+    # the culled form on the same catalog: short and long last edges
+    for cax in [ax, (40, 0.5, 12.0, "log"), (8, 0.0, 1000.0, "lin")]:
+        want_fast, _ = run(ctx, [d], [(0, 0)], cax, tp.MODE_FAST)
+        got_cells, _ = run(ctx, [d], [(0, 0)], cax, tp.MODE_CELLS)
+        assert np.array_equal(got_cells, want_fast), cax
 
 
 def test_bad_arguments_raise(ctx):
@@ -207,6 +214,8 @@ CLI_CASES = [
     ["d200.dat", "d200.dat", "-X", "2", "-x", "001000000"],
     ["d200.dat", "r180.dat", "-X", "3", "-x", "002001000"],
     ["d200.dat", "r180.dat", "--bins", "25", "--lo", "1", "--hi", "60", "--log"],
+    ["d200.dat", "r180.dat", "--mode", "cells"],
+    ["d200.dat", "d200.dat", "--mode", "cells", "-X", "2", "-x", "000001000"],
 ]
 
 
@@ -226,6 +235,8 @@ def test_cli_matches_oracle_cli(tmp_path, args):
             hi = next(it)
         elif a == "--log":
             oargs += ["--axis", f"0,{n},{lo},{hi},log"]
+        elif a == "--mode":
+            next(it)
         else:
             oargs.append(a)
     out = tmp_path / "o.dat"
