@@ -31,8 +31,7 @@ R123, SQT, JBJ = 0, 1, 2
 LIN, LOG = 0, 1
 MODE_FAST, MODE_PLAIN, MODE_FAST64, MODE_CELLS = 0, 1, 2, 3
 PARAMS = {"r123": R123, "sqt": SQT, "jbj": JBJ}
-MODES = {"fast": MODE_FAST, "plain": MODE_PLAIN, "fast64": MODE_FAST64}
-PAIR_MODES = {"fast": MODE_FAST, "plain": MODE_PLAIN, "fast64": MODE_FAST64, "cells": MODE_CELLS}
+MODES = {"fast": MODE_FAST, "plain": MODE_PLAIN, "fast64": MODE_FAST64, "cells": MODE_CELLS}
 
 # DDD, DDR, DRR, RRR over segment 0 = data, segment 1 = randoms.
 FOUR_TERMS = ((0, 0, 0), (0, 0, 1), (0, 1, 1), (1, 1, 1))

@@ -14,7 +14,7 @@
 //   --axis I,N,LO,HI[,log]  bin spec of axis I (0..2); r123 uses axis 0
 //   --bins N                r123: N bins per side (with --lo/--hi)
 //   --lo X --hi X --log     r123 range and log scale
-//   --mode fast|plain|fast64
+//   --mode fast|plain|fast64|cells
 //   --format tri|plain      output layout (default per program)
 //   --recheck-rel X         FP64 recheck window (default 1e-5)
 //   --device N, --quiet, --stats
@@ -329,7 +329,8 @@ int main(int argc, char **argv) {
             if (!std::strcmp(optarg, "fast")) mode = TPCF_MODE_FAST;
             else if (!std::strcmp(optarg, "plain")) mode = TPCF_MODE_PLAIN;
             else if (!std::strcmp(optarg, "fast64")) mode = TPCF_MODE_FAST64;
-            else { std::fprintf(stderr, "error: --mode fast|plain|fast64\n"); return 2; }
+            else if (!std::strcmp(optarg, "cells")) mode = TPCF_MODE_CELLS;
+            else { std::fprintf(stderr, "error: --mode fast|plain|fast64|cells\n"); return 2; }
             break;
         case O_FORMAT: format = std::strcmp(optarg, "plain") == 0 ? 1 : 0; break;
         case O_RECHECK: spec.recheck_rel = std::atof(optarg); break;

@@ -47,25 +47,25 @@ __global__ void morton_keys_kernel(const double *__restrict__ x, const double *_
 
 __global__ void gather_kernel(const double *__restrict__ x, const double *__restrict__ y,
                               const double *__restrict__ z, const uint32_t *__restrict__ idx, int n,
-                              double *__restrict__ out) {
+                              int64_t stride, double *__restrict__ out) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t s = idx[i];
         out[i] = x[s];
-        out[n + i] = y[s];
-        out[2 * n + i] = z[s];
+        out[stride + i] = y[s];
+        out[2 * stride + i] = z[s];
     }
 }
 
 // Box of chunk c (points [c*chunk, min(n, (c+1)*chunk))): lo x, y, z then
 // hi x, y, z.
-__global__ void chunk_bounds_kernel(const double *__restrict__ p, int n, int chunk, int nchunks,
+__global__ void chunk_bounds_kernel(const double *__restrict__ p, int n, int64_t stride, int chunk, int nchunks,
                                     double *__restrict__ box) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
         const int b = c * chunk, e = min(n, b + chunk);
-        double lo[3] = {p[b], p[n + b], p[2 * n + b]}, hi[3] = {lo[0], lo[1], lo[2]};
+        double lo[3] = {p[b], p[stride + b], p[2 * stride + b]}, hi[3] = {lo[0], lo[1], lo[2]};
         for (int i = b + 1; i < e; ++i)
             for (int a = 0; a < 3; ++a) {
-                const double v = p[a * n + i];
+                const double v = p[a * stride + i];
                 lo[a] = fmin(lo[a], v);
                 hi[a] = fmax(hi[a], v);
             }
@@ -89,7 +89,7 @@ size_t sort_segment_scratch_bytes(int n) {
 }
 
 hipError_t sort_segment(const double *x, const double *y, const double *z, int n, const double lo[3], double span,
-                        void *scratch, size_t scratch_bytes, double *sorted, int chunk, double *box,
+                        void *scratch, size_t scratch_bytes, double *sorted, int64_t stride, int chunk, double *box,
                         hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const auto a = [](size_t v) { return (v + 255) & ~size_t(255); };
@@ -110,10 +110,11 @@ hipError_t sort_segment(const double *x, const double *y, const double *z, int n
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = rocprim::radix_sort_pairs(tmp, temp, k0, k1, v0, v1, (size_t)n, 0, 3 * kMortonBits, st)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, y, z, v1, n, sorted);
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, y, z, v1, n, stride, sorted);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const int nchunks = (n + chunk - 1) / chunk;
-    hipLaunchKernelGGL(chunk_bounds_kernel, dim3(grid_for(nchunks)), dim3(256), 0, st, sorted, n, chunk, nchunks, box);
+    hipLaunchKernelGGL(chunk_bounds_kernel, dim3(grid_for(nchunks)), dim3(256), 0, st, sorted, n, stride, chunk,
+                       nchunks, box);
     return hipGetLastError();
 }
 

@@ -174,7 +174,7 @@ int sort_segments(tpcf_ctx *ctx, const tpcf_job *jobs, int njobs, std::vector<st
         double *sorted = reinterpret_cast<double *>(base + o_sorted[i]);
         double *box = reinterpret_cast<double *>(base + o_box[i]);
         HIP_TRY(sort_segment(c->d + lo, c->d + c->n + lo, c->d + 2 * c->n + lo, n, blo, span, base + o_scratch,
-                             scratch, sorted, kPairBoxPoints, box, ctx->stream));
+                             scratch, sorted, n, kPairBoxPoints, box, ctx->stream));
         boxes[i].resize((size_t)((n + kPairBoxPoints - 1) / kPairBoxPoints) * 6);
         HIP_TRY(hipMemcpyAsync(boxes[i].data(), box, boxes[i].size() * 8, hipMemcpyDeviceToHost, ctx->stream));
     }

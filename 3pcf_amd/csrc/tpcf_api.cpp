@@ -110,12 +110,65 @@ static uint64_t term_triplets(const Layout &L, const int *tr) {
     return a * b * c;
 }
 
-// Work items of one term: blocks A <= B <= C of the term's segments.
-static void term_items(const Layout &L, const int *tr, int term_slot, std::vector<TripletItem> &out) {
+// Squared FP64 gap between the boxes (lo xyz, hi xyz) of two blocks, in the
+// order of operations of the squared side: a lower bound of the squared side
+// of every pair of their points (each coordinate difference is at least the
+// axis gap and FP64 rounding is monotone).
+static double block_gap2(const double *a, const double *b) {
+    double g[3];
+    for (int c = 0; c < 3; ++c) g[c] = std::max(0.0, std::max(b[c] - a[3 + c], a[c] - b[3 + c]));
+    return (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2];
+}
+
+// Work items of one term: blocks A <= B <= C of the term's segments.  With
+// block boxes (TPCF_MODE_CELLS), only the block triples whose three pairwise
+// gaps are below cut2 (no triplet with a squared side >= cut2 can count).
+static void term_items(const Layout &L, const int *tr, int term_slot, std::vector<TripletItem> &out,
+                       const std::vector<double> *boxes = nullptr, double cut2 = HUGE_VAL) {
     const int64_t a0 = L.seg_blk0[tr[0]], a1 = a0 + L.seg_nblk[tr[0]];
     const int64_t b0 = L.seg_blk0[tr[1]], b1 = b0 + L.seg_nblk[tr[1]];
     const int64_t cb0 = L.seg_blk0[tr[2]], cb1 = cb0 + L.seg_nblk[tr[2]];
     constexpr int64_t kMaxItemBlocks = 512;
+    if (boxes) {
+        const double *bx = boxes->data();
+        // near[X]: the blocks of segment c (for A and B) within reach of X.
+        auto reach = [&](int64_t X, std::vector<int> &nb) {
+            nb.clear();
+            for (int64_t c = cb0; c < cb1; ++c)
+                if (block_gap2(bx + 6 * X, bx + 6 * c) < cut2) nb.push_back((int)c);
+        };
+        std::vector<int> na, nbv, both;
+        for (int64_t A = a0; A < a1; ++A) {
+            reach(A, na);
+            if (na.empty()) continue;
+            for (int64_t B = (tr[0] == tr[1] ? A : b0); B < b1; ++B) {
+                if (!(block_gap2(bx + 6 * A, bx + 6 * B) < cut2)) continue;
+                reach(B, nbv);
+                both.clear();
+                std::set_intersection(na.begin(), na.end(), nbv.begin(), nbv.end(), std::back_inserter(both));
+                const int64_t cs = (tr[1] == tr[2]) ? B : cb0;
+                int64_t r0 = -1, r1 = -1;
+                auto emit = [&]() {
+                    TripletItem it{};
+                    it.term = term_slot;
+                    it.A = (int)A;
+                    it.B = (int)B;
+                    it.c0 = (int)r0;
+                    it.c1 = (int)r1;
+                    out.push_back(it);
+                    r0 = -1;
+                };
+                for (int c : both) {
+                    if (c < cs) continue;
+                    if (r0 >= 0 && (c != r1 || r1 - r0 >= kMaxItemBlocks)) emit();
+                    if (r0 < 0) r0 = c;
+                    r1 = c + 1;
+                }
+                if (r0 >= 0) emit();
+            }
+        }
+        return;
+    }
     for (int64_t A = a0; A < a1; ++A)
         for (int64_t B = (tr[0] == tr[1] ? A : b0); B < b1; ++B) {
             const int64_t cs = (tr[1] == tr[2]) ? B : cb0;
@@ -155,12 +208,18 @@ struct JobPlan {
     bool need_tiles = true;
     double far_s = 0;  // SQT FAST: squared sides above this cannot be in any entry
     tpcf_axis axq{}, axt{};  // SQT: q and theta/pi axes
+    // TPCF_MODE_CELLS: segments staged in Morton order, 64-point block boxes
+    // (filled by stage_job), and the squared side no counted triplet reaches.
+    bool cull = false;
+    double cut2 = HUGE_VAL;
+    mutable std::vector<double> boxes;
+    size_t o_box = 0, o_sort = 0, sort_bytes = 0;
 };
 
 static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, JobPlan &P) {
     if (int e = check_job(job)) return e;
     if (int e = binspec_check(spec)) return e;
-    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_PLAIN && mode != TPCF_MODE_FAST64)
+    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_PLAIN && mode != TPCF_MODE_FAST64 && mode != TPCF_MODE_CELLS)
         return set_error(TPCF_EINVAL, "unknown mode %d", mode);
     if (spec.param != TPCF_R123 && mode != TPCF_MODE_PLAIN && spec.ax[0].n > kMaxFastSqtBins)
         return set_error(TPCF_EINVAL, "the FAST (s,q,theta) kernel takes up to %d s bins", kMaxFastSqtBins);
@@ -246,6 +305,19 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
         const double dmax = spec.ax[0].hi * (1.0 / qlo + vhi) * (1.0 + 1e-6);
         P.far_s = qlo > 0 ? dmax * dmax : HUGE_VAL;
     }
+    P.cull = mode == TPCF_MODE_CELLS;
+    if (P.cull) {
+        // (r1,r2,r3): every side of a counted triplet is below T[n].
+        // (s,q,theta), JBJ: no side of an entry's triangle exceeds far_s.
+        P.cut2 = spec.param == TPCF_R123 ? P.T64[P.n] : std::nextafter(P.far_s, HUGE_VAL);
+        P.o_box = o;
+        o = align256(o + sizeof(double) * 6 * P.L.nblk);
+        int64_t nmax = 0;
+        for (int s = 0; s < P.L.nseg; ++s) nmax = std::max(nmax, P.L.seg_n[s]);
+        P.sort_bytes = sort_segment_scratch_bytes((int)nmax);
+        P.o_sort = o;
+        o = align256(o + P.sort_bytes);
+    }
     P.bytes = o;
     return TPCF_OK;
 }
@@ -259,9 +331,31 @@ static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStrea
         const tpcf_catalog *c = job.seg[s];
         const int64_t m = P.L.seg_n[s];
         if (!m) continue;
+        if (P.cull) {
+            // Morton order (the counts do not depend on the order) and the
+            // box of every 64-point block.
+            double lo[3], hi[3];
+            for (int d = 0; d < 3; ++d) {
+                const double *v = c->h.data() + (size_t)d * c->n + job.seg_lo[s];
+                lo[d] = *std::min_element(v, v + m);
+                hi[d] = *std::max_element(v, v + m);
+            }
+            const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+            const int64_t lo_i = job.seg_lo[s];
+            HIP_TRY(sort_segment(c->d + lo_i, c->d + c->n + lo_i, c->d + 2 * c->n + lo_i, (int)m, lo, span,
+                                 base + P.o_sort, P.sort_bytes, u64 + P.L.seg_off[s], P.L.npad, kBlock,
+                                 reinterpret_cast<double *>(base + P.o_box) + 6 * P.L.seg_blk0[s], st));
+            continue;
+        }
         for (int d = 0; d < 3; ++d)
             HIP_TRY(hipMemcpyAsync(u64 + d * P.L.npad + P.L.seg_off[s], c->d + d * c->n + job.seg_lo[s],
                                    sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+    }
+    if (P.cull && P.L.nblk) {
+        P.boxes.resize((size_t)6 * P.L.nblk);
+        HIP_TRY(hipMemcpyAsync(P.boxes.data(), base + P.o_box, sizeof(double) * P.boxes.size(), hipMemcpyDeviceToHost,
+                               st));
+        HIP_TRY(hipStreamSynchronize(st));
     }
     HIP_TRY(hipMemcpyAsync(base + P.o_blk, P.blk_count.data(), P.blk_count.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(base + P.o_T64, P.T64.data(), sizeof(double) * P.T64.size(), hipMemcpyHostToDevice, st));
@@ -369,7 +463,7 @@ static void build_queue(const std::vector<const JobPlan *> &plans, const std::ve
             T.hist = reinterpret_cast<unsigned long long *>(bases[j] + P.o_hist) + (size_t)t * P.cells;
             T.n = P.n;
             const size_t before = Q.items.size();
-            term_items(P.L, jobs[j]->term[t], (int)Q.terms.size(), Q.items);
+            term_items(P.L, jobs[j]->term[t], (int)Q.terms.size(), Q.items, P.cull ? &P.boxes : nullptr, P.cut2);
             std::stable_sort(Q.items.begin() + before, Q.items.end(), [](const TripletItem &a, const TripletItem &b) {
                 return (a.c1 - a.c0) > (b.c1 - b.c0);
             });
@@ -532,7 +626,7 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
         for (int t = 0; t < P.nterms; ++t) {
             T.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist) + (size_t)t * P.cells;
             const size_t before = items.size();
-            term_items(P.L, jobs[j]->term[t], (int)terms.size(), items);
+            term_items(P.L, jobs[j]->term[t], (int)terms.size(), items, P.cull ? &P.boxes : nullptr, P.cut2);
             std::stable_sort(items.begin() + before, items.end(), [](const TripletItem &a, const TripletItem &b) {
                 return (a.c1 - a.c0) > (b.c1 - b.c0);
             });
@@ -612,6 +706,7 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
     const auto t0 = std::chrono::steady_clock::now();
     JobPlan P;
     if (int e = plan_job(job, spec, mode, P)) return e;
+    if (mode == TPCF_MODE_CELLS) mode = TPCF_MODE_FAST;  // the plan holds the culling
     if (int e = ws.init()) return e;
     if (int e = ws.region.reserve(P.bytes)) return e;
     char *base = reinterpret_cast<char *>(ws.region.p);
@@ -662,7 +757,7 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
 // the batch's kernel time).
 int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode,
               uint64_t *const *hist, tpcf_stats *per, tpcf_stats *total, hipStream_t st) {
-    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_FAST64)
+    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_FAST64 && mode != TPCF_MODE_CELLS)
         return set_error(TPCF_EINVAL, "batched counting runs the FAST paths only");
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<JobPlan> plans(njobs);
@@ -673,6 +768,7 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
         off[j] = bytes;
         bytes = align256(bytes + plans[j].bytes);
     }
+    if (mode == TPCF_MODE_CELLS) mode = TPCF_MODE_FAST;  // the plans hold the culling
     if (int e = ws.init()) return e;
     if (int e = ws.region.reserve(std::max<size_t>(bytes, 256))) return e;
     char *base = reinterpret_cast<char *>(ws.region.p);

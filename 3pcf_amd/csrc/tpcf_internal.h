@@ -180,11 +180,13 @@ const void *pair_hist_kernel_ptr(int waves, bool cull);  // 8 waves per workgrou
 const void *pair_plain_kernel_ptr();
 
 // Morton ordering of a segment (kernels_cells.hip): sorted SoA copy
-// (x[n] y[n] z[n]) and the box (lo xyz, hi xyz) of every chunk of consecutive
-// sorted points.  scratch: sort_segment_scratch_bytes(n) bytes.
+// (x at sorted[0..n), y at sorted[stride..], z at sorted[2*stride..]) and the
+// box (lo xyz, hi xyz) of every chunk of consecutive sorted points.
+// scratch: sort_segment_scratch_bytes(n) bytes.
 size_t sort_segment_scratch_bytes(int n);
 hipError_t sort_segment(const double *x, const double *y, const double *z, int n, const double lo[3], double span,
-                        void *scratch, size_t scratch_bytes, double *sorted, int chunk, double *box, hipStream_t st);
+                        void *scratch, size_t scratch_bytes, double *sorted, int64_t stride, int chunk, double *box,
+                        hipStream_t st);
 
 // Launchers (kernels_r123.hip).
 hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st);

@@ -77,8 +77,11 @@ enum {
                              arithmetic with FP64 recheck of triangles near an edge. */
     TPCF_MODE_PLAIN = 1,  /* FP64 distances recomputed per triplet, one job per launch */
     TPCF_MODE_FAST64 = 2, /* as FAST with every decision in FP64 */
-    TPCF_MODE_CELLS = 3   /* pair counts only: FAST over Morton-ordered segments, skipping
-                             every pair of point runs whose boxes lie beyond the last edge */
+    TPCF_MODE_CELLS = 3   /* FAST over Morton-ordered copies of the segments, skipping every
+                             group of points that cannot hold a counted pair or triplet: pair
+                             runs (pair counts) or block triples ((r1,r2,r3): a side >= the
+                             last edge; (s,q,theta), JBJ: a side beyond any entry) whose box
+                             gaps rule it out.  Same histograms; not for prepared batches */
 };
 
 typedef struct tpcf_ctx tpcf_ctx;

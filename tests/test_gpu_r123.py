@@ -14,7 +14,7 @@ tp = importlib.import_module("3pcf_amd")
 synth = importlib.import_module("3pcf_amd.synth")
 
 pytestmark = pytest.mark.gpu
-MODES = [tp.MODE_FAST, tp.MODE_FAST64, tp.MODE_PLAIN]
+MODES = [tp.MODE_FAST, tp.MODE_FAST64, tp.MODE_PLAIN, tp.MODE_CELLS]
 
 
 def oracle_r123(segs, terms, n, lo, hi, scale="lin"):

@@ -30,7 +30,7 @@ def oracle(segs, terms, param, axes):
     return orc.clean_count(segs, terms, param, [orc.axis(*a) for a in axes])
 
 
-SQT_MODES = [tp.MODE_PLAIN, tp.MODE_FAST, tp.MODE_FAST64]
+SQT_MODES = [tp.MODE_PLAIN, tp.MODE_FAST, tp.MODE_FAST64, tp.MODE_CELLS]
 
 
 @pytest.mark.parametrize("mode", SQT_MODES)
