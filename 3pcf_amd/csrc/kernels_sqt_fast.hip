@@ -206,28 +206,33 @@ __device__ inline void sqt_flush64(uint32_t *h, int cells, unsigned long long *g
 
 // One 64x64 tile (R, C), R <= C, per 256-thread workgroup; thread t owns row
 // t/4 and 16 consecutive columns.
+// One workgroup per tile (R, C), R <= C.  Lane j of each wave owns column j;
+// wave w walks rows 16w .. 16w+15 (the row point is uniform: scalar loads).
+// A row's masks are wave ballots; a column's mask bits stay in the lane's
+// registers and the four waves' parts are OR-ed at the end, so no mask word
+// is written by more than one lane (points in spatial order make whole tiles
+// near, which would serialize atomics on shared words).
 __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
     const int64_t R = blockIdx.y, C = blockIdx.x;
     if (C < R) return;
     __shared__ double sT[kMaxBins + 1];
-    __shared__ unsigned long long smask[kMaskWords];
-    const int t = threadIdx.x;
+    __shared__ unsigned long long colN[4][kBlock], colS[4][kBlock];
+    const int t = threadIdx.x, j = t & 63, w = t >> 6;
     for (int m = t; m <= J.ns; m += 256) sT[m] = J.Ts[m];
-    for (int m = t; m < kMaskWords; m += 256) smask[m] = 0ull;
     __syncthreads();
     const int cntR = J.blk_count[R], cntC = J.blk_count[C];
-    const int i = t >> 2, jb = (t & 3) * 16;
-    const int64_t gi = R * kBlock + i;
-    uint32_t packed[4] = {0u, 0u, 0u, 0u};
-    uint32_t rn = 0u, rs = 0u;  // this thread's 16 columns: near / with an s bin
-    float dist[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int j = jb + q;
+    const int64_t gj = C * kBlock + j;
+    const int64_t tile = tile_index(R, C, J.nblk);
+    uint8_t *codes = J.codes + tile * kTileBytes;
+    float *dists = J.dists + tile * kTileBytes;
+    unsigned long long *masks = J.masks + tile * kMaskWords;
+    unsigned long long cn = 0ull, cs = 0ull;
+    for (int r = 0; r < 16; ++r) {
+        const int i = w * 16 + r;
         uint32_t code = kCodeFar;
         float d = 0.f;
         if (i < cntR && j < cntC && (R != C || j > i)) {
-            const double s = sq64f(J.u64, J.npad, gi, C * kBlock + j);
+            const double s = sq64f(J.u64, J.npad, R * kBlock + i, gj);
             d = (float)sqrt(s);
             if (s == 0.0) {
                 code = kCodeZero;
@@ -236,27 +241,24 @@ __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
                 code = m >= 0 ? (uint32_t)m : kCodeNear;
             }
         }
-        dist[q] = d;
-        packed[q >> 2] |= code << (8 * (q & 3));
-        if (code < kCodeZero) {
-            rn |= 1u << q;
-            atomicOr(&smask[kMaskColNear + j], 1ull << i);
+        codes[i * kBlock + j] = (uint8_t)code;
+        dists[i * kBlock + j] = d;
+        const bool near = code < kCodeZero, sbin = code < kCodeNear;
+        const unsigned long long bn = __ballot(near), bs = __ballot(sbin);
+        if (j == 0) {
+            masks[kMaskRowNear + i] = bn;
+            masks[kMaskRowShort + i] = bs;
         }
-        if (code < kCodeNear) {
-            rs |= 1u << q;
-            atomicOr(&smask[kMaskColShort + j], 1ull << i);
-        }
+        cn |= (unsigned long long)near << i;
+        cs |= (unsigned long long)sbin << i;
     }
-    if (rn) atomicOr(&smask[kMaskRowNear + i], (unsigned long long)rn << jb);
-    if (rs) atomicOr(&smask[kMaskRowShort + i], (unsigned long long)rs << jb);
-    const int64_t tile = tile_index(R, C, J.nblk);
-    *reinterpret_cast<uint4 *>(J.codes + tile * kTileBytes + i * kBlock + jb) =
-        make_uint4(packed[0], packed[1], packed[2], packed[3]);
-    float4 *dd = reinterpret_cast<float4 *>(J.dists + tile * kTileBytes + i * kBlock + jb);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) dd[q] = make_float4(dist[4 * q], dist[4 * q + 1], dist[4 * q + 2], dist[4 * q + 3]);
+    colN[w][j] = cn;
+    colS[w][j] = cs;
     __syncthreads();
-    for (int m = t; m < kMaskWords; m += 256) J.masks[tile * kMaskWords + m] = smask[m];
+    if (t < kBlock) {
+        masks[kMaskColNear + t] = (colN[0][t] | colN[1][t]) | (colN[2][t] | colN[3][t]);
+        masks[kMaskColShort + t] = (colS[0][t] | colS[1][t]) | (colS[2][t] | colS[3][t]);
+    }
 }
 
 namespace {

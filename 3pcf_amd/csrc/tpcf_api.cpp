@@ -120,51 +120,80 @@ static double block_gap2(const double *a, const double *b) {
     return (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2];
 }
 
+// Blocks within reach of each block, as bitsets of nblk bits (block X's
+// words at [X * words, (X + 1) * words)): the blocks whose box gap to it is
+// below cut2.
+struct Reach {
+    int64_t words = 0;
+    std::vector<uint64_t> bits;
+    const uint64_t *row(int64_t X) const { return bits.data() + X * words; }
+    bool has(int64_t X, int64_t Y) const { return (row(X)[Y >> 6] >> (Y & 63)) & 1u; }
+};
+
+static Reach block_reach(const std::vector<double> &boxes, int64_t nblk, double cut2) {
+    Reach r;
+    r.words = (nblk + 63) / 64;
+    r.bits.assign((size_t)(nblk * r.words), 0ull);
+    const double *bx = boxes.data();
+    for (int64_t X = 0; X < nblk; ++X)
+        for (int64_t Y = X; Y < nblk; ++Y)
+            if (block_gap2(bx + 6 * X, bx + 6 * Y) < cut2) {
+                r.bits[(size_t)(X * r.words + (Y >> 6))] |= 1ull << (Y & 63);
+                r.bits[(size_t)(Y * r.words + (X >> 6))] |= 1ull << (X & 63);
+            }
+    return r;
+}
+
 // Work items of one term: blocks A <= B <= C of the term's segments.  With
-// block boxes (TPCF_MODE_CELLS), only the block triples whose three pairwise
-// gaps are below cut2 (no triplet with a squared side >= cut2 can count).
+// the blocks' reach lists (TPCF_MODE_CELLS), only the block triples whose
+// three pairwise box gaps are below the cut (no triplet with a squared side
+// at or above it can count).
 static void term_items(const Layout &L, const int *tr, int term_slot, std::vector<TripletItem> &out,
-                       const std::vector<double> *boxes = nullptr, double cut2 = HUGE_VAL) {
+                       const Reach *reach = nullptr) {
     const int64_t a0 = L.seg_blk0[tr[0]], a1 = a0 + L.seg_nblk[tr[0]];
     const int64_t b0 = L.seg_blk0[tr[1]], b1 = b0 + L.seg_nblk[tr[1]];
     const int64_t cb0 = L.seg_blk0[tr[2]], cb1 = cb0 + L.seg_nblk[tr[2]];
     constexpr int64_t kMaxItemBlocks = 512;
-    if (boxes) {
-        const double *bx = boxes->data();
-        // near[X]: the blocks of segment c (for A and B) within reach of X.
-        auto reach = [&](int64_t X, std::vector<int> &nb) {
-            nb.clear();
-            for (int64_t c = cb0; c < cb1; ++c)
-                if (block_gap2(bx + 6 * X, bx + 6 * c) < cut2) nb.push_back((int)c);
-        };
-        std::vector<int> na, nbv, both;
+    if (reach) {
+        const int64_t W = reach->words;
+        std::vector<uint64_t> both(W);
         for (int64_t A = a0; A < a1; ++A) {
-            reach(A, na);
-            if (na.empty()) continue;
+            const uint64_t *ra = reach->row(A);
             for (int64_t B = (tr[0] == tr[1] ? A : b0); B < b1; ++B) {
-                if (!(block_gap2(bx + 6 * A, bx + 6 * B) < cut2)) continue;
-                reach(B, nbv);
-                both.clear();
-                std::set_intersection(na.begin(), na.end(), nbv.begin(), nbv.end(), std::back_inserter(both));
-                const int64_t cs = (tr[1] == tr[2]) ? B : cb0;
-                int64_t r0 = -1, r1 = -1;
-                auto emit = [&]() {
-                    TripletItem it{};
-                    it.term = term_slot;
-                    it.A = (int)A;
-                    it.B = (int)B;
-                    it.c0 = (int)r0;
-                    it.c1 = (int)r1;
-                    out.push_back(it);
-                    r0 = -1;
-                };
-                for (int c : both) {
-                    if (c < cs) continue;
-                    if (r0 >= 0 && (c != r1 || r1 - r0 >= kMaxItemBlocks)) emit();
-                    if (r0 < 0) r0 = c;
-                    r1 = c + 1;
+                if (!reach->has(A, B)) continue;
+                const uint64_t *rb = reach->row(B);
+                const int64_t cs = std::max<int64_t>((tr[1] == tr[2]) ? B : cb0, cb0);
+                // Runs of consecutive set bits of ra & rb within [cs, cb1).
+                for (int64_t w = cs >> 6; w <= (cb1 - 1) >> 6 && cs < cb1; ++w) {
+                    uint64_t m = ra[w] & rb[w];
+                    if (w == cs >> 6) m &= ~0ull << (cs & 63);
+                    if (w == (cb1 - 1) >> 6 && (cb1 & 63)) m &= ~0ull >> (64 - (cb1 & 63));
+                    both[w] = m;
                 }
-                if (r0 >= 0) emit();
+                int64_t c = cs;
+                while (c < cb1) {
+                    // next set bit at or after c
+                    int64_t w = c >> 6;
+                    uint64_t m = both[w] & (~0ull << (c & 63));
+                    while (!m && ++w <= (cb1 - 1) >> 6) m = both[w];
+                    if (!m) break;
+                    const int64_t r0 = w * 64 + __builtin_ctzll(m);
+                    // end of the run: next clear bit after r0
+                    int64_t e = r0, ew = e >> 6;
+                    uint64_t z = ~both[ew] & (~0ull << (e & 63));
+                    while (!z && ++ew <= (cb1 - 1) >> 6) z = ~both[ew];
+                    const int64_t r1 = std::min<int64_t>(cb1, z ? ew * 64 + __builtin_ctzll(z) : cb1);
+                    for (int64_t q = r0; q < r1; q += kMaxItemBlocks) {
+                        TripletItem it{};
+                        it.term = term_slot;
+                        it.A = (int)A;
+                        it.B = (int)B;
+                        it.c0 = (int)q;
+                        it.c1 = (int)std::min<int64_t>(r1, q + kMaxItemBlocks);
+                        out.push_back(it);
+                    }
+                    c = r1;
+                }
             }
         }
         return;
@@ -213,6 +242,7 @@ struct JobPlan {
     bool cull = false;
     double cut2 = HUGE_VAL;
     mutable std::vector<double> boxes;
+    mutable Reach reach;  // per block: the blocks within the cut
     size_t o_box = 0, o_sort = 0, sort_bytes = 0;
 };
 
@@ -356,6 +386,7 @@ static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStrea
         HIP_TRY(hipMemcpyAsync(P.boxes.data(), base + P.o_box, sizeof(double) * P.boxes.size(), hipMemcpyDeviceToHost,
                                st));
         HIP_TRY(hipStreamSynchronize(st));
+        P.reach = block_reach(P.boxes, P.L.nblk, P.cut2);
     }
     HIP_TRY(hipMemcpyAsync(base + P.o_blk, P.blk_count.data(), P.blk_count.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(base + P.o_T64, P.T64.data(), sizeof(double) * P.T64.size(), hipMemcpyHostToDevice, st));
@@ -463,7 +494,7 @@ static void build_queue(const std::vector<const JobPlan *> &plans, const std::ve
             T.hist = reinterpret_cast<unsigned long long *>(bases[j] + P.o_hist) + (size_t)t * P.cells;
             T.n = P.n;
             const size_t before = Q.items.size();
-            term_items(P.L, jobs[j]->term[t], (int)Q.terms.size(), Q.items, P.cull ? &P.boxes : nullptr, P.cut2);
+            term_items(P.L, jobs[j]->term[t], (int)Q.terms.size(), Q.items, P.cull ? &P.reach : nullptr);
             std::stable_sort(Q.items.begin() + before, Q.items.end(), [](const TripletItem &a, const TripletItem &b) {
                 return (a.c1 - a.c0) > (b.c1 - b.c0);
             });
@@ -626,7 +657,7 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
         for (int t = 0; t < P.nterms; ++t) {
             T.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist) + (size_t)t * P.cells;
             const size_t before = items.size();
-            term_items(P.L, jobs[j]->term[t], (int)terms.size(), items, P.cull ? &P.boxes : nullptr, P.cut2);
+            term_items(P.L, jobs[j]->term[t], (int)terms.size(), items, P.cull ? &P.reach : nullptr);
             std::stable_sort(items.begin() + before, items.end(), [](const TripletItem &a, const TripletItem &b) {
                 return (a.c1 - a.c0) > (b.c1 - b.c0);
             });

@@ -118,6 +118,18 @@ def bench_batch(args) -> int:
         fast = [o.copy() for o in outs]
         info = b.info
         b.close()
+        # The culled form (Morton-ordered voxels, block triples out of reach
+        # skipped) as one count_batch call, against the same call unculled.
+        specs = [spec] * len(jobs)
+        ctx.count_batch(jobs, specs, tp.MODE_CELLS)
+        t1 = time.perf_counter()
+        couts, cst = ctx.count_batch(jobs, specs, tp.MODE_CELLS)
+        cells_s = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        uouts, ust = ctx.count_batch(jobs, specs, tp.MODE_FAST)
+        uncull_s = time.perf_counter() - t1
+        cells_exact = all(np.array_equal(a, b_) for a, b_ in zip(couts, fast)) and \
+            all(np.array_equal(a, b_) for a, b_ in zip(uouts, fast))
         # plain FP64, one voxel job per launch
         plain_lat = []
         exact = True
@@ -161,6 +173,11 @@ def bench_batch(args) -> int:
             "plain_job_latency_p99_ms": pct(plain_lat, 99) * 1e3,
             "speedup_vs_plain": fast_rate / plain_rate,
             "histograms_identical_to_plain": exact,
+            "count_batch_fast_ms": uncull_s * 1e3,
+            "count_batch_cells_ms": cells_s * 1e3,
+            "count_batch_cells_triplet_kernel_ms": cst.triplet_ms,
+            "cells_voxel_jobs_per_s": nv / cells_s,
+            "cells_histograms_identical": cells_exact,
         },
         "roofline": {
             "bound": "valu",
@@ -308,6 +325,10 @@ def bench_sqt(args) -> int:
             out, st = ctx.count(job, spec, tp.MODE_FAST)
             stats.append(st)
         dt = time.perf_counter() - t0
+        ctx.count(job, spec, tp.MODE_CELLS)
+        t1 = time.perf_counter()
+        outc, stc = ctx.count(job, spec, tp.MODE_CELLS)
+        dtc = time.perf_counter() - t1
         out64 = plain = None
         dt64 = dtp = None
         if not args.no_plain:
@@ -350,6 +371,9 @@ def bench_sqt(args) -> int:
             "speedup_vs_plain": dtp / (dt / reps) if dtp else None,
             "fast_equals_fast64": bool(np.array_equal(out, out64)) if out64 is not None else None,
             "fast_equals_plain": bool(np.array_equal(out, plain)) if plain is not None else None,
+            "cells_ms_per_step": dtc * 1e3,
+            "cells_triplet_kernel_ms": stc.triplet_ms,
+            "cells_equals_fast": bool(np.array_equal(out, outc)),
         },
         "roofline": {
             "bound": "valu",
@@ -485,6 +509,10 @@ def bench_precision(args) -> int:
             stats.append(st)
         dt = time.perf_counter() - t0
         out64, st64 = ctx.count(job, spec, tp.MODE_FAST64)
+        ctx.count(job, spec, tp.MODE_CELLS)
+        t1 = time.perf_counter()
+        outc, stc = ctx.count(job, spec, tp.MODE_CELLS)
+        dtc = time.perf_counter() - t1
     triplets = stats[0].triplets
     tri_s = sum(x.triplet_ms for x in stats) / len(stats) / 1e3
     achieved = OPS_PER_TRIPLET * triplets / tri_s
@@ -513,6 +541,9 @@ def bench_precision(args) -> int:
             "pair_stage_fp64_ms": st64.pair_ms,
             "triplet_kernel_ms": stats[0].triplet_ms,
             "fast_equals_fast64": bool(np.array_equal(out, out64)),
+            "cells_ms_per_step": dtc * 1e3,
+            "cells_triplet_kernel_ms": stc.triplet_ms,
+            "cells_equals_fast": bool(np.array_equal(out, outc)),
         },
         "roofline": {
             "bound": "valu",
