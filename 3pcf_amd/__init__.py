@@ -341,6 +341,24 @@ class Context:
     def footprint(self, job: Job, spec: BinSpec, mode: int = MODE_FAST) -> int:
         return _check(self.lib.tpcf_job_footprint(C.byref(job), C.byref(spec), mode))
 
+    def footprint_triplets(self, job: Job, spec: BinSpec) -> int:
+        """Unique triplets of the job's terms (host arithmetic on the segment sizes)."""
+        c2 = lambda n: n * (n - 1) // 2
+        c3 = lambda n: n * (n - 1) * (n - 2) // 6
+        n = [job.seg_hi[s] - job.seg_lo[s] for s in range(job.nseg)]
+        total = 0
+        for t in range(job.nterms):
+            a, b, c = (job.term[t][k] for k in range(3))
+            if a == b == c:
+                total += c3(n[a])
+            elif a == b:
+                total += c2(n[a]) * n[c]
+            elif b == c:
+                total += n[a] * c2(n[b])
+            else:
+                total += n[a] * n[b] * n[c]
+        return total
+
     def submit(self, job: Job, spec: BinSpec, mode: int = MODE_FAST):
         """Enqueue a job; returns (ticket, output array).  Keep the array alive."""
         out = np.zeros((job.nterms, hist_cells(spec)), dtype=np.uint64)

@@ -5,16 +5,20 @@
 // The reference has no counterpart; it runs one OS process per (term, voxel)
 // job (examples/build_voxel_submission_scripts.py:9-41).  Here a context owns
 // a small pool of workers, each with its own HIP stream and workspace.  An
-// idle worker takes every queued job that fits the memory left in the budget
-// (first fit, in arrival order, up to its share of the queued work, so that
-// the other workers get jobs too; a job larger than the whole budget fails
-// with TPCF_EBUSY) and runs them as one batch: pair stages back to back, then one
+// idle worker takes queued jobs that fit the memory left in the budget, the
+// least work first (jobs waiting longer than kStarveSeconds come first, in
+// arrival order, so a steady stream of small jobs cannot hold a large one
+// back), up to its share of the queued work so that the other workers get
+// jobs too and within a band of sizes (a batch completes with its largest
+// job; a job larger than the whole budget fails with TPCF_EBUSY), and runs
+// them as one batch: pair stages back to back, then one
 // persistent triplet launch over all their work items (one for the
 // (r1,r2,r3) jobs, one for the triangle parametrizations).  Jobs that arrive
 // while a
 // batch is on the GPU are picked up by another worker as soon as memory
 // allows, so their kernels overlap with the running batch on the device.
 // PLAIN-mode jobs (the FP64 baseline) run one per launch.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -41,6 +45,12 @@ static bool solo(const Ticket &t) { return t.mode == TPCF_MODE_PLAIN; }
 
 static constexpr int kWorkers = 3;
 static constexpr int kMaxBatch = 64;
+static constexpr double kStarveSeconds = 60.0;
+// A batch holds jobs of at most kBandFactor times its first job's work (or
+// kBandFloor triplets): a batch completes with its largest job, so mixing
+// sizes would hold small jobs back.
+static constexpr double kBandFactor = 4.0;
+static constexpr double kBandFloor = 1e9;
 
 static void worker_main(tpcf_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
@@ -59,19 +69,36 @@ static void worker_main(tpcf_ctx *ctx) {
                 uint64_t queued = 0, taken = 0;
                 for (int64_t id : ctx->pending) queued += ctx->tickets[id]->work;
                 const uint64_t share = queued / kWorkers;
-                // First fit in arrival order; PLAIN jobs run alone.
-                for (auto it = ctx->pending.begin();
-                     it != ctx->pending.end() && (int)batch.size() < kMaxBatch && (batch.empty() || taken < share);) {
-                    auto t = ctx->tickets[*it];
+                // Visit order: starving jobs by arrival, then the rest by
+                // work (stable: ties keep arrival order).
+                const auto now = std::chrono::steady_clock::now();
+                std::vector<int64_t> order(ctx->pending.begin(), ctx->pending.end());
+                auto starving = [&](int64_t id) {
+                    return std::chrono::duration<double>(now - ctx->tickets[id]->t_submit).count() > kStarveSeconds;
+                };
+                std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+                    const bool sa = starving(a), sb = starving(b);
+                    if (sa != sb) return sa;
+                    if (sa) return false;
+                    return ctx->tickets[a]->work < ctx->tickets[b]->work;
+                });
+                // First fit in that order; PLAIN jobs run alone.
+                for (size_t k = 0; k < order.size() && (int)batch.size() < kMaxBatch && (batch.empty() || taken < share);
+                     ++k) {
+                    const int64_t id = order[k];
+                    auto t = ctx->tickets[id];
                     const bool plain = solo(*t);
-                    if (plain && !batch.empty()) { ++it; continue; }
+                    if (plain && !batch.empty()) continue;
                     if (!batch.empty() && solo(*batch[0])) break;
-                    if (!batch.empty() && t->mode != batch[0]->mode) { ++it; continue; }
+                    if (!batch.empty() && t->mode != batch[0]->mode) continue;
+                    if (!batch.empty() &&
+                        (double)t->work > kBandFactor * std::max((double)batch[0]->work, kBandFloor))
+                        continue;
                     if ((uint64_t)t->footprint > ctx->budget) {
                         t->state = 3;
                         t->rc = TPCF_EBUSY;
                         t->err = "job workspace exceeds the device memory budget";
-                        it = ctx->pending.erase(it);
+                        ctx->pending.erase(std::find(ctx->pending.begin(), ctx->pending.end(), id));
                         ctx->cv.notify_all();
                         continue;
                     }
@@ -80,11 +107,9 @@ static void worker_main(tpcf_ctx *ctx) {
                         taken += t->work;
                         t->state = 1;
                         batch.push_back(t);
-                        ids.push_back(*it);
-                        it = ctx->pending.erase(it);
+                        ids.push_back(id);
+                        ctx->pending.erase(std::find(ctx->pending.begin(), ctx->pending.end(), id));
                         if (plain) break;
-                    } else {
-                        ++it;
                     }
                 }
                 if (!batch.empty()) break;
