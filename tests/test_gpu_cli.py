@@ -96,3 +96,32 @@ def test_voxel_driver_plus_reducer_equals_full_run(tmp_path):
          os.path.join(INP, "d200.dat"), os.path.join(INP, "r180.dat"), "--axis", "0,10,1,12", "--axis", "1,8,1,3",
          "--axis", "2,8,0,1", "-o", str(full), "--quiet"])
     assert summed == full.read_bytes() + b"\n"
+
+
+@pytest.mark.parametrize("program,mode", [("3pcf_C_version_triangle_parametrization", "clean-sqt"),
+                                          ("3pcf_C_version_triangle_parametrization_JBJ", "clean-jbj")])
+def test_triangle_programs_cells_mode(program, mode, tmp_path):
+    """--mode cells (spatial order + culling) writes the same bytes."""
+    orc.build()
+    for order in ("DDD", "DRR"):
+        a, b = tmp_path / "a.dat", tmp_path / "b.dat"
+        run([prog(program), *ORDERS[order], "--mode", "cells", "-o", str(a), "--quiet"])
+        run([orc.CLI, mode, "--threads", "8", *ORDERS[order], "-o", str(b)])
+        assert a.read_bytes() == b.read_bytes(), order
+
+
+def test_cube_program_cells_mode(tmp_path):
+    orc.build()
+    a, b = tmp_path / "a.dat", tmp_path / "b.dat"
+    run([prog("3pcf_C_version"), "input_1k_0.cat", "input_1k_0.cat", "input_1k_1.cat", "--mode", "cells", "-o",
+         str(a), "--quiet"])
+    run([prog("3pcf_C_version"), "input_1k_0.cat", "input_1k_0.cat", "input_1k_1.cat", "-o", str(b), "--quiet"])
+    assert a.read_bytes() == b.read_bytes()
+
+
+def test_two_point_program_ignores_l_flag(tmp_path):
+    """src/2pcf_C_version.cc:148-150 parses -l but bins linearly regardless."""
+    a, b = tmp_path / "a.dat", tmp_path / "b.dat"
+    run([prog("2pcf_C_version"), "d200.dat", "r180.dat", "-l", "1", "-o", str(a)])
+    run([prog("2pcf_C_version"), "d200.dat", "r180.dat", "-o", str(b)])
+    assert a.read_bytes() == b.read_bytes()
