@@ -112,14 +112,15 @@ constexpr double kFarPoint = 1e150;
 // j <= i, which are skipped.
 template <int WAVES, int IPL, bool LOG, bool DIAG, bool CULL>
 __device__ __forceinline__ void pair_item(CPairTerm *T, int i0, int j0, int j1, const double *D, uint32_t *hk) {
-    constexpr int BI = WAVES * 64;
     const int na = T->na, n = T->n;
     const gdouble *ax = (const gdouble *)T->ax, *ay = (const gdouble *)T->ay, *az = (const gdouble *)T->az;
     double xi[IPL], yi[IPL], zi[IPL];
     int jmin[IPL];  // DIAG: the lane counts the pairs with j > jmin
+    // The lane's points are consecutive, so a wave's 64 * IPL points are one
+    // run of the segment (a compact box in Morton order).
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
-        const int i = i0 + q * BI + (int)threadIdx.x;
+        const int i = i0 + (int)threadIdx.x * IPL + q;
         if (i < na) {
             xi[q] = ax[i];
             yi[q] = ay[i];
