@@ -61,6 +61,7 @@ template <bool F32>
 __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
     const int64_t R = blockIdx.y, C = blockIdx.x;
     if (C < R) return;
+    if (J.need && !((J.need[R * J.need_words + (C >> 6)] >> (C & 63)) & 1ull)) return;
     __shared__ float4 sR[kBlock], sC[kBlock];
     __shared__ double sT64[kMaxBins + 1];
     __shared__ float sT32[kMaxBins + 1];

@@ -215,6 +215,7 @@ __device__ inline void sqt_flush64(uint32_t *h, int cells, unsigned long long *g
 __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
     const int64_t R = blockIdx.y, C = blockIdx.x;
     if (C < R) return;
+    if (J.need && !((J.need[R * J.need_words + (C >> 6)] >> (C & 63)) & 1ull)) return;
     __shared__ double sT[kMaxBins + 1];
     __shared__ unsigned long long colN[4][kBlock], colS[4][kBlock];
     const int t = threadIdx.x, j = t & 63, w = t >> 6;

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
+#include <thread>
 
 #include "tpcf_host.h"
 
@@ -130,17 +131,34 @@ struct Reach {
     bool has(int64_t X, int64_t Y) const { return (row(X)[Y >> 6] >> (Y & 63)) & 1u; }
 };
 
+// Run body(lo, hi, worker) over [0, n) split into contiguous ranges on up to
+// 16 host threads (one range per thread; small n stays on the caller).
+template <class F>
+static void parallel_ranges(int64_t n, F body) {
+    const int64_t nt = std::min<int64_t>(std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency())),
+                                         std::max<int64_t>(1, n / 64));
+    if (nt <= 1) {
+        body((int64_t)0, n, 0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int64_t w = 0; w < nt; ++w) th.emplace_back(body, n * w / nt, n * (w + 1) / nt, (int)w);
+    for (auto &t : th) t.join();
+}
+
 static Reach block_reach(const std::vector<double> &boxes, int64_t nblk, double cut2) {
     Reach r;
     r.words = (nblk + 63) / 64;
     r.bits.assign((size_t)(nblk * r.words), 0ull);
     const double *bx = boxes.data();
-    for (int64_t X = 0; X < nblk; ++X)
-        for (int64_t Y = X; Y < nblk; ++Y)
-            if (block_gap2(bx + 6 * X, bx + 6 * Y) < cut2) {
-                r.bits[(size_t)(X * r.words + (Y >> 6))] |= 1ull << (Y & 63);
-                r.bits[(size_t)(Y * r.words + (X >> 6))] |= 1ull << (X & 63);
-            }
+    // Each thread fills whole rows X (every Y), so no word is shared.
+    parallel_ranges(nblk, [&](int64_t x0, int64_t x1, int) {
+        for (int64_t X = x0; X < x1; ++X) {
+            uint64_t *row = r.bits.data() + (size_t)(X * r.words);
+            for (int64_t Y = 0; Y < nblk; ++Y)
+                if (block_gap2(bx + 6 * X, bx + 6 * Y) < cut2) row[Y >> 6] |= 1ull << (Y & 63);
+        }
+    });
     return r;
 }
 
@@ -156,46 +174,53 @@ static void term_items(const Layout &L, const int *tr, int term_slot, std::vecto
     constexpr int64_t kMaxItemBlocks = 512;
     if (reach) {
         const int64_t W = reach->words;
-        std::vector<uint64_t> both(W);
-        for (int64_t A = a0; A < a1; ++A) {
-            const uint64_t *ra = reach->row(A);
-            for (int64_t B = (tr[0] == tr[1] ? A : b0); B < b1; ++B) {
-                if (!reach->has(A, B)) continue;
-                const uint64_t *rb = reach->row(B);
-                const int64_t cs = std::max<int64_t>((tr[1] == tr[2]) ? B : cb0, cb0);
-                // Runs of consecutive set bits of ra & rb within [cs, cb1).
-                for (int64_t w = cs >> 6; w <= (cb1 - 1) >> 6 && cs < cb1; ++w) {
-                    uint64_t m = ra[w] & rb[w];
-                    if (w == cs >> 6) m &= ~0ull << (cs & 63);
-                    if (w == (cb1 - 1) >> 6 && (cb1 & 63)) m &= ~0ull >> (64 - (cb1 & 63));
-                    both[w] = m;
-                }
-                int64_t c = cs;
-                while (c < cb1) {
-                    // next set bit at or after c
-                    int64_t w = c >> 6;
-                    uint64_t m = both[w] & (~0ull << (c & 63));
-                    while (!m && ++w <= (cb1 - 1) >> 6) m = both[w];
-                    if (!m) break;
-                    const int64_t r0 = w * 64 + __builtin_ctzll(m);
-                    // end of the run: next clear bit after r0
-                    int64_t e = r0, ew = e >> 6;
-                    uint64_t z = ~both[ew] & (~0ull << (e & 63));
-                    while (!z && ++ew <= (cb1 - 1) >> 6) z = ~both[ew];
-                    const int64_t r1 = std::min<int64_t>(cb1, z ? ew * 64 + __builtin_ctzll(z) : cb1);
-                    for (int64_t q = r0; q < r1; q += kMaxItemBlocks) {
-                        TripletItem it{};
-                        it.term = term_slot;
-                        it.A = (int)A;
-                        it.B = (int)B;
-                        it.c0 = (int)q;
-                        it.c1 = (int)std::min<int64_t>(r1, q + kMaxItemBlocks);
-                        out.push_back(it);
+        // Blocks A of segment a split over host threads, items gathered per
+        // thread and appended in A order.
+        std::vector<std::vector<TripletItem>> part(16);
+        parallel_ranges(a1 - a0, [&](int64_t lo, int64_t hi, int worker) {
+            std::vector<TripletItem> &mine = part[worker];
+            std::vector<uint64_t> both(W);
+            for (int64_t A = a0 + lo; A < a0 + hi; ++A) {
+                const uint64_t *ra = reach->row(A);
+                for (int64_t B = (tr[0] == tr[1] ? A : b0); B < b1; ++B) {
+                    if (!reach->has(A, B)) continue;
+                    const uint64_t *rb = reach->row(B);
+                    const int64_t cs = std::max<int64_t>((tr[1] == tr[2]) ? B : cb0, cb0);
+                    // Runs of consecutive set bits of ra & rb within [cs, cb1).
+                    for (int64_t w = cs >> 6; w <= (cb1 - 1) >> 6 && cs < cb1; ++w) {
+                        uint64_t m = ra[w] & rb[w];
+                        if (w == cs >> 6) m &= ~0ull << (cs & 63);
+                        if (w == (cb1 - 1) >> 6 && (cb1 & 63)) m &= ~0ull >> (64 - (cb1 & 63));
+                        both[w] = m;
                     }
-                    c = r1;
+                    int64_t c = cs;
+                    while (c < cb1) {
+                        // next set bit at or after c
+                        int64_t w = c >> 6;
+                        uint64_t m = both[w] & (~0ull << (c & 63));
+                        while (!m && ++w <= (cb1 - 1) >> 6) m = both[w];
+                        if (!m) break;
+                        const int64_t r0 = w * 64 + __builtin_ctzll(m);
+                        // end of the run: next clear bit after r0
+                        int64_t e = r0, ew = e >> 6;
+                        uint64_t z = ~both[ew] & (~0ull << (e & 63));
+                        while (!z && ++ew <= (cb1 - 1) >> 6) z = ~both[ew];
+                        const int64_t r1 = std::min<int64_t>(cb1, z ? ew * 64 + __builtin_ctzll(z) : cb1);
+                        for (int64_t q = r0; q < r1; q += kMaxItemBlocks) {
+                            TripletItem it{};
+                            it.term = term_slot;
+                            it.A = (int)A;
+                            it.B = (int)B;
+                            it.c0 = (int)q;
+                            it.c1 = (int)std::min<int64_t>(r1, q + kMaxItemBlocks);
+                            mine.push_back(it);
+                        }
+                        c = r1;
+                    }
                 }
             }
-        }
+        });
+        for (auto &p : part) out.insert(out.end(), p.begin(), p.end());
         return;
     }
     for (int64_t A = a0; A < a1; ++A)
@@ -243,7 +268,7 @@ struct JobPlan {
     double cut2 = HUGE_VAL;
     mutable std::vector<double> boxes;
     mutable Reach reach;  // per block: the blocks within the cut
-    size_t o_box = 0, o_sort = 0, sort_bytes = 0;
+    size_t o_box = 0, o_sort = 0, sort_bytes = 0, o_need = 0;
 };
 
 static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, JobPlan &P) {
@@ -347,6 +372,10 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
         P.sort_bytes = sort_segment_scratch_bytes((int)nmax);
         P.o_sort = o;
         o = align256(o + P.sort_bytes);
+        // The reach bitsets on the device: the pair stage skips the tiles
+        // no culled item reads.
+        P.o_need = o;
+        o = align256(o + sizeof(uint64_t) * (size_t)P.L.nblk * (size_t)((P.L.nblk + 63) / 64));
     }
     P.bytes = o;
     return TPCF_OK;
@@ -387,6 +416,8 @@ static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStrea
                                st));
         HIP_TRY(hipStreamSynchronize(st));
         P.reach = block_reach(P.boxes, P.L.nblk, P.cut2);
+        HIP_TRY(hipMemcpyAsync(base + P.o_need, P.reach.bits.data(), sizeof(uint64_t) * P.reach.bits.size(),
+                               hipMemcpyHostToDevice, st));
     }
     HIP_TRY(hipMemcpyAsync(base + P.o_blk, P.blk_count.data(), P.blk_count.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(base + P.o_T64, P.T64.data(), sizeof(double) * P.T64.size(), hipMemcpyHostToDevice, st));
@@ -416,6 +447,8 @@ static PairJob pair_job(const JobPlan &P, char *base, int shift) {
     J.rmax = P.rmax;
     J.rel = P.rel;
     J.shift = shift;
+    J.need = P.cull ? reinterpret_cast<const unsigned long long *>(base + P.o_need) : nullptr;
+    J.need_words = (P.L.nblk + 63) / 64;
     return J;
 }
 
@@ -628,6 +661,8 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
         J.far_s = P.far_s;
         J.npad = P.L.npad;
         J.nblk = P.L.nblk;
+        J.need = P.cull ? reinterpret_cast<const unsigned long long *>(base + P.o_need) : nullptr;
+        J.need_words = (P.L.nblk + 63) / 64;
         HIP_TRY(launch_sqt_pairs(J, st));
         SqtTerm T{};
         T.codes = J.codes;

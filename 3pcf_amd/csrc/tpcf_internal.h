@@ -51,6 +51,10 @@ struct PairJob {
     float rmax;            // max |coordinate - origin| over the union (FP32, rounded up)
     float rel;             // extra relative window around each threshold (squared units)
     int shift;             // stored pair value = bin << shift (2 for the LDS kernel, 0 otherwise)
+    // Culled form: bit C of word [R * need_words + C / 64] set for the tiles
+    // (R, C) the triplet stage reads; the others are skipped.  nullptr: all.
+    const unsigned long long *need;
+    int64_t need_words;
 };
 
 // Plain FP64 baseline of one job.
@@ -102,6 +106,8 @@ struct SqtPairJob {
     int ns;
     double far_s;    // squared sides above this are kCodeFar
     int64_t npad, nblk;
+    const unsigned long long *need;  // as PairJob::need
+    int64_t need_words;
 };
 // Per-tile mask words: row near, row short, column near, column short.
 constexpr int kMaskWords = 4 * 64;
