@@ -259,8 +259,10 @@ class Catalog:
 
 
 def make_job(segments: Sequence[tuple], terms: Iterable[tuple]) -> Job:
-    """segments: (Catalog, lo, hi) triples; terms: sorted label triples."""
+    """segments: (Catalog, lo, hi) triples; terms: sorted label triples (pair
+    terms: label pairs).  The job keeps its catalogs alive."""
     job = Job()
+    job._catalogs = [seg[0] for seg in segments]
     job.nseg = len(segments)
     for s, (cat, lo, hi) in enumerate(segments):
         job.seg[s] = cat.handle.value
@@ -383,6 +385,7 @@ class Batch:
     def __init__(self, ctx: Context, jobs: Sequence[Job], specs: Sequence[BinSpec], mode: int, flags: int):
         self.ctx = ctx
         n = len(jobs)
+        self._keep = list(jobs)  # the jobs (and their catalogs) live as long as the batch
         self._jobs = (Job * n)(*jobs)
         self._specs = (BinSpec * n)(*specs)
         self.outs = [np.zeros((j.nterms, hist_cells(s)), dtype=np.uint64) for j, s in zip(jobs, specs)]

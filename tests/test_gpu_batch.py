@@ -185,3 +185,19 @@ def test_serving_mix_with_sqt_alone_equals_mix(ctx):
     for (t, out), a in zip(tickets, alone):
         ctx.wait(t)
         assert np.array_equal(out, a)
+
+
+def test_job_keeps_temporary_catalogs_alive(ctx):
+    """A job built from catalogs nobody else references still counts them."""
+    import gc
+    d = synth.uniform_cube(300, 20.0, seed=71)
+    r = synth.uniform_cube(250, 20.0, seed=72)
+    job = tp.four_term_job(ctx.catalog(d), ctx.catalog(r))
+    gc.collect()
+    got, _ = ctx.count(job, tp.r123_spec(8, 0.0, 10.0))
+    want = orc.clean_count([d, r], tp.FOUR_TERMS, orc.R123, [orc.axis(8, 0.0, 10.0)])
+    assert np.array_equal(got, want)
+    pj = tp.pair_job(ctx.catalog(d), ctx.catalog(r))
+    gc.collect()
+    pairs, _ = ctx.count_pairs(pj, tp.axis(10, 0.0, 10.0))
+    assert np.array_equal(pairs, orc.clean_pairs([d, r], tp.PAIR_TERMS, orc.axis(10, 0.0, 10.0)))
