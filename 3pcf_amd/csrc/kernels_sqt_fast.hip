@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
     if (C < R) return;
     if (J.need && !((J.need[R * J.need_words + (C >> 6)] >> (C & 63)) & 1ull)) return;
     __shared__ double sT[kMaxBins + 1];
-    __shared__ unsigned long long colN[4][kBlock], colS[4][kBlock];
+    __shared__ unsigned long long colN[4][kBlock], colS[4][kBlock], colM[4][kBlock];
     const int t = threadIdx.x, j = t & 63, w = t >> 6;
     for (int m = t; m <= J.ns; m += 256) sT[m] = J.Ts[m];
     __syncthreads();
@@ -227,11 +227,12 @@ __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
     uint8_t *codes = J.codes + tile * kTileBytes;
     float *dists = J.dists + tile * kTileBytes;
     unsigned long long *masks = J.masks + tile * kMaskWords;
-    unsigned long long cn = 0ull, cs = 0ull;
+    unsigned long long cn = 0ull, cs = 0ull, cm = 0ull;
     for (int r = 0; r < 16; ++r) {
         const int i = w * 16 + r;
         uint32_t code = kCodeFar;
         float d = 0.f;
+        bool mid = false;
         if (i < cntR && j < cntC && (R != C || j > i)) {
             const double s = sq64f(J.u64, J.npad, R * kBlock + i, gj);
             d = (float)sqrt(s);
@@ -240,25 +241,30 @@ __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
             } else if (!(s > J.far_s)) {
                 const int m = bin_asc64(s, sT, J.ns);
                 code = m >= 0 ? (uint32_t)m : kCodeNear;
+                mid = !(s > J.mid_s) || m >= 0;
             }
         }
         codes[i * kBlock + j] = (uint8_t)code;
         dists[i * kBlock + j] = d;
         const bool near = code < kCodeZero, sbin = code < kCodeNear;
-        const unsigned long long bn = __ballot(near), bs = __ballot(sbin);
+        const unsigned long long bn = __ballot(near), bs = __ballot(sbin), bm = __ballot(mid);
         if (j == 0) {
             masks[kMaskRowNear + i] = bn;
             masks[kMaskRowShort + i] = bs;
+            masks[kMaskRowMid + i] = bm;
         }
         cn |= (unsigned long long)near << i;
         cs |= (unsigned long long)sbin << i;
+        cm |= (unsigned long long)mid << i;
     }
     colN[w][j] = cn;
     colS[w][j] = cs;
+    colM[w][j] = cm;
     __syncthreads();
     if (t < kBlock) {
         masks[kMaskColNear + t] = (colN[0][t] | colN[1][t]) | (colN[2][t] | colN[3][t]);
         masks[kMaskColShort + t] = (colS[0][t] | colS[1][t]) | (colS[2][t] | colS[3][t]);
+        masks[kMaskColMid + t] = (colM[0][t] | colM[1][t]) | (colM[2][t] | colM[3][t]);
     }
 }
 
@@ -392,12 +398,17 @@ __device__ inline uint32_t wave_inclusive_sum(uint32_t x, int lane) {
 }
 
 // Candidate columns j of row i for this lane: sides (i,j), (i,k), (j,k) all
-// near and at least one with an s bin.
-__device__ inline unsigned long long row_candidates(unsigned long long xn, unsigned long long xs, int i,
-                                                    unsigned long long yn, unsigned long long ys,
-                                                    unsigned long long zn, unsigned long long zs) {
+// near, at least one with an s bin and at least two mid.
+__device__ inline unsigned long long row_candidates(unsigned long long xn, unsigned long long xs,
+                                                    unsigned long long xm, int i, unsigned long long yn,
+                                                    unsigned long long ys, unsigned long long ym,
+                                                    unsigned long long zn, unsigned long long zs,
+                                                    unsigned long long zm) {
     const unsigned long long yb = 1ull << i;
-    return (yn & yb) ? (xn & zn & ((ys & yb) ? ~0ull : (xs | zs))) : 0ull;
+    if (!(yn & yb)) return 0ull;
+    const unsigned long long shortx = (ys & yb) ? ~0ull : (xs | zs);
+    const unsigned long long mid2 = (ym & yb) ? (xm | zm) : (xm & zm);
+    return xn & zn & shortx & mid2;
 }
 
 // Persistent over (term, A, B, C-range) items.  Candidate triangles (no far
@@ -414,7 +425,7 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
     extern __shared__ uint32_t hist[];
     __shared__ double sT[kMaxBins + 1], sQ[kMaxBins + 1], sA[kMaxBins + 1];
     __shared__ float fQ[kMaxBins + 1], fA[kMaxBins + 1];
-    __shared__ unsigned long long xnear[kBlock], xshort[kBlock];
+    __shared__ unsigned long long xnear[kBlock], xshort[kBlock], xmid[kBlock];
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -473,6 +484,7 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         for (int i = threadIdx.x; i < kBlock; i += blockDim.x) {
             xnear[i] = mAB[kMaskRowNear + i];
             xshort[i] = mAB[kMaskRowShort + i];
+            xmid[i] = mAB[kMaskRowMid + i];
         }
         __syncthreads();
         // This wave's ring of queued candidates: entries [qhead, qtail)
@@ -495,15 +507,17 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
             // Lane k's column of tiles (B,C) and (A,C): bits j of side (j,k)
             // and bits i of side (i,k).
             const unsigned long long *mBC = T.masks + tile_index(X.B, C, nblk) * kMaskWords;
-            const unsigned long long zn = mBC[kMaskColNear + lane], zs = mBC[kMaskColShort + lane];
+            const unsigned long long zn = mBC[kMaskColNear + lane], zs = mBC[kMaskColShort + lane],
+                                     zm = mBC[kMaskColMid + lane];
             if (__ballot(zn != 0ull) == 0ull) continue;
             const unsigned long long *mAC = T.masks + tile_index(X.A, C, nblk) * kMaskWords;
-            const unsigned long long yn = mAC[kMaskColNear + lane], ys = mAC[kMaskColShort + lane];
+            const unsigned long long yn = mAC[kMaskColNear + lane], ys = mAC[kMaskColShort + lane],
+                                     ym = mAC[kMaskColMid + lane];
             const uint32_t coff = (uint32_t)(C - X.c0) << 18;
             for (int i = 0; i < kBlock; ++i) {
                 const unsigned long long xn = xnear[i];
                 if (xn == 0ull) continue;
-                unsigned long long m = row_candidates(xn, xshort[i], i, yn, ys, zn, zs);
+                unsigned long long m = row_candidates(xn, xshort[i], xmid[i], i, yn, ys, ym, zn, zs, zm);
                 const uint32_t cnt = (uint32_t)__popcll(m);
                 const uint32_t incl = wave_inclusive_sum(cnt, lane);
                 const uint32_t total = __builtin_amdgcn_readlane(incl, 63);

@@ -261,6 +261,7 @@ struct JobPlan {
     size_t bytes = 0;
     bool need_tiles = true;
     double far_s = 0;  // SQT FAST: squared sides above this cannot be in any entry
+    double mid_s = 0;  // SQT FAST: squared q sides above this cannot be in any entry
     tpcf_axis axq{}, axt{};  // SQT: q and theta/pi axes
     // TPCF_MODE_CELLS: segments staged in Morton order, 64-point block boxes
     // (filled by stage_job), and the squared side no counted triplet reaches.
@@ -352,6 +353,10 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
         // most their sum); a relative margin keeps the cut conservative.
         const double dmax = spec.ax[0].hi * (1.0 + std::max(spec.ax[1].hi, 1.0)) * (1.0 + 1e-6);
         P.far_s = dmax * dmax;
+        // An entry's q side is below q_hi times its s side, which is below
+        // s_hi (and the s side is short itself).
+        const double dmid = spec.ax[0].hi * std::max(spec.ax[1].hi, 1.0) * (1.0 + 1e-6);
+        P.mid_s = dmid * dmid;
     } else if (sqt_fast) {
         // JBJ: the shortest side l1 < s_hi, l2 <= l1/q_lo and l3 < l2 +
         // v_hi*l1, so no side of an entry reaches s_hi*(1/q_lo + v_hi); with
@@ -359,6 +364,9 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
         const double qlo = spec.ax[1].lo, vhi = std::max(spec.ax[2].hi, 0.0);
         const double dmax = spec.ax[0].hi * (1.0 / qlo + vhi) * (1.0 + 1e-6);
         P.far_s = qlo > 0 ? dmax * dmax : HUGE_VAL;
+        // The entry's second side l2 <= l1 / q_lo < s_hi / q_lo.
+        const double dmid = spec.ax[0].hi / qlo * (1.0 + 1e-6);
+        P.mid_s = qlo > 0 ? dmid * dmid : HUGE_VAL;
     }
     P.cull = mode == TPCF_MODE_CELLS;
     if (P.cull) {
@@ -659,6 +667,7 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
         J.Ts = reinterpret_cast<const double *>(base + P.o_T64);
         J.ns = P.n;
         J.far_s = P.far_s;
+        J.mid_s = P.mid_s;
         J.npad = P.L.npad;
         J.nblk = P.L.nblk;
         J.need = P.cull ? reinterpret_cast<const unsigned long long *>(base + P.o_need) : nullptr;

@@ -99,19 +99,24 @@ struct SqtPairJob {
     const uint8_t *blk_count;
     uint8_t *codes;  // 4 KiB per tile
     float *dists;    // 16 KiB per tile: (float)sqrt(s) of the FP64 squared side
-    // 2 KiB per tile, kMaskWords words: bit j of row i near / with an s bin,
-    // bit i of column j near / with an s bin.
+    // 3 KiB per tile, kMaskWords words: bit j of row i near / with an s bin /
+    // mid, bit i of column j likewise.
     unsigned long long *masks;
     const double *Ts;
     int ns;
     double far_s;    // squared sides above this are kCodeFar
+    double mid_s;    // "mid": near and squared side at most this (a q side bound)
     int64_t npad, nblk;
     const unsigned long long *need;  // as PairJob::need
     int64_t need_words;
 };
-// Per-tile mask words: row near, row short, column near, column short.
-constexpr int kMaskWords = 4 * 64;
+// Per-tile mask words: row near, row short, column near, column short, row
+// mid, column mid.  Every entry has its s side short and its q side mid, and
+// a short side is mid too, so a triangle with an entry has at least two mid
+// sides.
+constexpr int kMaskWords = 6 * 64;
 constexpr int kMaskRowNear = 0, kMaskRowShort = 64, kMaskColNear = 128, kMaskColShort = 192;
+constexpr int kMaskRowMid = 256, kMaskColMid = 320;
 // One (job, term) of a FAST (s,q,theta) launch.
 struct SqtTerm {
     const uint8_t *codes;
