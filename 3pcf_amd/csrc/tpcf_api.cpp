@@ -266,6 +266,7 @@ struct JobPlan {
     // TPCF_MODE_CELLS: segments staged in Morton order, 64-point block boxes
     // (filled by stage_job), and the squared side no counted triplet reaches.
     bool cull = false;
+    bool morton = false;  // segments staged in Morton order (cull, or the (s,q,theta)/JBJ FAST paths)
     double cut2 = HUGE_VAL;
     mutable std::vector<double> boxes;
     mutable Reach reach;  // per block: the blocks within the cut
@@ -369,10 +370,16 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
         P.mid_s = qlo > 0 ? dmid * dmid : HUGE_VAL;
     }
     P.cull = mode == TPCF_MODE_CELLS;
-    if (P.cull) {
+    // Spatial order also helps the (s,q,theta) and JBJ kernels unculled: rows
+    // and blocks of the masks are empty as a whole more often (10k + 10k,
+    // four terms, s [2,20) in a 200 Mpc/h cube: 50.8 -> 42.8 ms).  The
+    // (r1,r2,r3) kernel keeps the catalog order: similar bins along a tile
+    // row would raise same-address LDS conflicts.
+    P.morton = P.cull || sqt_fast;
+    if (P.morton) {
         // (r1,r2,r3): every side of a counted triplet is below T[n].
         // (s,q,theta), JBJ: no side of an entry's triangle exceeds far_s.
-        P.cut2 = spec.param == TPCF_R123 ? P.T64[P.n] : std::nextafter(P.far_s, HUGE_VAL);
+        if (P.cull) P.cut2 = spec.param == TPCF_R123 ? P.T64[P.n] : std::nextafter(P.far_s, HUGE_VAL);
         P.o_box = o;
         o = align256(o + sizeof(double) * 6 * P.L.nblk);
         int64_t nmax = 0;
@@ -398,7 +405,7 @@ static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStrea
         const tpcf_catalog *c = job.seg[s];
         const int64_t m = P.L.seg_n[s];
         if (!m) continue;
-        if (P.cull) {
+        if (P.morton) {
             // Morton order (the counts do not depend on the order) and the
             // box of every 64-point block.
             double lo[3], hi[3];
