@@ -1217,7 +1217,7 @@ int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *s
                       tpcf_batch **out) {
     clear_error();
     if (!ctx || !jobs || !specs || njobs < 1 || !out) return set_error(TPCF_EINVAL, "bad batch arguments");
-    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_FAST64)
+    if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_FAST64 && mode != TPCF_MODE_CELLS)
         return set_error(TPCF_EINVAL, "batches run the FAST paths only");
     HIP_TRY(hipSetDevice(ctx->device));
     std::unique_ptr<tpcf_batch> b(new tpcf_batch());
@@ -1237,6 +1237,9 @@ int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *s
         bytes = align256(bytes + b->plans[j].bytes);
         max_n = std::max(max_n, b->plans[j].n);
     }
+    // CELLS: the plans hold the culling (Morton staging, reach, culled
+    // items, all done once below); the replayed kernels are FAST's.
+    if (mode == TPCF_MODE_CELLS) b->mode = TPCF_MODE_FAST;
     b->shift = max_n <= kMaxLdsR123Bins ? 2 : 0;
     HIP_TRY(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
     for (auto &e : b->ev) HIP_TRY(hipEventCreate(&e));

@@ -119,8 +119,18 @@ def bench_batch(args) -> int:
         info = b.info
         b.close()
         # The culled form (Morton-ordered voxels, block triples out of reach
-        # skipped) as one count_batch call, against the same call unculled.
+        # skipped) as a prepared batch (culled once, graph-replayed) and as
+        # one count_batch call, against the same call unculled.
         specs = [spec] * len(jobs)
+        bc = ctx.batch(jobs, specs, tp.MODE_CELLS, autotune=True, graph=True)
+        bc.run()
+        clat = []
+        for _ in range(reps):
+            t1 = time.perf_counter()
+            couts_p, _ = bc.run()
+            clat.append(time.perf_counter() - t1)
+        cells_prepared_exact = all(np.array_equal(a, b_) for a, b_ in zip(couts_p, fast))
+        bc.close()
         ctx.count_batch(jobs, specs, tp.MODE_CELLS)
         t1 = time.perf_counter()
         couts, cst = ctx.count_batch(jobs, specs, tp.MODE_CELLS)
@@ -178,6 +188,10 @@ def bench_batch(args) -> int:
             "count_batch_cells_triplet_kernel_ms": cst.triplet_ms,
             "cells_voxel_jobs_per_s": nv / cells_s,
             "cells_histograms_identical": cells_exact,
+            "cells_prepared_voxel_jobs_per_s": nv / (sum(clat) / len(clat)),
+            "cells_prepared_latency_p50_ms": pct(clat, 50) * 1e3,
+            "cells_prepared_latency_p99_ms": pct(clat, 99) * 1e3,
+            "cells_prepared_identical": cells_prepared_exact,
         },
         "roofline": {
             "bound": "valu",

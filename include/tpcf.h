@@ -81,7 +81,7 @@ enum {
                              group of points that cannot hold a counted pair or triplet: pair
                              runs (pair counts) or block triples ((r1,r2,r3): a side >= the
                              last edge; (s,q,theta), JBJ: a side beyond any entry) whose box
-                             gaps rule it out.  Same histograms; not for prepared batches */
+                             gaps rule it out.  Same histograms */
 };
 
 typedef struct tpcf_ctx tpcf_ctx;

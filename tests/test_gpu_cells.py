@@ -102,8 +102,22 @@ def test_culled_batch_and_serving_equal_fast(ctx):
         assert np.array_equal(out, want)
 
 
-def test_prepared_batch_rejects_cells(ctx):
-    d = ctx.catalog(synth.uniform_cube(100, 10.0, seed=0))
-    job = tp.make_job([(d, 0, 100)], [(0, 0, 0)])
-    with pytest.raises(tp.TpcfError):
-        ctx.batch([job], [tp.r123_spec(8, 0.0, 5.0)], tp.MODE_CELLS)
+def test_prepared_batch_cells_replays_equal_fast(ctx):
+    """A prepared CELLS batch (staged and culled once, graph-replayed)
+    gives the unculled histograms on every replay."""
+    print("case prepared cells batch", flush=True)
+    jobs, want = [], []
+    for k in range(5):
+        d = synth.neyman_scott(900 + 150 * k, nclust=20, sigma=2.0, size=200.0, seed=80 + k)
+        r = synth.uniform_cube(800 + 100 * k, 200.0, seed=90 + k)
+        job, cats = job_of(ctx, [d, r], tp.FOUR_TERMS)
+        jobs.append(job)
+        want.append(ctx.count(job, tp.r123_spec(16, 1.0, 30.0, "log"), tp.MODE_FAST)[0])
+    specs = [tp.r123_spec(16, 1.0, 30.0, "log")] * len(jobs)
+    for graph in (True, False):
+        b = ctx.batch(jobs, specs, tp.MODE_CELLS, autotune=False, graph=graph)
+        for _ in range(3):
+            outs, _ = b.run()
+            for o, w in zip(outs, want):
+                assert np.array_equal(o, w)
+        b.close()
