@@ -648,19 +648,25 @@ static int launch_plain(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tp
     return TPCF_OK;
 }
 
-// FAST (s,q,theta): FP64 pair codes and FP32 side lengths, then the
-// persistent candidate kernel over the items of every term (FP32 entries,
-// FP64 recheck near edges; FAST64 sends every candidate through FP64).
-// FAST (s,q,theta) for several jobs on stream st: every job's pair stage,
-// then one persistent triplet launch over the items of every job and term
-// (a workgroup reloads the edge tables and flushes its histogram when the
-// term changes).  bases[j] is job j's staged device region.
-static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<const JobPlan *> &plans,
-                                const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases, int mode,
-                                hipStream_t st) {
+// FAST (s,q,theta) and JBJ for several jobs: every job's pair stage (FP64
+// pair codes, FP32 side lengths, candidate masks), then one persistent
+// candidate launch over the items of every job and term (FP32 entries, FP64
+// recheck near edges; FAST64 sends every candidate through FP64).  A
+// workgroup reloads the edge tables and flushes its histogram when the term
+// changes.  The host part (SqtLaunch) is built once; sqt_enqueue launches
+// kernels only, so a prepared batch can capture it in a graph.
+struct SqtLaunch {
+    std::vector<SqtPairJob> pairs;
     std::vector<SqtTerm> terms;
     std::vector<TripletItem> items;
-    int lds_cells = 1;
+    int lds_cells = 1, waves = 8, qcap = 0, per_cu = 0;
+    size_t o_terms = 0, o_ctr = 0;
+};
+
+static int sqt_plan(tpcf_ctx *ctx, const std::vector<const JobPlan *> &plans,
+                    const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases, int mode,
+                    SqtLaunch &L) {
+    L = SqtLaunch{};
     for (size_t j = 0; j < plans.size(); ++j) {
         const JobPlan &P = *plans[j];
         char *base = bases[j];
@@ -679,7 +685,7 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
         J.nblk = P.L.nblk;
         J.need = P.cull ? reinterpret_cast<const unsigned long long *>(base + P.o_need) : nullptr;
         J.need_words = (P.L.nblk + 63) / 64;
-        HIP_TRY(launch_sqt_pairs(J, st));
+        L.pairs.push_back(J);
         SqtTerm T{};
         T.codes = J.codes;
         T.dists = J.dists;
@@ -704,43 +710,28 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
         };
         guess(P.axq, T.q_lo, T.q_inv, T.q_log);
         guess(P.axt, T.t_lo, T.t_inv, T.t_log);
-        lds_cells = std::max(lds_cells, T.cells);
+        L.lds_cells = std::max(L.lds_cells, T.cells);
         for (int t = 0; t < P.nterms; ++t) {
             T.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist) + (size_t)t * P.cells;
-            const size_t before = items.size();
-            term_items(P.L, jobs[j]->term[t], (int)terms.size(), items, P.cull ? &P.reach : nullptr);
-            std::stable_sort(items.begin() + before, items.end(), [](const TripletItem &a, const TripletItem &b) {
-                return (a.c1 - a.c0) > (b.c1 - b.c0);
-            });
-            terms.push_back(T);
+            const size_t before = L.items.size();
+            term_items(P.L, jobs[j]->term[t], (int)L.terms.size(), L.items, P.cull ? &P.reach : nullptr);
+            std::stable_sort(L.items.begin() + before, L.items.end(),
+                             [](const TripletItem &a, const TripletItem &b) { return (a.c1 - a.c0) > (b.c1 - b.c0); });
+            L.terms.push_back(T);
         }
     }
-    HIP_TRY(hipEventRecord(ws.ev[1], st));
-    if (items.empty()) {
-        HIP_TRY(hipEventRecord(ws.ev[3], st));
-        return TPCF_OK;
-    }
-    const size_t o_terms = align256(sizeof(TripletItem) * items.size());
-    const size_t o_ctr = align256(o_terms + sizeof(SqtTerm) * terms.size());
-    if (int e = ws.sqt_queue.reserve(o_ctr + 256)) return e;
-    char *q = reinterpret_cast<char *>(ws.sqt_queue.p);
-    HIP_TRY(hipMemcpyAsync(q, items.data(), sizeof(TripletItem) * items.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(q + o_terms, terms.data(), sizeof(SqtTerm) * terms.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(q + o_ctr), 1, 0u, st));
-    SqtFastArgs A{};
-    A.terms = reinterpret_cast<const SqtTerm *>(q + o_terms);
-    A.items = reinterpret_cast<const TripletItem *>(q);
-    A.nitems = (int)items.size();
-    A.counter = reinterpret_cast<int *>(q + o_ctr);
-    A.lds_cells = lds_cells;
+    if (L.items.empty()) return TPCF_OK;
+    L.o_terms = align256(sizeof(TripletItem) * L.items.size());
+    L.o_ctr = align256(L.o_terms + sizeof(SqtTerm) * L.terms.size());
     // Candidate rings of 512 words per wave, fewer where the histogram needs
     // the LDS (the limit on cells keeps 128 possible); 8 waves per workgroup,
     // or 16 when the LDS would leave a CU a single 8-wave workgroup.
-    const auto fit = [&](int waves, int &qcap, size_t &lds, int &per_cu) -> bool {
+    const auto fit = [&](int waves, int &qcap, int &per_cu) -> bool {
         hipFuncAttributes fa{};
         if (hipFuncGetAttributes(&fa, sqt_fast_kernel_ptr(waves)) != hipSuccess) return false;
+        size_t lds = 0;
         for (qcap = waves == 8 ? 512 : 256; qcap >= 128; qcap /= 2) {
-            lds = sqt_fast_lds_bytes(lds_cells, waves, qcap);
+            lds = sqt_fast_lds_bytes(L.lds_cells, waves, qcap);
             if (fa.sharedSizeBytes + lds <= 160 * 1024) break;
         }
         if (qcap < 128) return false;
@@ -750,26 +741,62 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
             per_cu = 0;
         return per_cu >= 1;
     };
-    int waves = 8, qcap = 0, per_cu = 0;
-    size_t lds = 0;
-    const bool ok8 = fit(8, qcap, lds, per_cu);
-    if (!ok8 || per_cu < 2) {
+    const bool ok8 = fit(8, L.qcap, L.per_cu);
+    if (!ok8 || L.per_cu < 2) {
         int q16 = 0, pc16 = 0;
-        size_t l16 = 0;
-        if (fit(16, q16, l16, pc16)) {
-            waves = 16;
-            qcap = q16;
-            per_cu = pc16;
+        if (fit(16, q16, pc16)) {
+            L.waves = 16;
+            L.qcap = q16;
+            L.per_cu = pc16;
         } else if (!ok8) {
             (void)hipGetLastError();
             return set_error(TPCF_EINVAL, "(s,q,theta) histogram of %d cells does not fit the FAST kernel's LDS",
-                             lds_cells);
+                             L.lds_cells);
         }
     }
-    A.qcap = qcap;
-    HIP_TRY(hipEventRecord(ws.ev[3], st));
-    HIP_TRY(launch_sqt_fast(A, waves, ctx->num_cu * per_cu, st));
     return TPCF_OK;
+}
+
+// Copies the items and term descriptors of L into buf (asynchronously on st).
+static int sqt_upload(const SqtLaunch &L, DevBuf &buf, hipStream_t st) {
+    if (L.items.empty()) return TPCF_OK;
+    if (int e = buf.reserve(L.o_ctr + 256)) return e;
+    char *q = reinterpret_cast<char *>(buf.p);
+    HIP_TRY(hipMemcpyAsync(q, L.items.data(), sizeof(TripletItem) * L.items.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(q + L.o_terms, L.terms.data(), sizeof(SqtTerm) * L.terms.size(), hipMemcpyHostToDevice,
+                           st));
+    return TPCF_OK;
+}
+
+// The kernels of L: pair stages, then the counter reset and the persistent
+// candidate launch.  When given, ev_pairs is recorded after the pair stages
+// and ev_kernel before the candidate launch (stage timings).
+static int sqt_enqueue(tpcf_ctx *ctx, const SqtLaunch &L, const DevBuf &buf, hipStream_t st, hipEvent_t ev_pairs,
+                       hipEvent_t ev_kernel) {
+    for (const SqtPairJob &J : L.pairs) HIP_TRY(launch_sqt_pairs(J, st));
+    if (ev_pairs) HIP_TRY(hipEventRecord(ev_pairs, st));
+    if (ev_kernel) HIP_TRY(hipEventRecord(ev_kernel, st));
+    if (L.items.empty()) return TPCF_OK;
+    char *q = reinterpret_cast<char *>(buf.p);
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(q + L.o_ctr), 1, 0u, st));
+    SqtFastArgs A{};
+    A.terms = reinterpret_cast<const SqtTerm *>(q + L.o_terms);
+    A.items = reinterpret_cast<const TripletItem *>(q);
+    A.nitems = (int)L.items.size();
+    A.counter = reinterpret_cast<int *>(q + L.o_ctr);
+    A.lds_cells = L.lds_cells;
+    A.qcap = L.qcap;
+    HIP_TRY(launch_sqt_fast(A, L.waves, ctx->num_cu * L.per_cu, st));
+    return TPCF_OK;
+}
+
+static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<const JobPlan *> &plans,
+                                const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases, int mode,
+                                hipStream_t st) {
+    SqtLaunch L;
+    if (int e = sqt_plan(ctx, plans, jobs, bases, mode, L)) return e;
+    if (int e = sqt_upload(L, ws.sqt_queue, st)) return e;
+    return sqt_enqueue(ctx, L, ws.sqt_queue, st, ws.ev[1], ws.ev[3]);
 }
 
 // Pair-stage and triplet-kernel times from the workspace events.
@@ -1114,8 +1141,9 @@ void tpcf_free(void *p) { free(p); }
 
 // ---------------------------------------------------------------------------
 // prepared batches: stage once, autotune the launch shape, capture the
-// counting sequence (histogram reset, pair stage, persistent triplet kernel)
-// in a HIP graph and replay it
+// counting sequence (histogram reset, pair stages, the persistent candidate
+// kernel of the (s,q,theta) and JBJ jobs, the persistent triplet kernel of
+// the (r1,r2,r3) jobs) in a HIP graph and replay it
 // ---------------------------------------------------------------------------
 
 struct tpcf_batch {
@@ -1125,7 +1153,8 @@ struct tpcf_batch {
     std::vector<tpcf_job> jobs;
     std::vector<JobPlan> plans;
     std::vector<size_t> off;
-    DevBuf region, queue;
+    DevBuf region, queue, sqt_queue;
+    SqtLaunch sqt;  // the (s,q,theta) and JBJ jobs' launch
     hipStream_t st = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     QueueDev qd;
@@ -1154,10 +1183,12 @@ static int batch_compute(tpcf_batch *b, const TripletShape &sh, hipStream_t st) 
         HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(jb + P.o_cnt), 16, 0u, st));
         HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(jb + P.o_hist), 2 * P.cells * P.nterms, 0u, st));
     }
+    if (int e = sqt_enqueue(b->ctx, b->sqt, b->sqt_queue, st, nullptr, nullptr)) return e;
     for (int j = 0; j < b->njobs; ++j)
-        if (b->plans[j].L.nblk)
+        if (b->plans[j].param == TPCF_R123 && b->plans[j].L.nblk)
             HIP_TRY(launch_pair_tiles(pair_job(b->plans[j], base + b->off[j], b->shift), b->mode == TPCF_MODE_FAST,
                                       st));
+    if (!b->qd.nitems) return TPCF_OK;
     return launch_queue(b->ctx, b->qd, sh, st);
 }
 
@@ -1230,12 +1261,10 @@ int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *s
     size_t bytes = 0;
     int max_n = 1;
     for (int j = 0; j < njobs; ++j) {
-        if (specs[j].param != TPCF_R123)
-            return set_error(TPCF_EINVAL, "batches hold (r1,r2,r3) jobs; count triangle parametrizations one by one");
         if (int e = plan_job(jobs[j], specs[j], mode, b->plans[j])) return e;
         b->off[j] = bytes;
         bytes = align256(bytes + b->plans[j].bytes);
-        max_n = std::max(max_n, b->plans[j].n);
+        if (specs[j].param == TPCF_R123) max_n = std::max(max_n, b->plans[j].n);
     }
     // CELLS: the plans hold the culling (Morton staging, reach, culled
     // items, all done once below); the replayed kernels are FAST's.
@@ -1245,19 +1274,22 @@ int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *s
     for (auto &e : b->ev) HIP_TRY(hipEventCreate(&e));
     if (int e = b->region.reserve(std::max<size_t>(bytes, 256))) return e;
     char *base = reinterpret_cast<char *>(b->region.p);
-    std::vector<const JobPlan *> pp;
-    std::vector<const tpcf_job *> jj;
-    std::vector<char *> bb;
+    std::vector<const JobPlan *> pp, sp;
+    std::vector<const tpcf_job *> jj, sj;
+    std::vector<char *> bb, sb;
     for (int j = 0; j < njobs; ++j) {
         if (int e = stage_job(b->jobs[j], b->plans[j], base + b->off[j], b->st)) return e;
         if (!b->plans[j].L.nblk) continue;
-        pp.push_back(&b->plans[j]);
-        jj.push_back(&b->jobs[j]);
-        bb.push_back(base + b->off[j]);
+        const bool r123 = b->plans[j].param == TPCF_R123;
+        (r123 ? pp : sp).push_back(&b->plans[j]);
+        (r123 ? jj : sj).push_back(&b->jobs[j]);
+        (r123 ? bb : sb).push_back(base + b->off[j]);
     }
     Queue Q;
     build_queue(pp, jj, bb, Q);
     if (int e = upload_queue(b->queue, Q, b->st, b->qd)) return e;
+    if (int e = sqt_plan(ctx, sp, sj, sb, b->mode, b->sqt)) return e;
+    if (int e = sqt_upload(b->sqt, b->sqt_queue, b->st)) return e;
     HIP_TRY(hipStreamSynchronize(b->st));
     b->shape = triplet_shape(ctx, b->qd.max_n);
     if (flags & TPCF_BATCH_AUTOTUNE)
@@ -1319,8 +1351,9 @@ const char *tpcf_batch_info(const tpcf_batch *b) {
     if (!b) return "";
     static thread_local std::string info;
     char head[160];
-    std::snprintf(head, sizeof head, "shape waves=%d copies=%d wg=%d graph=%d items=%d\n", b->shape.waves,
-                  b->shape.copies, b->shape.grid, b->exec ? 1 : 0, b->qd.nitems);
+    std::snprintf(head, sizeof head, "shape waves=%d copies=%d wg=%d graph=%d items=%d sqt_items=%d\n",
+                  b->shape.waves, b->shape.copies, b->shape.grid, b->exec ? 1 : 0, b->qd.nitems,
+                  (int)b->sqt.items.size());
     info = head + b->tuning;
     return info.c_str();
 }

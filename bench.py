@@ -96,8 +96,12 @@ def bench_batch(args) -> int:
     Mpc/h) of 150k clustered data + 150k randoms with lognormal occupancy
     500..8000; all four terms of every voxel, 20 log bins on [1,50) Mpc/h;
     FAST path as one prepared batch (autotuned shape, HIP graph) against the
-    FP64 plain kernel running the voxel jobs one at a time."""
+    FP64 plain kernel running the voxel jobs one at a time.  Also: the culled
+    form (CELLS) prepared and by count_batch, and the same voxels in the
+    (s,q,theta) parametrization as a prepared batch against plain FP64."""
     vox = synth.angular_voxels(8, 150000, seed=0)
+    sys.stderr.write("batch: 64 voxels of 150k + 150k\n")
+    sys.stderr.flush()
     spec = tp.r123_spec(20, 1.0, 50.0, "log", 1e-5)
     reps = max(1, args.steps)
     with tp.Context(0) as ctx:
@@ -148,6 +152,33 @@ def bench_batch(args) -> int:
             out, _ = ctx.count(job, spec, tp.MODE_PLAIN)
             plain_lat.append(time.perf_counter() - t1)
             exact &= bool(np.array_equal(out, fast[j]))
+        sys.stderr.write("batch: r123 prepared, culled and plain done\n")
+        sys.stderr.flush()
+        # The same voxels in the triangle parametrization (what the
+        # reference's voxel drivers run): s [2,20) 10 log, q [1,3) 8,
+        # theta/pi 16, as one prepared graph-replayed batch against the FP64
+        # plain kernel one voxel job at a time.
+        sspec = tp.binspec(tp.SQT, [tp.axis(10, 2.0, 20.0, "log"), tp.axis(8, 1.0, 3.0, "lin"),
+                                    tp.axis(16, 0.0, 1.0, "lin")])
+        bs = ctx.batch(jobs, [sspec] * len(jobs), tp.MODE_FAST, autotune=False, graph=True)
+        bs.run()
+        slat = []
+        for _ in range(reps):
+            t1 = time.perf_counter()
+            souts, sst = bs.run()
+            slat.append(time.perf_counter() - t1)
+        sfast = [o.copy() for o in souts]
+        bs.close()
+        splain_s = 0.0
+        sexact = True
+        for j, job in enumerate(jobs):
+            t1 = time.perf_counter()
+            out, _ = ctx.count(job, sspec, tp.MODE_PLAIN)
+            splain_s += time.perf_counter() - t1
+            sexact &= bool(np.array_equal(out, sfast[j]))
+            if j % 16 == 15:
+                sys.stderr.write(f"batch: sqt plain {j + 1}/{len(jobs)} voxels, {splain_s:.1f} s\n")
+                sys.stderr.flush()
     nv = len(jobs)
     mean = sum(lat) / len(lat)
     fast_rate = nv / mean
@@ -192,6 +223,14 @@ def bench_batch(args) -> int:
             "cells_prepared_latency_p50_ms": pct(clat, 50) * 1e3,
             "cells_prepared_latency_p99_ms": pct(clat, 99) * 1e3,
             "cells_prepared_identical": cells_prepared_exact,
+            "sqt_spec": "s [2,20) 10 log, q [1,3) 8 lin, theta/pi [0,1) 16 lin",
+            "sqt_prepared_voxel_jobs_per_s": nv / (sum(slat) / len(slat)),
+            "sqt_prepared_latency_p50_ms": pct(slat, 50) * 1e3,
+            "sqt_prepared_latency_p99_ms": pct(slat, 99) * 1e3,
+            "sqt_triplets_rechecked": sst.pairs_rechecked,
+            "sqt_plain_fp64_voxel_jobs_per_s": nv / splain_s,
+            "sqt_speedup_vs_plain": (nv / (sum(slat) / len(slat))) / (nv / splain_s),
+            "sqt_histograms_identical_to_plain": sexact,
         },
         "roofline": {
             "bound": "valu",

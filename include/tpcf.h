@@ -190,18 +190,21 @@ int tpcf_count(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int
  * the per-voxel process fan-out of examples/build_voxel_submission_scripts.py
  * :9-41 and examples/run_CPU_calculation_each_voxels.csh:60-67.  Jobs of any
  * parametrization (one persistent launch per family); mode must be
- * TPCF_MODE_FAST or TPCF_MODE_FAST64. */
+ * TPCF_MODE_FAST, TPCF_MODE_FAST64 or TPCF_MODE_CELLS. */
 int tpcf_count_batch(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs,
                      int mode, uint64_t *const *hist, tpcf_stats *stats);
 
 /* Prepared batch: the jobs are planned and staged on the device once; each
- * tpcf_batch_run() recomputes every histogram (histogram reset, pair stage,
- * one persistent triplet launch) -- replayed from a HIP graph with
- * TPCF_BATCH_GRAPH, with the triplet kernel's launch shape (waves per
- * workgroup, LDS histogram copies, workgroups per CU) picked by timing the
- * candidates on the batch itself with TPCF_BATCH_AUTOTUNE.  The catalogs must
- * stay alive until tpcf_batch_destroy().  No reference counterpart (the
- * reference runs one process per voxel and term). */
+ * tpcf_batch_run() recomputes every histogram (histogram reset, pair stages,
+ * one persistent launch for the (s,q,theta) and JBJ jobs and one for the
+ * (r1,r2,r3) jobs) -- replayed from a HIP graph with TPCF_BATCH_GRAPH, with
+ * the (r1,r2,r3) kernel's launch shape (waves per workgroup, LDS histogram
+ * copies, workgroups per CU) picked by timing the candidates on the batch
+ * itself with TPCF_BATCH_AUTOTUNE.  Modes FAST, FAST64 and CELLS (culling
+ * staged once at creation).  The catalogs must stay alive until
+ * tpcf_batch_destroy().  No reference counterpart (the reference runs one
+ * process per voxel and term, examples/run_CPU_calculation_each_voxels.csh,
+ * with the triangle-parametrization program). */
 typedef struct tpcf_batch tpcf_batch;
 enum { TPCF_BATCH_AUTOTUNE = 1, TPCF_BATCH_GRAPH = 2 };
 int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode,

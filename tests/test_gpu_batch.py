@@ -157,6 +157,45 @@ def test_count_batch_with_jbj_and_sqt(ctx):
         assert np.array_equal(o, w)
 
 
+def test_prepared_batch_mixed_parametrizations(ctx):
+    """A prepared batch of (r1,r2,r3), (s,q,theta) and JBJ jobs: staged once,
+    replayed from a graph (and eagerly), every replay equal to the oracle
+    with eager work in between; FAST, FAST64 and CELLS."""
+    print("case prepared batch mixed r123/sqt/jbj", flush=True)
+    rng = np.random.Generator(np.random.PCG64(21))
+    jaxes = [(10, 1.0, 10.0, "log"), (5, 0.0, 1.0, "lin"), (6, 0.0, 1.2, "lin")]
+    jobs, specs, wants, keep = [], [], [], []
+    for k in range(9):
+        n = int(rng.integers(40, 150))
+        d = synth.neyman_scott(n, nclust=6, sigma=1.5, size=25.0, seed=1200 + k)
+        r = synth.uniform_cube(n, 25.0, seed=1300 + k)
+        cd, cr = ctx.catalog(d), ctx.catalog(r)
+        keep += [cd, cr]
+        terms = [t for t in tp.FOUR_TERMS if rng.random() < 0.7] or [tp.FOUR_TERMS[0]]
+        jobs.append(tp.make_job([(cd, 0, n), (cr, 0, n)], terms))
+        if k % 3 == 0:
+            spec, oaxes = _sqt_spec(rng)
+            wants.append(orc.clean_count([d, r], terms, orc.SQT, oaxes))
+        elif k % 3 == 1:
+            spec = tp.binspec(tp.JBJ, [tp.axis(*a) for a in jaxes])
+            wants.append(orc.clean_count([d, r], terms, orc.JBJ, [orc.axis(*a) for a in jaxes]))
+        else:
+            spec = tp.r123_spec(int(rng.integers(8, 25)), 0.5, 40.0, "log")
+            wants.append(orc.clean_count([d, r], terms, orc.R123, [orc.axis(spec.ax[0].n, 0.5, 40.0, "log")]))
+        specs.append(spec)
+    for mode in (tp.MODE_FAST, tp.MODE_FAST64, tp.MODE_CELLS):
+        for graph in (True, False):
+            b = ctx.batch(jobs, specs, mode, autotune=graph, graph=graph)
+            assert "sqt_items=0" not in b.info
+            for rep in range(3):
+                outs, st = b.run()
+                assert st.triplets > 0
+                for k, (o, w) in enumerate(zip(outs, wants)):
+                    assert np.array_equal(o, w), (mode, graph, rep, k)
+                ctx.count(jobs[0], specs[0])
+            b.close()
+
+
 def test_serving_mix_with_sqt_alone_equals_mix(ctx):
     """(s,q,theta) and (r1,r2,r3) requests arriving together are batched by
     parametrization; each result equals the job run alone and the oracle."""
