@@ -110,28 +110,29 @@ __device__ inline void entry_jbj_fp64(double s1, double s2, double s3, const dou
 constexpr float kU = 5.9604644775390625e-8f;  // 2^-24, FP32 unit roundoff
 
 // Bin hint on an axis: lin (v - lo) * inv, log (log2(v) - lo) * inv; -1 below,
-// n at or above the range (NaN gives -1).
+// n at or above the range (NaN gives -1).  Branch-free: fmaxf returns -1 for
+// a NaN g, the clamp keeps floor(g) in [-1, n].
 __device__ inline int guess_bin(float v, float lo, float inv, int lg, int n) {
-    const float g = (lg ? __log2f(v) - lo : v - lo) * inv;
-    if (!(g >= 0.f)) return -1;
-    if (!(g < (float)n)) return n;
-    return (int)g;
+    const float g = ((lg ? __builtin_amdgcn_logf(v) : v) - lo) * inv;
+    return (int)__builtin_floorf(fminf(fmaxf(g, -1.f), (float)n));
 }
 
 // arccos(x)/pi for the theta hint, after Abramowitz & Stegun 4.4.46
 // (Hastings): arccos(x) = sqrt(1-x) * (a0 + a1 x + ... + a7 x^7) on [0,1],
 // absolute error below 2e-8; arccos(-x) = pi - arccos(x).
+// Only a hint (confirmed against the edge table), so fused multiply-adds and
+// the bare v_sqrt_f32 are fine.
 __device__ inline float acos_over_pi(float c) {
     const float x = fminf(fabsf(c), 1.f);
     float p = -0.0012624911f;
-    p = p * x + 0.0066700901f;
-    p = p * x - 0.0170881256f;
-    p = p * x + 0.0308918810f;
-    p = p * x - 0.0501743046f;
-    p = p * x + 0.0889789874f;
-    p = p * x - 0.2145988016f;
-    p = p * x + 1.5707963050f;
-    const float a = sqrtf(1.f - x) * p * 0.318309886f;
+    p = __builtin_fmaf(p, x, 0.0066700901f);
+    p = __builtin_fmaf(p, x, -0.0170881256f);
+    p = __builtin_fmaf(p, x, 0.0308918810f);
+    p = __builtin_fmaf(p, x, -0.0501743046f);
+    p = __builtin_fmaf(p, x, 0.0889789874f);
+    p = __builtin_fmaf(p, x, -0.2145988016f);
+    p = __builtin_fmaf(p, x, 1.5707963050f);
+    const float a = __builtin_amdgcn_sqrtf(1.f - x) * p * 0.318309886f;
     return c < 0.f ? 1.f - a : a;
 }
 
@@ -139,33 +140,25 @@ __device__ inline float acos_over_pi(float c) {
 // [-1, n]: returns m (or -1 out of range) and sets *unsure unless v is more
 // than its error d (plus the edge's own FP32 rounding) away from every edge
 // that decides it.  A wrong hint always fails the test.
+// Branch-free: the hint m is in [-1, n]; the lower edge e[m] decides unless
+// m < 0, the upper edge e[m+1] unless m >= n (the tests of the three cases
+// m < 0, m >= n and in range, unified).
 __device__ inline int bracket_asc(float v, float d, const float *e, int n, int m, bool *unsure) {
-    if (m < 0) {
-        if (!(e[0] - v > d + kU * fabsf(e[0]))) *unsure = true;
-        return -1;
-    }
-    if (m >= n) {
-        if (!(v - e[n] > d + kU * fabsf(e[n]))) *unsure = true;
-        return -1;
-    }
-    const float a = e[m], b = e[m + 1];
-    if (!(v - a > d + kU * fabsf(a)) || !(b - v > d + kU * fabsf(b))) *unsure = true;
-    return m;
+    const float a = e[m > 0 ? m : 0], b = e[m < n ? m + 1 : n];
+    const bool lo_ok = m < 0 || v - a > d + kU * fabsf(a);
+    const bool hi_ok = m >= n || b - v > d + kU * fabsf(b);
+    *unsure = *unsure || !(lo_ok && hi_ok);
+    return m >= 0 && m < n ? m : -1;
 }
 
 // Same for the descending cosine table (bin m iff C[m+1] < c <= C[m]); d
 // includes the table's rounding.
 __device__ inline int bracket_desc(float c, float d, const float *C, int n, int m, bool *unsure) {
-    if (m < 0) {
-        if (!(c - C[0] > d)) *unsure = true;
-        return -1;
-    }
-    if (m >= n) {
-        if (!(C[n] - c > d)) *unsure = true;
-        return -1;
-    }
-    if (!(C[m] - c > d) || !(c - C[m + 1] > d)) *unsure = true;
-    return m;
+    const float a = C[m > 0 ? m : 0], b = C[m < n ? m + 1 : n];
+    const bool up_ok = m < 0 || a - c > d;
+    const bool lo_ok = m >= n || c - b > d;
+    *unsure = *unsure || !(up_ok && lo_ok);
+    return m >= 0 && m < n ? m : -1;
 }
 
 // One entry in FP32 from sides a <= b (rounded FP32 lengths of the FP64
