@@ -191,9 +191,24 @@ typedef const uint32_t __attribute__((address_space(4))) *scalar_u32_ptr;
 // against the LDS counter too).
 typedef const uint8_t __attribute__((address_space(1))) *global_u8_ptr;
 
-__device__ inline void lds_add1(uint32_t *hist, uint32_t byte_addr) {
-    __hip_atomic_fetch_add(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + byte_addr), 1u,
+__device__ inline void lds_add(uint32_t *hist, uint32_t byte_addr, uint32_t v) {
+    __hip_atomic_fetch_add(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + byte_addr), v,
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Column j of a fully valid row: lanes whose run of equal cells ends at j
+// (the carry out of doubling w, which holds the run ends of the remaining
+// columns from the top bit down) add the run's length to its cell.
+__device__ __forceinline__ void r123_column(uint32_t *hist, uint32_t &w, int j, int &last, uint32_t zs,
+                                            uint32_t ys, uint32_t x4) {
+    uint32_t w2;
+    const bool end = __builtin_add_overflow(w, w, &w2);
+    w = w2;
+    if (end) {
+        const uint32_t run = (uint32_t)(j - last);
+        last = j;
+        lds_add(hist, zs + ys + x4, run);
+    }
 }
 
 // Persistent: workgroups pull items (term, A, B, C-range) from a global
@@ -202,9 +217,18 @@ __device__ inline void lds_add1(uint32_t *hist, uint32_t byte_addr) {
 // LDS histogram copies.  Lane k owns point k of block C: z = bin(j,k) is
 // preloaded for the 64 columns j of block B, y = bin(i,k) is per lane and
 // fixed for a row i of block A, and x = bin(i,j) is uniform, read with scalar
-// loads.  A row whose 64 x are all valid takes the branch-free path (one
-// s_bfe, one v_add3, one ds_add per 64 triplets); a row with none valid is
-// skipped; mixed rows test x per column.
+// loads.  The blocks hold spatially ordered (Morton) points, so along a row a
+// lane's cell (x, y, z) often stays the same for several consecutive
+// columns: each lane adds one run of equal cells at a time (the run's length
+// at its last column).  A run ends where x changes along the row (one ballot
+// per row) or z changes along the lane's column (one mask per C block), so
+// the add is issued only by the lanes whose run ends there: fewer LDS atomics
+// and fewer lanes per atomic, hence fewer bank and same-address conflicts.
+// Every triplet is still counted one by one in its run.  A row with no valid
+// x is skipped; a row with some invalid x (sparse rows of culled or wide
+// geometries, where the run bookkeeping per column costs more than it saves)
+// adds one triplet per valid column, the invalid ones skipped by a scalar
+// test.
 template <int WAVES, int COPIES>
 __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletItem *__restrict__ items,
                                                                   int nitems, int *__restrict__ counter,
@@ -264,6 +288,11 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
             uint32_t y4n = gAC[lane], xvn = gAB[lane];
 #pragma unroll
             for (int j = 0; j < kBlock; ++j) Zs[j] = Zs[j] < 128u ? __umul24(Zs[j], sz) + copy_base : dump;
+            // Run ends along the lane's column: bit 63-j set where the z slot
+            // of column j differs from that of column j+1 (and at j = 63);
+            // computed by the first row of the C block that merges runs.
+            unsigned long long zend = 0ull;
+            bool have_zend = false;
             for (int i = 0; i < kBlock; ++i) {
                 const uint32_t y4 = y4n;
                 const uint32_t xv = xvn;
@@ -272,23 +301,54 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
                 xvn = gAB[inext * kBlock + lane];
                 const unsigned long long rowmask = __ballot(xv < 128u);
                 if (rowmask == 0ull) continue;
+                // Fully valid rows merge runs.  Their run ends are found
+                // here, in uniform control flow (every lane active), before
+                // the lanes with invalid y drop out.
+                const bool runs = rowmask == ~0ull;
+                unsigned long long xend = 0ull;
+                if (runs) {
+                    // Along the row: x of column j differs from x of column
+                    // j+1 (lane j+1's value by a wave shift left; lane 63
+                    // keeps its own), bit j of the ballot reversed to 63-j.
+                    const uint32_t xv1 = (uint32_t)__builtin_amdgcn_update_dpp((int)xv, (int)xv, 0x130, 0xf, 0xf,
+                                                                                false);  // wave_shl:1
+                    xend = __builtin_bitreverse64(__ballot(xv != xv1));
+                    if (!have_zend) {
+                        zend = 1ull;
+#pragma unroll
+                        for (int j = 0; j + 1 < kBlock; ++j)
+                            zend |= (unsigned long long)(Zs[j] != Zs[j + 1]) << (kBlock - 1 - j);
+                        have_zend = true;
+                    }
+                }
                 if (y4 >= 128u) continue;
                 const uint32_t Ys = __umul24(y4, sy);
                 // x(i, j) for the 64 columns: one scalar load of the row.
                 // (Reading lane j of xv instead would be wrong here: inside
                 // the y-divergent region the compiler may rebuild xv in a
                 // fresh register for the active lanes only.)
-                scalar_u32_ptr xrow = (scalar_u32_ptr)(tAB + i * kBlock);
+                const scalar_u32_ptr xrow = (scalar_u32_ptr)(tAB + i * kBlock);
                 uint32_t xw[16];
 #pragma unroll
                 for (int q = 0; q < 16; ++q) xw[q] = xrow[q];
-                if (rowmask == ~0ull) {
+                // The row's words are waited for here, once: the columns
+                // below may each be skipped, and a wait inside them would
+                // also wait for the wave's outstanding LDS adds (scalar loads
+                // and LDS share a counter).  Emits no instruction.
+                asm volatile("" ::"s"(xw[0]), "s"(xw[1]), "s"(xw[2]), "s"(xw[3]), "s"(xw[4]), "s"(xw[5]),
+                             "s"(xw[6]), "s"(xw[7]), "s"(xw[8]), "s"(xw[9]), "s"(xw[10]), "s"(xw[11]),
+                             "s"(xw[12]), "s"(xw[13]), "s"(xw[14]), "s"(xw[15]));
+                const unsigned long long ends = xend | zend;
+                uint32_t whi = (uint32_t)(ends >> 32), wlo = (uint32_t)ends;
+                int last = -1;
+                if (runs) {
 #pragma unroll
                     for (int q = 0; q < 16; ++q) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
+                            const int j = q * 4 + r;
                             const uint32_t x4 = (xw[q] >> (8 * r)) & 0xFFu;
-                            lds_add1(hist, Zs[q * 4 + r] + Ys + x4);
+                            r123_column(hist, j < 32 ? whi : wlo, j, last, Zs[j], Ys, x4);
                         }
                     }
                 } else {
@@ -297,7 +357,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const uint32_t x4 = (xw[q] >> (8 * r)) & 0xFFu;
-                            if (x4 < 128u) lds_add1(hist, Zs[q * 4 + r] + Ys + x4);
+                            if (x4 < 128u) lds_add(hist, Zs[q * 4 + r] + Ys + x4, 1u);
                         }
                     }
                 }

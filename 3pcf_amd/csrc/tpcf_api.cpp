@@ -373,9 +373,9 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     // Spatial order also helps the (s,q,theta) and JBJ kernels unculled: rows
     // and blocks of the masks are empty as a whole more often (10k + 10k,
     // four terms, s [2,20) in a 200 Mpc/h cube: 50.8 -> 42.8 ms).  The
-    // (r1,r2,r3) kernel keeps the catalog order: similar bins along a tile
-    // row would raise same-address LDS conflicts.
-    P.morton = P.cull || sqt_fast;
+    // (r1,r2,r3) triplet kernel adds runs of equal cells along a tile row,
+    // which spatial order makes long.
+    P.morton = mode != TPCF_MODE_PLAIN;
     if (P.morton) {
         // (r1,r2,r3): every side of a counted triplet is below T[n].
         // (s,q,theta), JBJ: no side of an entry's triangle exceeds far_s.
