@@ -198,7 +198,9 @@ __device__ inline void lds_add(uint32_t *hist, uint32_t byte_addr, uint32_t v) {
 
 // Column j of a fully valid row: lanes whose run of equal cells ends at j
 // (the carry out of doubling w, which holds the run ends of the remaining
-// columns from the top bit down) add the run's length to its cell.
+// columns from the top bit down) add the run's length to its cell.  Only
+// those lanes reach the LDS (an add from every lane, the others into their
+// dump slots, measured 10 % slower on the 50k + 50k workload).
 __device__ __forceinline__ void r123_column(uint32_t *hist, uint32_t &w, int j, int &last, uint32_t zs,
                                             uint32_t ys, uint32_t x4) {
     uint32_t w2;
