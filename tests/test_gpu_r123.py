@@ -48,6 +48,20 @@ def test_four_terms_log_bins(ctx, mode):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_long_runs_of_equal_cells(ctx, mode):
+    """Tight clusters with every pair in range: in Morton order a lane's cell
+    stays the same over many consecutive columns of a full row, so the
+    kernel adds long runs (and short ones where clusters meet)."""
+    print(f"case long runs mode={mode}", flush=True)
+    d = synth.neyman_scott(600, nclust=6, sigma=0.3, size=20.0, seed=21)
+    r = synth.uniform_cube(520, 20.0, seed=22)
+    got, st = run(ctx, [d, r], tp.FOUR_TERMS, 12, 0.05, 40.0, "log", mode=mode)
+    want = oracle_r123([d, r], tp.FOUR_TERMS, 12, 0.05, 40.0, "log")
+    assert np.array_equal(got, want)
+    assert int(got.sum()) > 0.999 * st.triplets  # nearly every triplet in range: full rows
+
+
 def test_dense_all_in_range_totals(ctx):
     """Bins cover every pair: each term's total equals its closed form."""
     print("case dense_totals", flush=True)
