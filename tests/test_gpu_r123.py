@@ -62,6 +62,21 @@ def test_long_runs_of_equal_cells(ctx, mode):
     assert int(got.sum()) > 0.999 * st.triplets  # nearly every triplet in range: full rows
 
 
+def test_headline_geometry_fast_equals_plain(ctx):
+    """The 50k + 50k workload's geometry and spec at 3k + 3k (clustered data
+    and randoms in a 200 Mpc/h cube, 20 log bins on [1,350): every pair in
+    range, full rows and long runs): FAST, FAST64 and CELLS equal the FP64
+    brute-force kernel, which the small cases pin to the oracle."""
+    print("case headline geometry 3k+3k", flush=True)
+    d = synth.neyman_scott(3000, nclust=200, sigma=2.0, size=200.0, seed=0)
+    r = synth.uniform_cube(3000, 200.0, seed=1)
+    plain, st = run(ctx, [d, r], tp.FOUR_TERMS, 20, 1.0, 350.0, "log", mode=tp.MODE_PLAIN)
+    assert int(plain.sum()) > 0.999 * st.triplets
+    for mode in (tp.MODE_FAST, tp.MODE_FAST64, tp.MODE_CELLS):
+        got, _ = run(ctx, [d, r], tp.FOUR_TERMS, 20, 1.0, 350.0, "log", mode=mode)
+        assert np.array_equal(got, plain), mode
+
+
 def test_dense_all_in_range_totals(ctx):
     """Bins cover every pair: each term's total equals its closed form."""
     print("case dense_totals", flush=True)
