@@ -162,6 +162,9 @@ static Reach block_reach(const std::vector<double> &boxes, int64_t nblk, double 
     return r;
 }
 
+// Longest C-range of one work item, in blocks.
+constexpr int64_t kMaxItemBlocks = 512;
+
 // Work items of one term: blocks A <= B <= C of the term's segments.  With
 // the blocks' reach lists (TPCF_MODE_CELLS), only the block triples whose
 // three pairwise box gaps are below the cut (no triplet with a squared side
@@ -171,7 +174,6 @@ static void term_items(const Layout &L, const int *tr, int term_slot, std::vecto
     const int64_t a0 = L.seg_blk0[tr[0]], a1 = a0 + L.seg_nblk[tr[0]];
     const int64_t b0 = L.seg_blk0[tr[1]], b1 = b0 + L.seg_nblk[tr[1]];
     const int64_t cb0 = L.seg_blk0[tr[2]], cb1 = cb0 + L.seg_nblk[tr[2]];
-    constexpr int64_t kMaxItemBlocks = 512;
     if (reach) {
         const int64_t W = reach->words;
         // Blocks A of segment a split over host threads, items gathered per
@@ -236,6 +238,27 @@ static void term_items(const Layout &L, const int *tr, int term_slot, std::vecto
                 out.push_back(it);
             }
         }
+}
+
+// Items [begin, end) of v ordered by C-range length, longest first (the
+// persistent kernels take them in that order, so the long ones do not
+// finish last), ties in their order: a stable counting sort on the length,
+// which is at most kMaxItemBlocks.
+static void sort_items_longest_first(std::vector<TripletItem> &v, size_t begin) {
+    constexpr int kKeys = (int)kMaxItemBlocks + 1;
+    const size_t n = v.size() - begin;
+    if (n < 2) return;
+    std::vector<size_t> start(kKeys + 1, 0);
+    for (size_t i = begin; i < v.size(); ++i) ++start[kKeys - (v[i].c1 - v[i].c0)];
+    size_t acc = 0;
+    for (int k = 0; k <= kKeys; ++k) {
+        const size_t c = start[k];
+        start[k] = acc;
+        acc += c;
+    }
+    std::vector<TripletItem> tmp(n);
+    for (size_t i = begin; i < v.size(); ++i) tmp[start[kKeys - (v[i].c1 - v[i].c0)]++] = v[i];
+    std::copy(tmp.begin(), tmp.end(), v.begin() + (std::ptrdiff_t)begin);
 }
 
 // Everything one job needs on the device, laid out in one region.
@@ -543,9 +566,7 @@ static void build_queue(const std::vector<const JobPlan *> &plans, const std::ve
             T.n = P.n;
             const size_t before = Q.items.size();
             term_items(P.L, jobs[j]->term[t], (int)Q.terms.size(), Q.items, P.cull ? &P.reach : nullptr);
-            std::stable_sort(Q.items.begin() + before, Q.items.end(), [](const TripletItem &a, const TripletItem &b) {
-                return (a.c1 - a.c0) > (b.c1 - b.c0);
-            });
+            sort_items_longest_first(Q.items, before);
             Q.terms.push_back(T);
             Q.max_n = std::max(Q.max_n, P.n);
         }
@@ -715,8 +736,7 @@ static int sqt_plan(tpcf_ctx *ctx, const std::vector<const JobPlan *> &plans,
             T.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist) + (size_t)t * P.cells;
             const size_t before = L.items.size();
             term_items(P.L, jobs[j]->term[t], (int)L.terms.size(), L.items, P.cull ? &P.reach : nullptr);
-            std::stable_sort(L.items.begin() + before, L.items.end(),
-                             [](const TripletItem &a, const TripletItem &b) { return (a.c1 - a.c0) > (b.c1 - b.c0); });
+            sort_items_longest_first(L.items, before);
             L.terms.push_back(T);
         }
     }
