@@ -684,6 +684,25 @@ struct SqtLaunch {
     size_t o_terms = 0, o_ctr = 0;
 };
 
+// The pair stage of one staged (s,q,theta) or JBJ job.
+static SqtPairJob sqt_pair_job(const JobPlan &P, char *base) {
+    SqtPairJob J{};
+    J.u64 = reinterpret_cast<const double *>(base + P.o_u64);
+    J.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
+    J.codes = reinterpret_cast<uint8_t *>(base + P.o_tiles);
+    J.dists = reinterpret_cast<float *>(base + P.o_dists);
+    J.masks = reinterpret_cast<unsigned long long *>(base + P.o_masks);
+    J.Ts = reinterpret_cast<const double *>(base + P.o_T64);
+    J.ns = P.n;
+    J.far_s = P.far_s;
+    J.mid_s = P.mid_s;
+    J.npad = P.L.npad;
+    J.nblk = P.L.nblk;
+    J.need = P.cull ? reinterpret_cast<const unsigned long long *>(base + P.o_need) : nullptr;
+    J.need_words = (P.L.nblk + 63) / 64;
+    return J;
+}
+
 static int sqt_plan(tpcf_ctx *ctx, const std::vector<const JobPlan *> &plans,
                     const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases, int mode,
                     SqtLaunch &L) {
@@ -692,20 +711,7 @@ static int sqt_plan(tpcf_ctx *ctx, const std::vector<const JobPlan *> &plans,
         const JobPlan &P = *plans[j];
         char *base = bases[j];
         if (!P.L.nblk) continue;
-        SqtPairJob J{};
-        J.u64 = reinterpret_cast<const double *>(base + P.o_u64);
-        J.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
-        J.codes = reinterpret_cast<uint8_t *>(base + P.o_tiles);
-        J.dists = reinterpret_cast<float *>(base + P.o_dists);
-        J.masks = reinterpret_cast<unsigned long long *>(base + P.o_masks);
-        J.Ts = reinterpret_cast<const double *>(base + P.o_T64);
-        J.ns = P.n;
-        J.far_s = P.far_s;
-        J.mid_s = P.mid_s;
-        J.npad = P.L.npad;
-        J.nblk = P.L.nblk;
-        J.need = P.cull ? reinterpret_cast<const unsigned long long *>(base + P.o_need) : nullptr;
-        J.need_words = (P.L.nblk + 63) / 64;
+        const SqtPairJob J = sqt_pair_job(P, base);
         L.pairs.push_back(J);
         SqtTerm T{};
         T.codes = J.codes;
@@ -788,12 +794,14 @@ static int sqt_upload(const SqtLaunch &L, DevBuf &buf, hipStream_t st) {
     return TPCF_OK;
 }
 
-// The kernels of L: pair stages, then the counter reset and the persistent
-// candidate launch.  When given, ev_pairs is recorded after the pair stages
-// and ev_kernel before the candidate launch (stage timings).
+// The kernels of L: pair stages (unless already enqueued), then the counter
+// reset and the persistent candidate launch.  When given, ev_pairs is
+// recorded after the pair stages and ev_kernel before the candidate launch
+// (stage timings).
 static int sqt_enqueue(tpcf_ctx *ctx, const SqtLaunch &L, const DevBuf &buf, hipStream_t st, hipEvent_t ev_pairs,
-                       hipEvent_t ev_kernel) {
-    for (const SqtPairJob &J : L.pairs) HIP_TRY(launch_sqt_pairs(J, st));
+                       hipEvent_t ev_kernel, bool pairs = true) {
+    if (pairs)
+        for (const SqtPairJob &J : L.pairs) HIP_TRY(launch_sqt_pairs(J, st));
     if (ev_pairs) HIP_TRY(hipEventRecord(ev_pairs, st));
     if (ev_kernel) HIP_TRY(hipEventRecord(ev_kernel, st));
     if (L.items.empty()) return TPCF_OK;
@@ -813,10 +821,15 @@ static int sqt_enqueue(tpcf_ctx *ctx, const SqtLaunch &L, const DevBuf &buf, hip
 static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<const JobPlan *> &plans,
                                 const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases, int mode,
                                 hipStream_t st) {
+    // The pair stages first: the device runs them while the host plans the
+    // items.
+    for (size_t j = 0; j < plans.size(); ++j)
+        if (plans[j]->L.nblk) HIP_TRY(launch_sqt_pairs(sqt_pair_job(*plans[j], bases[j]), st));
+    HIP_TRY(hipEventRecord(ws.ev[1], st));
     SqtLaunch L;
     if (int e = sqt_plan(ctx, plans, jobs, bases, mode, L)) return e;
     if (int e = sqt_upload(L, ws.sqt_queue, st)) return e;
-    return sqt_enqueue(ctx, L, ws.sqt_queue, st, ws.ev[1], ws.ev[3]);
+    return sqt_enqueue(ctx, L, ws.sqt_queue, st, nullptr, ws.ev[3], false);
 }
 
 // Pair-stage and triplet-kernel times from the workspace events.
