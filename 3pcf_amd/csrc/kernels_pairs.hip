@@ -8,7 +8,9 @@
 // axis, for any runtime axis of up to kMaxPairBins bins; (a, a) terms count
 // i < j (no self pairs), (a, b) terms the full product.
 //
-// FAST (pair_hist_kernel<.., CULL = false>): persistent workgroups pull items -- an i block of
+// FAST (pair_hist_kernel<.., CULL = false>): every pair's squared distance
+// in FP64 over Morton-ordered copies of the segments (the order does not
+// change the counts); persistent workgroups pull items -- an i block of
 // WAVES*64*IPL points of segment a against a j range of segment b -- from an
 // atomic counter.  Each lane keeps IPL i points in registers; the j point is
 // the same for the whole wave and comes through scalar loads.  The FP32 sqrt
@@ -64,25 +66,32 @@ __device__ __noinline__ int bin_walk(double s, int k, const double *D, int n) {
 }
 
 // Pairs of the lane's IPL points with U consecutive j points (j .. j+U-1):
-// all squared distances and guesses first, then all table reads, so the LDS
-// latency is paid once per batch.
+// all squared distances first; if none of the wave's pairs is below the last
+// squared threshold Tn (out of range by the FP64 definition itself) the
+// batch ends there; otherwise all guesses, then all table reads, so the LDS
+// latency is paid once per batch.  (Segments are in Morton order, so a
+// wave's batch is mostly all in or all out of range.)
 template <int IPL, int U, bool LOG, bool DIAG>
 __device__ __forceinline__ void pair_batch(const double (&xi)[IPL], const double (&yi)[IPL],
                                            const double (&zi)[IPL], const int (&jmin)[IPL], int j,
                                            const double (&X)[U], const double (&Y)[U], const double (&Z)[U],
-                                           int n, float mul, float add, const double *D, uint32_t *hk) {
+                                           int n, float mul, float add, const double *D, double Tn, uint32_t *hk) {
     constexpr int M = IPL * U;
     const float top = (float)(n + 1);
     double s[M];
     int k[M];
+    bool near = false;
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int q = 0; q < IPL; ++q) {
             const double dx = xi[q] - X[u], dy = yi[q] - Y[u], dz = zi[q] - Z[u];
             s[u * IPL + q] = (dx * dx + dy * dy) + dz * dz;
-            k[u * IPL + q] = bin_guess<LOG>((float)s[u * IPL + q], mul, add, top);
+            near = near || s[u * IPL + q] < Tn;
         }
+    if (__ballot(near) == 0ull) return;
+#pragma unroll
+    for (int m = 0; m < M; ++m) k[m] = bin_guess<LOG>((float)s[m], mul, add, top);
     double lo[M], hi[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) {
@@ -133,6 +142,7 @@ __device__ __forceinline__ void pair_item(CPairTerm *T, int i0, int j0, int j1, 
     }
     const double *bx = T->bx, *by = T->by, *bz = T->bz;
     const float mul = T->h_mul, add = T->h_add;
+    const double Tn = D[n + 1];  // the last squared threshold
     // The pairs of the lane's points with j in [ja, jb).
     auto run = [&](int ja, int jb) {
         int j = ja;
@@ -145,11 +155,11 @@ __device__ __forceinline__ void pair_item(CPairTerm *T, int i0, int j0, int j1, 
                 Y[u] = ld_uniform(by, j + u);
                 Z[u] = ld_uniform(bz, j + u);
             }
-            pair_batch<IPL, kJBatch, LOG, DIAG>(xi, yi, zi, jmin, j, X, Y, Z, n, mul, add, D, hk);
+            pair_batch<IPL, kJBatch, LOG, DIAG>(xi, yi, zi, jmin, j, X, Y, Z, n, mul, add, D, Tn, hk);
         }
         for (; j < jb; ++j) {
             const double X[1] = {ld_uniform(bx, j)}, Y[1] = {ld_uniform(by, j)}, Z[1] = {ld_uniform(bz, j)};
-            pair_batch<IPL, 1, LOG, DIAG>(xi, yi, zi, jmin, j, X, Y, Z, n, mul, add, D, hk);
+            pair_batch<IPL, 1, LOG, DIAG>(xi, yi, zi, jmin, j, X, Y, Z, n, mul, add, D, Tn, hk);
         }
     };
     if (!CULL) {
@@ -178,7 +188,6 @@ __device__ __forceinline__ void pair_item(CPairTerm *T, int i0, int j0, int j1, 
             hi[a] = fmax(hi[a], __shfl_xor(hi[a], m));
         }
     if (!(lo[0] <= hi[0])) return;  // no point of segment a in this wave
-    const double Tn = D[n + 1];
     for (int j = j0; j < j1;) {
         const int c = j / kPairBoxPoints;
         const int je = min(j1, (c + 1) * kPairBoxPoints);

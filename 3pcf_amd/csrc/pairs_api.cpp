@@ -296,11 +296,12 @@ static int run_pairs(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_axis *axes,
         if (cells) chunk = std::min(chunk, 2048);
     }
 
-    // Segments as the kernel reads them: the catalogs, or their
-    // Morton-ordered copies with chunk boxes.
+    // Segments as the kernel reads them: Morton-ordered copies with chunk
+    // boxes (FAST and CELLS: a wave's pairs are then mostly all in or all out
+    // of range), or the catalogs (PLAIN).
     std::vector<std::array<SegView, 3>> views(njobs);
     std::vector<std::vector<double>> boxes;
-    if (cells) {
+    if (!plain) {
         if (int e = sort_segments(ctx, jobs, njobs, views, boxes)) return e;
     } else {
         for (int j = 0; j < njobs; ++j)
