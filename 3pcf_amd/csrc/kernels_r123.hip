@@ -81,6 +81,7 @@ __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
     const int jb = (t & 3) * 16;
     const int64_t gi = R * kBlock + i;
     uint32_t packed[4] = {0u, 0u, 0u, 0u};
+    bool any = false;
     unsigned long long nrecheck = 0;
     const float u = 5.9604644775390625e-8f;  // 2^-24
     const float base = 2.002f * u * J.rmax;
@@ -127,10 +128,16 @@ __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
             }
         }
         packed[q >> 2] |= (b & 0xFFu) << (8 * (q & 3));
+        any |= b != kSentinel;
     }
     uint4 *dst = reinterpret_cast<uint4 *>(J.tiles + tile_index(R, C, J.nblk) * kTileBytes + i * kBlock + jb);
     *dst = make_uint4(packed[0], packed[1], packed[2], packed[3]);
     if (F32 && nrecheck) atomicAdd(J.rechecked, nrecheck);
+    // Whether any pair of the tile has a bin: a triplet with a side in this
+    // tile counts only if that side does, so the triplet stage skips the
+    // block pairs (and C blocks) whose tiles have none.
+    const int tile_has = __syncthreads_or(any ? 1 : 0);
+    if (t == 0 && J.tile_any) J.tile_any[tile_index(R, C, J.nblk)] = tile_has ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -273,12 +280,18 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
         }
         since_flush += add;
         const int64_t nblk = T.nblk;
+        // No triplet of the item counts when its (A, B) tile has no pair
+        // with a bin.
+        const uint8_t *tany = T.tile_any;
+        if (tany && !tany[tile_index(A, B, nblk)]) continue;
         const uint32_t sy = (uint32_t)r123_sy(n), sz = (uint32_t)r123_sz(n);
         const uint32_t copy_base = (uint32_t)((lane % COPIES) * r123_copy_stride(n, COPIES)) * 4u;
         const uint32_t dump = copy_base + ((uint32_t)n * sz + (uint32_t)lane) * 4u;
         const uint8_t *tAB = T.tiles + tile_index(A, B, nblk) * kTileBytes;
         const global_u8_ptr gAB = (global_u8_ptr)tAB;
         for (int C = c0 + wave; C < c1; C += WAVES) {
+            // Likewise a C block whose (A, C) or (B, C) tile has none.
+            if (tany && !(tany[tile_index(A, C, nblk)] & tany[tile_index(B, C, nblk)])) continue;
             const global_u8_ptr gAC = (global_u8_ptr)(T.tiles + tile_index(A, C, nblk) * kTileBytes);
             const global_u8_ptr gBC = (global_u8_ptr)(T.tiles + tile_index(B, C, nblk) * kTileBytes);
             // z of the 64 columns as byte offsets of the z slab in the lane's

@@ -280,7 +280,7 @@ struct JobPlan {
     uint64_t triplets = 0;
     // byte offsets inside the job's device region
     size_t o_u64 = 0, o_u32 = 0, o_blk = 0, o_T64 = 0, o_T32 = 0, o_Eq = 0, o_Et = 0, o_cnt = 0, o_hist = 0,
-           o_tiles = 0, o_dists = 0, o_masks = 0;
+           o_tiles = 0, o_any = 0, o_dists = 0, o_masks = 0;
     size_t bytes = 0;
     bool need_tiles = true;
     double far_s = 0;  // SQT FAST: squared sides above this cannot be in any entry
@@ -367,6 +367,8 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     P.o_cnt = o; o = align256(o + 64);
     P.o_hist = o; o = align256(o + sizeof(uint64_t) * P.cells * P.nterms);
     P.o_tiles = o; o = align256(o + (P.need_tiles ? (size_t)tile_count(P.L.nblk) * kTileBytes : 0));
+    const bool r123_fast = spec.param == TPCF_R123 && mode != TPCF_MODE_PLAIN;
+    P.o_any = o; o = align256(o + (r123_fast ? (size_t)tile_count(P.L.nblk) : 0));
     const bool sqt_fast = spec.param != TPCF_R123 && mode != TPCF_MODE_PLAIN;
     P.o_dists = o; o = align256(o + (sqt_fast ? (size_t)tile_count(P.L.nblk) * kTileBytes * sizeof(float) : 0));
     P.o_masks = o;
@@ -476,6 +478,7 @@ static PairJob pair_job(const JobPlan &P, char *base, int shift) {
     J.u32 = reinterpret_cast<const float4 *>(base + P.o_u32);
     J.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
     J.tiles = reinterpret_cast<uint8_t *>(base + P.o_tiles);
+    J.tile_any = reinterpret_cast<uint8_t *>(base + P.o_any);
     J.T64 = reinterpret_cast<const double *>(base + P.o_T64);
     J.T32 = reinterpret_cast<const float *>(base + P.o_T32);
     J.rechecked = reinterpret_cast<unsigned long long *>(base + P.o_cnt);
@@ -561,6 +564,7 @@ static void build_queue(const std::vector<const JobPlan *> &plans, const std::ve
         for (int t = 0; t < P.nterms; ++t) {
             TripletTerm T{};
             T.tiles = reinterpret_cast<const uint8_t *>(bases[j] + P.o_tiles);
+            T.tile_any = reinterpret_cast<const uint8_t *>(bases[j] + P.o_any);
             T.nblk = P.L.nblk;
             T.hist = reinterpret_cast<unsigned long long *>(bases[j] + P.o_hist) + (size_t)t * P.cells;
             T.n = P.n;

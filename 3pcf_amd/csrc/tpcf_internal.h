@@ -23,6 +23,7 @@ __host__ __device__ inline int64_t tile_count(int64_t nblk) { return nblk * (nbl
 // One (job, term) of the triplet stage.
 struct TripletTerm {
     const uint8_t *tiles;  // pair-bin tiles of the job
+    const uint8_t *tile_any;  // per tile: 1 if some pair of it has a bin (nullptr: not kept)
     int64_t nblk;          // blocks in the job's union layout
     unsigned long long *hist;  // nterm histogram, n^3 uint64 (sorted-side cells)
     int n;                 // bins per side
@@ -42,6 +43,7 @@ struct PairJob {
     const float4 *u32;     // (x-o, y-o, z-o, 0) in FP32
     const uint8_t *blk_count;  // real points in each block (0..64)
     uint8_t *tiles;
+    uint8_t *tile_any;     // per tile: 1 if some pair of it has a bin, else 0 (nullptr: none kept)
     const double *T64;     // squared thresholds [0..n]
     const float *T32;      // T64 rounded to float
     unsigned long long *rechecked;  // FP64 recomputations
