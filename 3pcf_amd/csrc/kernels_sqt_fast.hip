@@ -480,6 +480,9 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
             xmid[i] = mAB[kMaskRowMid + i];
         }
         __syncthreads();
+        // No candidate in the item when its (A, B) tile has no near pair
+        // (every wave sees the same masks, so all take this together).
+        if (__ballot(xnear[lane] != 0ull) == 0ull) continue;
         // This wave's ring of queued candidates: entries [qhead, qtail)
         // (indices mod qcap); each packs i | j << 6 | k << 12 | (C - c0) << 18.
         uint32_t qhead = 0u, qtail = 0u;
