@@ -1,16 +1,20 @@
 // kernels_r123.hip -- gfx950 kernels of the sorted-side (r1,r2,r3) path.
 //
 //   pair_tiles_kernel     pair stage: every pair of a job's union layout gets
-//                         a u8 bin (or the sentinel) in 64x64 tiles.  FP32
-//                         squared distances against squared edges, with every
-//                         pair whose FP32 value lies within a rigorous error
-//                         bound (or the requested relative window) of an edge
-//                         recomputed in FP64; or FP64 throughout.
+//                         a u8 bin (or the sentinel) in 64x64 tiles, and every
+//                         tile a byte saying whether any pair of it has a bin.
+//                         FP32 squared distances against squared edges, with
+//                         every pair whose FP32 value lies within a rigorous
+//                         error bound (or the requested relative window) of an
+//                         edge recomputed in FP64; or FP64 throughout.
 //   r123_triplet_kernel   persistent triplet stage over a work queue of
-//                         (term, A, B, C-range) block triples: per triplet one
-//                         v_add3 and one LDS atomic into a privatized
+//                         (term, A, B, C-range) block triples of Morton-ordered
+//                         points: runs of equal cells along a tile row added
+//                         as one LDS atomic each (or one atomic per triplet on
+//                         rows with out-of-range pairs) into a privatized
 //                         unsorted-cube histogram, folded into sorted uint64
-//                         cells on flush.
+//                         cells on flush; block pairs and C blocks whose tiles
+//                         have no pair in range are skipped.
 //   r123_plain_kernel     the FP64 baseline: all three distances of every
 //                         triplet recomputed in FP64 and binned, like the
 //                         reference's distance() (src/3pcf_C_version.cc:54-142).
