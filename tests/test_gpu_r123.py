@@ -77,6 +77,21 @@ def test_headline_geometry_fast_equals_plain(ctx):
         assert np.array_equal(got, plain), mode
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_far_apart_groups_skip_empty_tiles(ctx, mode):
+    """Two groups of points farther apart than the last edge: every tile
+    between them has no pair in range, so the kernel skips those items and C
+    blocks; the histograms still equal the oracle."""
+    print(f"case far-apart groups mode={mode}", flush=True)
+    g1 = synth.uniform_cube(300, 15.0, seed=31)
+    g2 = synth.uniform_cube(260, 15.0, seed=32) + np.array([100.0, 0.0, 0.0])
+    d = np.vstack([g1[:170], g2[:150]])
+    r = np.vstack([g1[170:], g2[150:]])
+    got, _ = run(ctx, [d, r], tp.FOUR_TERMS, 10, 0.5, 30.0, "log", mode=mode)
+    want = oracle_r123([d, r], tp.FOUR_TERMS, 10, 0.5, 30.0, "log")
+    assert np.array_equal(got, want)
+
+
 def test_dense_all_in_range_totals(ctx):
     """Bins cover every pair: each term's total equals its closed form."""
     print("case dense_totals", flush=True)
