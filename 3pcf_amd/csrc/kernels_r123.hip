@@ -65,7 +65,11 @@ template <bool F32>
 __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
     const int64_t R = blockIdx.y, C = blockIdx.x;
     if (C < R) return;
-    if (J.need && !((J.need[R * J.need_words + (C >> 6)] >> (C & 63)) & 1ull)) return;
+    if (J.need && !((J.need[R * J.need_words + (C >> 6)] >> (C & 63)) & 1ull)) {
+        // Not computed: marked empty, so work items may span it.
+        if (threadIdx.x == 0 && J.tile_any) J.tile_any[tile_index(R, C, J.nblk)] = 0;
+        return;
+    }
     __shared__ float4 sR[kBlock], sC[kBlock];
     __shared__ double sT64[kMaxBins + 1];
     __shared__ float sT32[kMaxBins + 1];
@@ -455,8 +459,13 @@ __global__ __launch_bounds__(256) void r123_triplet_wide_kernel(const TripletIte
         const TripletItem I = items[it];
         const TripletTerm T = terms[I.term];
         const int n = T.n;
+        // Tiles with no pair in range (or not computed: culled items may
+        // span such blocks) are skipped, as in r123_triplet_kernel.
+        const uint8_t *tany = T.tile_any;
+        if (tany && !tany[tile_index(I.A, I.B, T.nblk)]) continue;
         const uint8_t *tAB = T.tiles + tile_index(I.A, I.B, T.nblk) * kTileBytes;
         for (int C = I.c0 + wave; C < I.c1; C += 4) {
+            if (tany && !(tany[tile_index(I.A, C, T.nblk)] & tany[tile_index(I.B, C, T.nblk)])) continue;
             const uint8_t *tAC = T.tiles + tile_index(I.A, C, T.nblk) * kTileBytes;
             const uint8_t *tBC = T.tiles + tile_index(I.B, C, T.nblk) * kTileBytes;
             for (int i = 0; i < kBlock; ++i) {

@@ -164,13 +164,16 @@ static Reach block_reach(const std::vector<double> &boxes, int64_t nblk, double 
 
 // Longest C-range of one work item, in blocks.
 constexpr int64_t kMaxItemBlocks = 512;
+// Culled (r1,r2,r3) items: runs of reachable C blocks at most this far apart
+// share one item (the tiles of the blocks between them are marked empty).
+constexpr int64_t kMergeGapBlocks = 32;
 
 // Work items of one term: blocks A <= B <= C of the term's segments.  With
 // the blocks' reach lists (TPCF_MODE_CELLS), only the block triples whose
 // three pairwise box gaps are below the cut (no triplet with a squared side
 // at or above it can count).
 static void term_items(const Layout &L, const int *tr, int term_slot, std::vector<TripletItem> &out,
-                       const Reach *reach = nullptr) {
+                       const Reach *reach = nullptr, int64_t merge_gap = 0) {
     const int64_t a0 = L.seg_blk0[tr[0]], a1 = a0 + L.seg_nblk[tr[0]];
     const int64_t b0 = L.seg_blk0[tr[1]], b1 = b0 + L.seg_nblk[tr[1]];
     const int64_t cb0 = L.seg_blk0[tr[2]], cb1 = cb0 + L.seg_nblk[tr[2]];
@@ -195,6 +198,21 @@ static void term_items(const Layout &L, const int *tr, int term_slot, std::vecto
                         if (w == (cb1 - 1) >> 6 && (cb1 & 63)) m &= ~0ull >> (64 - (cb1 & 63));
                         both[w] = m;
                     }
+                    // Runs closer than merge_gap blocks go into one item (the
+                    // kernel skips the blocks between them by their tiles'
+                    // empty flags).
+                    int64_t p0 = -1, p1 = -1;
+                    auto emit = [&](int64_t r0, int64_t r1) {
+                        for (int64_t q = r0; q < r1; q += kMaxItemBlocks) {
+                            TripletItem it{};
+                            it.term = term_slot;
+                            it.A = (int)A;
+                            it.B = (int)B;
+                            it.c0 = (int)q;
+                            it.c1 = (int)std::min<int64_t>(r1, q + kMaxItemBlocks);
+                            mine.push_back(it);
+                        }
+                    };
                     int64_t c = cs;
                     while (c < cb1) {
                         // next set bit at or after c
@@ -208,17 +226,16 @@ static void term_items(const Layout &L, const int *tr, int term_slot, std::vecto
                         uint64_t z = ~both[ew] & (~0ull << (e & 63));
                         while (!z && ++ew <= (cb1 - 1) >> 6) z = ~both[ew];
                         const int64_t r1 = std::min<int64_t>(cb1, z ? ew * 64 + __builtin_ctzll(z) : cb1);
-                        for (int64_t q = r0; q < r1; q += kMaxItemBlocks) {
-                            TripletItem it{};
-                            it.term = term_slot;
-                            it.A = (int)A;
-                            it.B = (int)B;
-                            it.c0 = (int)q;
-                            it.c1 = (int)std::min<int64_t>(r1, q + kMaxItemBlocks);
-                            mine.push_back(it);
+                        if (p1 >= 0 && r0 - p1 <= merge_gap) {
+                            p1 = r1;
+                        } else {
+                            if (p1 >= 0) emit(p0, p1);
+                            p0 = r0;
+                            p1 = r1;
                         }
                         c = r1;
                     }
+                    if (p1 >= 0) emit(p0, p1);
                 }
             }
         });
@@ -569,7 +586,8 @@ static void build_queue(const std::vector<const JobPlan *> &plans, const std::ve
             T.hist = reinterpret_cast<unsigned long long *>(bases[j] + P.o_hist) + (size_t)t * P.cells;
             T.n = P.n;
             const size_t before = Q.items.size();
-            term_items(P.L, jobs[j]->term[t], (int)Q.terms.size(), Q.items, P.cull ? &P.reach : nullptr);
+            term_items(P.L, jobs[j]->term[t], (int)Q.terms.size(), Q.items, P.cull ? &P.reach : nullptr,
+                       kMergeGapBlocks);
             sort_items_longest_first(Q.items, before);
             Q.terms.push_back(T);
             Q.max_n = std::max(Q.max_n, P.n);
