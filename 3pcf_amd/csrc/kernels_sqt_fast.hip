@@ -208,7 +208,11 @@ __device__ inline void sqt_flush64(uint32_t *h, int cells, unsigned long long *g
 __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
     const int64_t R = blockIdx.y, C = blockIdx.x;
     if (C < R) return;
-    if (J.need && !((J.need[R * J.need_words + (C >> 6)] >> (C & 63)) & 1ull)) return;
+    if (J.need && !((J.need[R * J.need_words + (C >> 6)] >> (C & 63)) & 1ull)) {
+        // Not computed: marked empty, so work items may span it.
+        if (threadIdx.x == 0) J.tile_any[tile_index(R, C, J.nblk)] = 0;
+        return;
+    }
     __shared__ double sT[kMaxBins + 1];
     __shared__ unsigned long long colN[4][kBlock], colS[4][kBlock], colM[4][kBlock];
     const int t = threadIdx.x, j = t & 63, w = t >> 6;
@@ -254,6 +258,10 @@ __global__ __launch_bounds__(256) void sqt_pairs_kernel(SqtPairJob J) {
     colS[w][j] = cs;
     colM[w][j] = cm;
     __syncthreads();
+    // Whether any pair of the tile is near (no candidate has a side in a
+    // tile without one).
+    const int tile_near = __syncthreads_or(cn != 0ull ? 1 : 0);
+    if (t == 0) J.tile_any[tile_index(R, C, J.nblk)] = tile_near ? 1 : 0;
     if (t < kBlock) {
         masks[kMaskColNear + t] = (colN[0][t] | colN[1][t]) | (colN[2][t] | colN[3][t]);
         masks[kMaskColShort + t] = (colS[0][t] | colS[1][t]) | (colS[2][t] | colS[3][t]);
@@ -473,6 +481,9 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         const int64_t tAB = tile_index(X.A, X.B, nblk) * kTileBytes;
         X.cAB = T.codes + tAB;
         X.dAB = T.dists + tAB;
+        // No candidate in the item when its (A, B) tile has no near pair
+        // (or was not computed).
+        if (!T.tile_any[tile_index(X.A, X.B, nblk)]) continue;
         const unsigned long long *mAB = T.masks + tile_index(X.A, X.B, nblk) * kMaskWords;
         for (int i = threadIdx.x; i < kBlock; i += blockDim.x) {
             xnear[i] = mAB[kMaskRowNear + i];
@@ -480,9 +491,6 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
             xmid[i] = mAB[kMaskRowMid + i];
         }
         __syncthreads();
-        // No candidate in the item when its (A, B) tile has no near pair
-        // (every wave sees the same masks, so all take this together).
-        if (__ballot(xnear[lane] != 0ull) == 0ull) continue;
         // This wave's ring of queued candidates: entries [qhead, qtail)
         // (indices mod qcap); each packs i | j << 6 | k << 12 | (C - c0) << 18.
         uint32_t qhead = 0u, qtail = 0u;
@@ -502,6 +510,7 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         for (int C = X.c0 + wave; C < c1; C += WAVES) {
             // Lane k's column of tiles (B,C) and (A,C): bits j of side (j,k)
             // and bits i of side (i,k).
+            if (!(T.tile_any[tile_index(X.A, C, nblk)] & T.tile_any[tile_index(X.B, C, nblk)])) continue;
             const unsigned long long *mBC = T.masks + tile_index(X.B, C, nblk) * kMaskWords;
             const unsigned long long zn = mBC[kMaskColNear + lane], zs = mBC[kMaskColShort + lane],
                                      zm = mBC[kMaskColMid + lane];

@@ -164,8 +164,9 @@ static Reach block_reach(const std::vector<double> &boxes, int64_t nblk, double 
 
 // Longest C-range of one work item, in blocks.
 constexpr int64_t kMaxItemBlocks = 512;
-// Culled (r1,r2,r3) items: runs of reachable C blocks at most this far apart
-// share one item (the tiles of the blocks between them are marked empty).
+// Culled items: runs of reachable C blocks at most this far apart share one
+// item (the pair stages mark the tiles they skip empty, and the triplet
+// kernels skip blocks with an empty tile).
 constexpr int64_t kMergeGapBlocks = 32;
 
 // Work items of one term: blocks A <= B <= C of the term's segments.  With
@@ -384,8 +385,7 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     P.o_cnt = o; o = align256(o + 64);
     P.o_hist = o; o = align256(o + sizeof(uint64_t) * P.cells * P.nterms);
     P.o_tiles = o; o = align256(o + (P.need_tiles ? (size_t)tile_count(P.L.nblk) * kTileBytes : 0));
-    const bool r123_fast = spec.param == TPCF_R123 && mode != TPCF_MODE_PLAIN;
-    P.o_any = o; o = align256(o + (r123_fast ? (size_t)tile_count(P.L.nblk) : 0));
+    P.o_any = o; o = align256(o + (mode != TPCF_MODE_PLAIN ? (size_t)tile_count(P.L.nblk) : 0));
     const bool sqt_fast = spec.param != TPCF_R123 && mode != TPCF_MODE_PLAIN;
     P.o_dists = o; o = align256(o + (sqt_fast ? (size_t)tile_count(P.L.nblk) * kTileBytes * sizeof(float) : 0));
     P.o_masks = o;
@@ -714,6 +714,7 @@ static SqtPairJob sqt_pair_job(const JobPlan &P, char *base) {
     J.codes = reinterpret_cast<uint8_t *>(base + P.o_tiles);
     J.dists = reinterpret_cast<float *>(base + P.o_dists);
     J.masks = reinterpret_cast<unsigned long long *>(base + P.o_masks);
+    J.tile_any = reinterpret_cast<uint8_t *>(base + P.o_any);
     J.Ts = reinterpret_cast<const double *>(base + P.o_T64);
     J.ns = P.n;
     J.far_s = P.far_s;
@@ -739,6 +740,7 @@ static int sqt_plan(tpcf_ctx *ctx, const std::vector<const JobPlan *> &plans,
         T.codes = J.codes;
         T.dists = J.dists;
         T.masks = J.masks;
+        T.tile_any = J.tile_any;
         T.u64 = J.u64;
         T.Ts = J.Ts;
         T.Eq = reinterpret_cast<const double *>(base + P.o_Eq);
@@ -763,7 +765,8 @@ static int sqt_plan(tpcf_ctx *ctx, const std::vector<const JobPlan *> &plans,
         for (int t = 0; t < P.nterms; ++t) {
             T.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist) + (size_t)t * P.cells;
             const size_t before = L.items.size();
-            term_items(P.L, jobs[j]->term[t], (int)L.terms.size(), L.items, P.cull ? &P.reach : nullptr);
+            term_items(P.L, jobs[j]->term[t], (int)L.terms.size(), L.items, P.cull ? &P.reach : nullptr,
+                       kMergeGapBlocks);
             sort_items_longest_first(L.items, before);
             L.terms.push_back(T);
         }

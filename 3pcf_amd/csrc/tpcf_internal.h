@@ -104,6 +104,7 @@ struct SqtPairJob {
     // 3 KiB per tile, kMaskWords words: bit j of row i near / with an s bin /
     // mid, bit i of column j likewise.
     unsigned long long *masks;
+    uint8_t *tile_any;  // per tile: 1 if some pair of it is near, else 0 (also for tiles not computed)
     const double *Ts;
     int ns;
     double far_s;    // squared sides above this are kCodeFar
@@ -124,6 +125,7 @@ struct SqtTerm {
     const uint8_t *codes;
     const float *dists;
     const unsigned long long *masks;
+    const uint8_t *tile_any;  // per tile: 1 if some pair of it is near
     const double *u64;
     const double *Ts, *Eq, *Et;
     unsigned long long *hist;       // this term's ns*nq*nt cells
