@@ -18,6 +18,9 @@
 //   --format tri|plain      output layout (default per program)
 //   --recheck-rel X         FP64 recheck window (default 1e-5)
 //   --device N, --quiet, --stats
+//   --all-voxels            with -X DIV: every chunk triple (i,j,k) of DIV in
+//                           one batched launch, one file per voxel named by
+//                           -o with "{index}" replaced by "%03d%03d%03d"
 // Which triplets are counted follows the catalogs given, compared as files
 // (realpath), not as strings: "D D R" counts C(N_D,2)*N_R triplets.
 #include <getopt.h>
@@ -118,6 +121,75 @@ bool write_two(const char *path, const int64_t ng[2], const tpcf_axis &ax, const
     std::fprintf(f, "%-6.3f %-6.3f %-6.3f\n", ax.lo, ax.hi, w);
     for (int b = 0; b < ax.n; ++b) std::fprintf(f, "%llu\n", (unsigned long long)h[b]);
     return std::fclose(f) == 0;
+}
+
+// Index ranges of the three roles: whole catalogs (div <= 0) or the
+// reference's voxel chunks vid[i] of div.  False when a chunk is empty.
+bool voxel_ranges(const Catalog cat[3], int div, const int vid[3], int64_t lo[3], int64_t hi[3]) {
+    for (int i = 0; i < 3; ++i) {
+        lo[i] = 0;
+        hi[i] = cat[i].n;
+        if (div > 0 && !vox_chunk(div, vid[i], cat[i].n, lo[i], hi[i])) return false;
+    }
+    return true;
+}
+
+// Roles of the same file must come in chunk order; otherwise the voxel
+// selects no triplet.
+bool selects_nothing(const Catalog cat[3], const int64_t lo[3]) {
+    for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b)
+            if (cat[a].real == cat[b].real && lo[a] > lo[b]) return true;
+    return false;
+}
+
+// One job: a segment per distinct (file, chunk), one term of the sorted
+// segment labels of the three roles.
+void voxel_job(const Catalog cat[3], tpcf_catalog *const dev[3], const int64_t lo[3], const int64_t hi[3],
+               tpcf_job &job) {
+    job = tpcf_job{};
+    int label[3] = {-1, -1, -1};
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < i; ++j)
+            if (cat[j].real == cat[i].real && lo[j] == lo[i]) {
+                label[i] = label[j];
+                break;
+            }
+        if (label[i] >= 0) continue;
+        label[i] = job.nseg;
+        job.seg[job.nseg] = dev[i];
+        job.seg_lo[job.nseg] = lo[i];
+        job.seg_hi[job.nseg] = hi[i];
+        job.nseg++;
+    }
+    int t3[3] = {label[0], label[1], label[2]};
+    std::sort(t3, t3 + 3);
+    job.nterms = 1;
+    std::memcpy(job.term[0], t3, sizeof t3);
+}
+
+// The program's histogram file: cube format, or TRI with the whole
+// catalogs' sizes in the header (as the reference writes for a voxel too).
+bool write_hist(const char *path, int format, const tpcf_binspec &spec, const Catalog cat[3], const uint64_t *h) {
+    if (format == 1) return write_plain(path, spec.ax[0].n, h);
+    const int64_t ng[3] = {cat[0].n, cat[1].n, cat[2].n};
+    int nb[3];
+    double alo[3], ahi[3], w[3];
+    for (int a = 0; a < 3; ++a) {
+        const tpcf_axis &ax = spec.param == TPCF_R123 ? spec.ax[0] : spec.ax[a];
+        nb[a] = ax.n;
+        alo[a] = ax.lo;
+        ahi[a] = ax.hi;
+        w[a] = ax.scale == TPCF_LOG ? std::log(ax.hi / ax.lo) / ax.n : (ax.hi - ax.lo) / ax.n;
+    }
+    return write_tri(path, ng, nb, alo, ahi, w, h);
+}
+
+void print_stats(const tpcf_stats &st) {
+    std::printf("{\"triplets\": %llu, \"pairs\": %llu, \"pairs_rechecked\": %llu, \"device_ms\": %.3f, "
+                "\"wall_ms\": %.3f}\n",
+                (unsigned long long)st.triplets, (unsigned long long)st.pairs,
+                (unsigned long long)st.pairs_rechecked, st.device_ms, st.wall_ms);
 }
 
 // 2pcf_C_version f0 f1 [-o out] [-l flag] [-X div -x AAABBBCCC] [-a]: the
@@ -262,7 +334,7 @@ int main(int argc, char **argv) {
     if (program == 3) return two_main(argc, argv);
     const bool triprog = program != 2;
     const char *outfile = "default_out.dat";
-    int vdiv = -999, vtemp = -999, device = 0, quiet = 0, want_stats = 0;
+    int vdiv = -999, vtemp = -999, device = 0, quiet = 0, want_stats = 0, all_voxels = 0;
     double scale = 1.0;
     tpcf_binspec spec{};
     spec.recheck_rel = 1e-5;
@@ -283,7 +355,7 @@ int main(int argc, char **argv) {
     double r_lo = 0, r_hi = 0;
     int r_n = 0, r_log = 0;
 
-    enum { O_PARAM = 1000, O_AXIS, O_BINS, O_LO, O_HI, O_LOG, O_MODE, O_FORMAT, O_RECHECK, O_DEVICE, O_QUIET, O_STATS };
+    enum { O_PARAM = 1000, O_AXIS, O_BINS, O_LO, O_HI, O_LOG, O_MODE, O_FORMAT, O_RECHECK, O_DEVICE, O_QUIET, O_STATS, O_ALLVOX };
     static option longopts[] = {{"param", required_argument, nullptr, O_PARAM},
                                 {"axis", required_argument, nullptr, O_AXIS},
                                 {"bins", required_argument, nullptr, O_BINS},
@@ -296,6 +368,7 @@ int main(int argc, char **argv) {
                                 {"device", required_argument, nullptr, O_DEVICE},
                                 {"quiet", no_argument, nullptr, O_QUIET},
                                 {"stats", no_argument, nullptr, O_STATS},
+                                {"all-voxels", no_argument, nullptr, O_ALLVOX},
                                 {nullptr, 0, nullptr, 0}};
     const char *shortopts = triprog ? "ao:l:x:X:" : "ao:L:l:w:sm";
     int c;
@@ -337,6 +410,7 @@ int main(int argc, char **argv) {
         case O_DEVICE: device = std::atoi(optarg); break;
         case O_QUIET: quiet = 1; break;
         case O_STATS: want_stats = 1; break;
+        case O_ALLVOX: all_voxels = 1; break;
         case '?': std::printf("unknown arg %c\n", optopt); break;  // the reference continues
         default: break;
         }
@@ -359,6 +433,10 @@ int main(int argc, char **argv) {
         return 2;
     }
     if (tpcf_binspec_check(&spec) != TPCF_OK) return die(2, "bad bin spec");
+    if (all_voxels && (vdiv <= 0 || vdiv > 999 || !std::strstr(outfile, "{index}"))) {
+        std::fprintf(stderr, "error: --all-voxels needs -X DIV (1..999) and an -o name containing {index}\n");
+        return 2;
+    }
 
     // Catalogs, deduplicated by file identity.
     Catalog cat[3];
@@ -385,92 +463,104 @@ int main(int argc, char **argv) {
         if (!quiet) std::printf("Opening input file %d: %s (%lld points)\n", i, cat[i].path.c_str(), (long long)cat[i].n);
     }
 
-    // Index ranges (whole catalogs, or the reference's voxel chunks).
-    int64_t lo[3], hi[3];
-    int vidx[3] = {0, 0, 0};
-    if (vdiv > 0) {
-        vidx[0] = vtemp / 1000000;
-        vidx[1] = (vtemp / 1000) % 1000;
-        vidx[2] = vtemp % 1000;
-    }
-    for (int i = 0; i < 3; ++i) {
-        lo[i] = 0;
-        hi[i] = cat[i].n;
-        if (vdiv > 0 && vtemp >= 0 && !vox_chunk(vdiv, vidx[i], cat[i].n, lo[i], hi[i])) {
-            std::printf("Voxel chunks are bigger than the number of galaxies!");
-            return 255;
-        }
-    }
-    // Roles of the same file must come in chunk order; otherwise the voxel
-    // selects no triplet.
-    bool empty = false;
-    for (int a = 0; a < 3; ++a)
-        for (int b = a + 1; b < 3; ++b)
-            if (cat[a].real == cat[b].real && lo[a] > lo[b]) empty = true;
-
+    // One device copy per distinct file.
     tpcf_ctx *ctx = nullptr;
     if (tpcf_ctx_create(device, &ctx) != TPCF_OK) return die(3, "no usable GPU");
     tpcf_catalog *dev[3] = {nullptr, nullptr, nullptr};
-    tpcf_job job{};
-    int label[3] = {-1, -1, -1};
     for (int i = 0; i < 3; ++i) {
         for (int j = 0; j < i; ++j)
-            if (cat[j].real == cat[i].real && lo[j] == lo[i]) {
-                label[i] = label[j];
-                break;
-            }
-        if (label[i] >= 0) continue;
-        int same_cat = -1;
-        for (int j = 0; j < i; ++j)
-            if (cat[j].real == cat[i].real) same_cat = j;
-        if (same_cat >= 0) dev[i] = dev[same_cat];
-        else if (tpcf_catalog_create(ctx, cat[i].xyz, cat[i].n, &dev[i]) != TPCF_OK)
+            if (cat[j].real == cat[i].real) dev[i] = dev[j];
+        if (!dev[i] && tpcf_catalog_create(ctx, cat[i].xyz, cat[i].n, &dev[i]) != TPCF_OK)
             return die(3, "catalog upload failed");
-        label[i] = job.nseg;
-        job.seg[job.nseg] = dev[i];
-        job.seg_lo[job.nseg] = lo[i];
-        job.seg_hi[job.nseg] = hi[i];
-        job.nseg++;
     }
-    int t3[3] = {label[0], label[1], label[2]};
-    std::sort(t3, t3 + 3);
-    job.nterms = 1;
-    std::memcpy(job.term[0], t3, sizeof t3);
-
     const int64_t cells = tpcf_hist_cells(&spec);
-    std::vector<uint64_t> h(cells, 0);
-    tpcf_stats st{};
-    if (!empty && tpcf_count(ctx, &job, &spec, mode, h.data(), &st) != TPCF_OK) return die(4, "counting failed");
 
-    uint64_t total = 0;
-    for (auto v : h) total += v;
-    bool ok;
-    if (format == 1) {
-        ok = write_plain(outfile, spec.ax[0].n, h.data());
-    } else {
-        const int64_t ng[3] = {cat[0].n, cat[1].n, cat[2].n};
-        int nb[3];
-        double alo[3], ahi[3], w[3];
-        for (int a = 0; a < 3; ++a) {
-            const tpcf_axis &ax = spec.param == TPCF_R123 ? spec.ax[0] : spec.ax[a];
-            nb[a] = ax.n;
-            alo[a] = ax.lo;
-            ahi[a] = ax.hi;
-            w[a] = ax.scale == TPCF_LOG ? std::log(ax.hi / ax.lo) / ax.n : (ax.hi - ax.lo) / ax.n;
+    if (all_voxels) {
+        // Every chunk triple (i,j,k) of -X div, as the per-voxel driver loop
+        // (examples/run_CPU_calculation_by_voxels.csh:61-85) runs them, in one
+        // batch; one file per voxel, named from the -o template.
+        const std::string tmpl = outfile;
+        const size_t at = tmpl.find("{index}");
+        const int nvox = vdiv * vdiv * vdiv;
+        std::vector<tpcf_job> jobs;
+        std::vector<int> job_of(nvox, -1);
+        for (int v = 0; v < nvox; ++v) {
+            const int vid[3] = {v / (vdiv * vdiv), (v / vdiv) % vdiv, v % vdiv};
+            int64_t lo[3], hi[3];
+            if (!voxel_ranges(cat, vdiv, vid, lo, hi)) {
+                std::printf("Voxel chunks are bigger than the number of galaxies!");
+                return 255;
+            }
+            if (selects_nothing(cat, lo)) continue;
+            job_of[v] = (int)jobs.size();
+            jobs.emplace_back();
+            voxel_job(cat, dev, lo, hi, jobs.back());
         }
-        ok = write_tri(outfile, ng, nb, alo, ahi, w, h.data());
+        std::vector<uint64_t> h((size_t)jobs.size() * cells, 0);
+        std::vector<uint64_t *> hp(jobs.size());
+        for (size_t j = 0; j < jobs.size(); ++j) hp[j] = h.data() + j * cells;
+        const std::vector<tpcf_binspec> specs(jobs.size(), spec);
+        tpcf_stats st{};
+        if (!jobs.empty()) {
+            int rc = TPCF_OK;
+            if (mode == TPCF_MODE_PLAIN) {
+                for (size_t j = 0; j < jobs.size() && rc == TPCF_OK; ++j) {
+                    tpcf_stats sj{};
+                    rc = tpcf_count(ctx, &jobs[j], &spec, mode, hp[j], &sj);
+                    st.triplets += sj.triplets;
+                    st.pairs += sj.pairs;
+                    st.pairs_rechecked += sj.pairs_rechecked;
+                    st.device_ms += sj.device_ms;
+                    st.wall_ms += sj.wall_ms;
+                }
+            } else {
+                rc = tpcf_count_batch(ctx, jobs.data(), specs.data(), (int)jobs.size(), mode, hp.data(), &st);
+            }
+            if (rc != TPCF_OK) return die(4, "counting failed");
+        }
+        const std::vector<uint64_t> zero(cells, 0);
+        for (int v = 0; v < nvox; ++v) {
+            char index[16];
+            std::snprintf(index, sizeof index, "%03d%03d%03d", v / (vdiv * vdiv), (v / vdiv) % vdiv, v % vdiv);
+            const std::string path = tmpl.substr(0, at) + index + tmpl.substr(at + 7);
+            const uint64_t *hv = job_of[v] >= 0 ? hp[job_of[v]] : zero.data();
+            if (!write_hist(path.c_str(), format, spec, cat, hv)) return die(1, "cannot write the histogram file");
+            if (!quiet) {
+                uint64_t total = 0;
+                for (int64_t c = 0; c < cells; ++c) total += hv[c];
+                std::printf("Output filename is %s\nTotal: %llu\n", path.c_str(), (unsigned long long)total);
+            }
+        }
+        if (!quiet) std::printf("numcalc: %llu\n", (unsigned long long)st.triplets);
+        if (want_stats) print_stats(st);
+    } else {
+        int vid[3] = {0, 0, 0};
+        if (vdiv > 0) {
+            vid[0] = vtemp / 1000000;
+            vid[1] = (vtemp / 1000) % 1000;
+            vid[2] = vtemp % 1000;
+        }
+        int64_t lo[3], hi[3];
+        if (!voxel_ranges(cat, vdiv > 0 && vtemp >= 0 ? vdiv : 0, vid, lo, hi)) {
+            std::printf("Voxel chunks are bigger than the number of galaxies!");
+            return 255;
+        }
+        tpcf_job job{};
+        voxel_job(cat, dev, lo, hi, job);
+        std::vector<uint64_t> h(cells, 0);
+        tpcf_stats st{};
+        if (!selects_nothing(cat, lo) && tpcf_count(ctx, &job, &spec, mode, h.data(), &st) != TPCF_OK)
+            return die(4, "counting failed");
+        uint64_t total = 0;
+        for (auto v : h) total += v;
+        if (!write_hist(outfile, format, spec, cat, h.data())) return die(1, "cannot write the histogram file");
+        if (!quiet) {
+            std::printf("Output filename is %s\n", outfile);
+            std::printf("Total: %llu\n", (unsigned long long)total);
+            std::printf("numcalc: %llu\n", (unsigned long long)st.triplets);
+        }
+        if (want_stats) print_stats(st);
     }
-    if (!ok) return die(1, "cannot write the histogram file");
-    if (!quiet) {
-        std::printf("Output filename is %s\n", outfile);
-        std::printf("Total: %llu\n", (unsigned long long)total);
-        std::printf("numcalc: %llu\n", (unsigned long long)st.triplets);
-    }
-    if (want_stats)
-        std::printf("{\"triplets\": %llu, \"pairs\": %llu, \"pairs_rechecked\": %llu, \"device_ms\": %.3f, "
-                    "\"wall_ms\": %.3f}\n",
-                    (unsigned long long)st.triplets, (unsigned long long)st.pairs,
-                    (unsigned long long)st.pairs_rechecked, st.device_ms, st.wall_ms);
     for (int i = 0; i < 3; ++i) {
         bool first = true;
         for (int j = 0; j < i; ++j)

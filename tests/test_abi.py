@@ -129,6 +129,11 @@ def test_cli_argument_errors(prog, tmp_path):
     assert r.returncode == 2
     r = subprocess.run([exe, f, f, str(tmp_path / "missing.dat")], capture_output=True, text=True)
     assert r.returncode == 1 and "cannot open" in r.stderr
+    # --all-voxels needs -X and a file-name template with {index}; checked
+    # before any device is touched.
+    for extra in (["-o", str(tmp_path / "v.dat"), "-X", "2"], ["-o", str(tmp_path / "v_{index}.dat")]):
+        r = subprocess.run([exe, f, f, f, "--all-voxels", *extra], capture_output=True, text=True)
+        assert r.returncode == 2 and "--all-voxels" in r.stderr
 
 
 def test_radecz_conversion_reproduces_reference_catalogs():

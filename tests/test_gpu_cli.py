@@ -98,6 +98,46 @@ def test_voxel_driver_plus_reducer_equals_full_run(tmp_path):
     assert summed == full.read_bytes() + b"\n"
 
 
+def test_batched_voxel_driver_writes_the_same_files(tmp_path):
+    """BATCH=1 run_CPU_calculation_by_voxels.sh (one --all-voxels run per
+    term, every voxel in one batched launch) writes byte for byte the files of
+    the per-voxel loop, including the voxels whose same-file chunks come out
+    of order and select nothing."""
+    env = dict(os.environ, INPUT0=os.path.join(INP, "d200.dat"), INPUT1=os.path.join(INP, "r180.dat"),
+               TAG="t", VOX_DIV="2", GLOBAL_PARAMS="--quiet --axis 0,10,1,12,log --axis 1,8,1,3 --axis 2,8,0,1")
+    drv = os.path.join(DRV, "run_CPU_calculation_by_voxels.sh")
+    loop, batch = tmp_path / "loop", tmp_path / "batch"
+    loop.mkdir()
+    batch.mkdir()
+    for part in ("0", "1"):
+        print("driver part", part, flush=True)
+        subprocess.run(["bash", drv, "1", part], cwd=loop, env=env, check=True, capture_output=True)
+        subprocess.run(["bash", drv, "1", part], cwd=batch, env=dict(env, BATCH="1"), check=True,
+                       capture_output=True)
+    names = sorted(os.listdir(loop))
+    assert len(names) == 16 and names == sorted(os.listdir(batch))
+    zero = 0
+    for n in names:
+        a = (loop / n).read_bytes()
+        assert a == (batch / n).read_bytes(), n
+        zero += all(v == b"0" for v in a.split(b"\n", 7)[7].split())
+    assert 0 < zero < 16  # some DDD voxels select nothing, the rest count
+
+
+@pytest.mark.parametrize("mode", ["fast", "plain", "cells"])
+def test_r123_all_voxels(mode, tmp_path):
+    """--param r123 --all-voxels (the (r1,r2,r3) batch path): each voxel file
+    equals its own -x run."""
+    files = ["d200.dat", "d200.dat", "r180.dat"]
+    args = [prog("3pcf_C_version_triangle_parametrization"), *files, "--param", "r123", "--axis", "0,12,1,30,log",
+            "--mode", mode, "-X", "2", "--quiet"]
+    run([*args, "--all-voxels", "-o", str(tmp_path / "v_{index}.dat")])
+    for idx in ("000000000", "000001001", "001000001", "001001000", "001001001"):
+        one = tmp_path / "one.dat"
+        run([*args, "-x", idx, "-o", str(one)])
+        assert one.read_bytes() == (tmp_path / f"v_{idx}.dat").read_bytes(), idx
+
+
 @pytest.mark.parametrize("program,mode", [("3pcf_C_version_triangle_parametrization", "clean-sqt"),
                                           ("3pcf_C_version_triangle_parametrization_JBJ", "clean-jbj")])
 def test_triangle_programs_cells_mode(program, mode, tmp_path):
