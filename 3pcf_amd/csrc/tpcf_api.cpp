@@ -127,6 +127,8 @@ static double block_gap2(const double *a, const double *b) {
 struct Reach {
     int64_t words = 0;
     std::vector<uint64_t> bits;
+    // First and last word of each row with a bit set (first > last: none).
+    std::vector<int32_t> first, last;
     const uint64_t *row(int64_t X) const { return bits.data() + X * words; }
     bool has(int64_t X, int64_t Y) const { return (row(X)[Y >> 6] >> (Y & 63)) & 1u; }
 };
@@ -150,13 +152,44 @@ static Reach block_reach(const std::vector<double> &boxes, int64_t nblk, double 
     Reach r;
     r.words = (nblk + 63) / 64;
     r.bits.assign((size_t)(nblk * r.words), 0ull);
+    r.first.assign((size_t)nblk, 0);
+    r.last.assign((size_t)nblk, -1);
     const double *bx = boxes.data();
+    // The union box of the blocks of each word: its gap to a box is at most
+    // the gap to any of them (FP64 subtraction, squares and sums are
+    // monotone), so a word whose union is out of reach has no bit set.
+    std::vector<double> wbox((size_t)6 * r.words);
+    for (int64_t w = 0; w < r.words; ++w) {
+        double *u = wbox.data() + 6 * w;
+        for (int c = 0; c < 3; ++c) {
+            u[c] = HUGE_VAL;
+            u[3 + c] = -HUGE_VAL;
+        }
+        for (int64_t Y = w * 64; Y < std::min<int64_t>(nblk, w * 64 + 64); ++Y)
+            for (int c = 0; c < 3; ++c) {
+                u[c] = std::min(u[c], bx[6 * Y + c]);
+                u[3 + c] = std::max(u[3 + c], bx[6 * Y + 3 + c]);
+            }
+    }
     // Each thread fills whole rows X (every Y), so no word is shared.
     parallel_ranges(nblk, [&](int64_t x0, int64_t x1, int) {
         for (int64_t X = x0; X < x1; ++X) {
             uint64_t *row = r.bits.data() + (size_t)(X * r.words);
-            for (int64_t Y = 0; Y < nblk; ++Y)
-                if (block_gap2(bx + 6 * X, bx + 6 * Y) < cut2) row[Y >> 6] |= 1ull << (Y & 63);
+            for (int64_t w = 0; w < r.words; ++w) {
+                if (!(block_gap2(bx + 6 * X, wbox.data() + 6 * w) < cut2)) continue;
+                uint64_t m = 0;
+                for (int64_t Y = w * 64; Y < std::min<int64_t>(nblk, w * 64 + 64); ++Y)
+                    m |= (uint64_t)(block_gap2(bx + 6 * X, bx + 6 * Y) < cut2) << (Y & 63);
+                row[w] = m;
+            }
+            int32_t f = (int32_t)r.words, l = -1;
+            for (int64_t w = 0; w < r.words; ++w)
+                if (row[w]) {
+                    f = std::min<int32_t>(f, (int32_t)w);
+                    l = (int32_t)w;
+                }
+            r.first[X] = f;
+            r.last[X] = l;
         }
     });
     return r;
@@ -168,95 +201,6 @@ constexpr int64_t kMaxItemBlocks = 512;
 // item (the pair stages mark the tiles they skip empty, and the triplet
 // kernels skip blocks with an empty tile).
 constexpr int64_t kMergeGapBlocks = 32;
-
-// Work items of one term: blocks A <= B <= C of the term's segments.  With
-// the blocks' reach lists (TPCF_MODE_CELLS), only the block triples whose
-// three pairwise box gaps are below the cut (no triplet with a squared side
-// at or above it can count).
-static void term_items(const Layout &L, const int *tr, int term_slot, std::vector<TripletItem> &out,
-                       const Reach *reach = nullptr, int64_t merge_gap = 0) {
-    const int64_t a0 = L.seg_blk0[tr[0]], a1 = a0 + L.seg_nblk[tr[0]];
-    const int64_t b0 = L.seg_blk0[tr[1]], b1 = b0 + L.seg_nblk[tr[1]];
-    const int64_t cb0 = L.seg_blk0[tr[2]], cb1 = cb0 + L.seg_nblk[tr[2]];
-    if (reach) {
-        const int64_t W = reach->words;
-        // Blocks A of segment a split over host threads, items gathered per
-        // thread and appended in A order.
-        std::vector<std::vector<TripletItem>> part(16);
-        parallel_ranges(a1 - a0, [&](int64_t lo, int64_t hi, int worker) {
-            std::vector<TripletItem> &mine = part[worker];
-            std::vector<uint64_t> both(W);
-            for (int64_t A = a0 + lo; A < a0 + hi; ++A) {
-                const uint64_t *ra = reach->row(A);
-                for (int64_t B = (tr[0] == tr[1] ? A : b0); B < b1; ++B) {
-                    if (!reach->has(A, B)) continue;
-                    const uint64_t *rb = reach->row(B);
-                    const int64_t cs = std::max<int64_t>((tr[1] == tr[2]) ? B : cb0, cb0);
-                    // Runs of consecutive set bits of ra & rb within [cs, cb1).
-                    for (int64_t w = cs >> 6; w <= (cb1 - 1) >> 6 && cs < cb1; ++w) {
-                        uint64_t m = ra[w] & rb[w];
-                        if (w == cs >> 6) m &= ~0ull << (cs & 63);
-                        if (w == (cb1 - 1) >> 6 && (cb1 & 63)) m &= ~0ull >> (64 - (cb1 & 63));
-                        both[w] = m;
-                    }
-                    // Runs closer than merge_gap blocks go into one item (the
-                    // kernel skips the blocks between them by their tiles'
-                    // empty flags).
-                    int64_t p0 = -1, p1 = -1;
-                    auto emit = [&](int64_t r0, int64_t r1) {
-                        for (int64_t q = r0; q < r1; q += kMaxItemBlocks) {
-                            TripletItem it{};
-                            it.term = term_slot;
-                            it.A = (int)A;
-                            it.B = (int)B;
-                            it.c0 = (int)q;
-                            it.c1 = (int)std::min<int64_t>(r1, q + kMaxItemBlocks);
-                            mine.push_back(it);
-                        }
-                    };
-                    int64_t c = cs;
-                    while (c < cb1) {
-                        // next set bit at or after c
-                        int64_t w = c >> 6;
-                        uint64_t m = both[w] & (~0ull << (c & 63));
-                        while (!m && ++w <= (cb1 - 1) >> 6) m = both[w];
-                        if (!m) break;
-                        const int64_t r0 = w * 64 + __builtin_ctzll(m);
-                        // end of the run: next clear bit after r0
-                        int64_t e = r0, ew = e >> 6;
-                        uint64_t z = ~both[ew] & (~0ull << (e & 63));
-                        while (!z && ++ew <= (cb1 - 1) >> 6) z = ~both[ew];
-                        const int64_t r1 = std::min<int64_t>(cb1, z ? ew * 64 + __builtin_ctzll(z) : cb1);
-                        if (p1 >= 0 && r0 - p1 <= merge_gap) {
-                            p1 = r1;
-                        } else {
-                            if (p1 >= 0) emit(p0, p1);
-                            p0 = r0;
-                            p1 = r1;
-                        }
-                        c = r1;
-                    }
-                    if (p1 >= 0) emit(p0, p1);
-                }
-            }
-        });
-        for (auto &p : part) out.insert(out.end(), p.begin(), p.end());
-        return;
-    }
-    for (int64_t A = a0; A < a1; ++A)
-        for (int64_t B = (tr[0] == tr[1] ? A : b0); B < b1; ++B) {
-            const int64_t cs = (tr[1] == tr[2]) ? B : cb0;
-            for (int64_t c = cs; c < cb1; c += kMaxItemBlocks) {
-                TripletItem it{};
-                it.term = term_slot;
-                it.A = (int)A;
-                it.B = (int)B;
-                it.c0 = (int)c;
-                it.c1 = (int)std::min(cb1, c + kMaxItemBlocks);
-                out.push_back(it);
-            }
-        }
-}
 
 // Items [begin, end) of v ordered by C-range length, longest first (the
 // persistent kernels take them in that order, so the long ones do not
@@ -277,6 +221,130 @@ static void sort_items_longest_first(std::vector<TripletItem> &v, size_t begin) 
     std::vector<TripletItem> tmp(n);
     for (size_t i = begin; i < v.size(); ++i) tmp[start[kKeys - (v[i].c1 - v[i].c0)]++] = v[i];
     std::copy(tmp.begin(), tmp.end(), v.begin() + (std::ptrdiff_t)begin);
+}
+
+// Work items of one term, appended to out longest C-range first (ties in
+// (A, B, C) order): blocks A <= B <= C of the term's segments.  With the
+// blocks' reach lists (TPCF_MODE_CELLS), only the block triples whose three
+// pairwise box gaps are below the cut (no triplet with a squared side at or
+// above it can count).
+static void term_items(const Layout &L, const int *tr, int term_slot, std::vector<TripletItem> &out,
+                       const Reach *reach = nullptr, int64_t merge_gap = 0) {
+    const int64_t a0 = L.seg_blk0[tr[0]], a1 = a0 + L.seg_nblk[tr[0]];
+    const int64_t b0 = L.seg_blk0[tr[1]], b1 = b0 + L.seg_nblk[tr[1]];
+    const int64_t cb0 = L.seg_blk0[tr[2]], cb1 = cb0 + L.seg_nblk[tr[2]];
+    if (reach) {
+        // Blocks A of segment a split over host threads; each thread's items
+        // (in A order) are counted per C-range length, then every thread
+        // writes its items at their places in the term's longest-first order
+        // (a stable counting sort without the concatenation).
+        constexpr int kKeys = (int)kMaxItemBlocks + 1;
+        const int64_t last_w = (cb1 - 1) >> 6;
+        std::vector<std::vector<TripletItem>> part(16);
+        std::vector<std::vector<size_t>> count(16);
+        parallel_ranges(a1 - a0, [&](int64_t lo, int64_t hi, int worker) {
+            std::vector<TripletItem> &mine = part[worker];
+            for (int64_t A = a0 + lo; A < a0 + hi; ++A) {
+                const uint64_t *ra = reach->row(A);
+                const int64_t bs = tr[0] == tr[1] ? A : b0;
+                const int64_t fa = reach->first[A], la = reach->last[A];
+                // Each B within reach of A, from the set bits of A's row.
+                for (int64_t wb = std::max<int64_t>(bs >> 6, fa); wb <= std::min<int64_t>((b1 - 1) >> 6, la); ++wb) {
+                    uint64_t mb = ra[wb];
+                    if (wb == bs >> 6) mb &= ~0ull << (bs & 63);
+                    if (wb == (b1 - 1) >> 6 && (b1 & 63)) mb &= ~0ull >> (64 - (b1 & 63));
+                    for (; mb; mb &= mb - 1) {
+                        const int64_t B = wb * 64 + __builtin_ctzll(mb);
+                        const uint64_t *rb = reach->row(B);
+                        const int64_t cs = std::max<int64_t>((tr[1] == tr[2]) ? B : cb0, cb0);
+                        if (cs >= cb1) continue;
+                        // Runs of set bits of ra & rb in [cs, cb1), word by
+                        // word; runs at most merge_gap blocks apart share an
+                        // item (the kernel skips the blocks between them by
+                        // their tiles' empty flags), so a run cut at a word
+                        // boundary joins up again.
+                        int64_t p0 = -1, p1 = -1;
+                        auto emit = [&](int64_t r0, int64_t r1) {
+                            for (int64_t q = r0; q < r1; q += kMaxItemBlocks) {
+                                TripletItem it{};
+                                it.term = term_slot;
+                                it.A = (int)A;
+                                it.B = (int)B;
+                                it.c0 = (int)q;
+                                it.c1 = (int)std::min<int64_t>(r1, q + kMaxItemBlocks);
+                                mine.push_back(it);
+                            }
+                        };
+                        const int64_t w1 = std::min<int64_t>({last_w, la, (int64_t)reach->last[B]});
+                        for (int64_t w = std::max<int64_t>({cs >> 6, fa, (int64_t)reach->first[B]}); w <= w1; ++w) {
+                            uint64_t m = ra[w] & rb[w];
+                            if (w == cs >> 6) m &= ~0ull << (cs & 63);
+                            if (w == last_w && (cb1 & 63)) m &= ~0ull >> (64 - (cb1 & 63));
+                            while (m) {
+                                const int b = __builtin_ctzll(m);
+                                const uint64_t ones = ~(m >> b);
+                                const int len = ones ? __builtin_ctzll(ones) : 64 - b;
+                                const int64_t r0 = w * 64 + b, r1 = r0 + len;
+                                m = b + len >= 64 ? 0ull : m & (~0ull << (b + len));
+                                if (p1 >= 0 && r0 - p1 <= merge_gap) {
+                                    p1 = r1;
+                                } else {
+                                    if (p1 >= 0) emit(p0, p1);
+                                    p0 = r0;
+                                    p1 = r1;
+                                }
+                            }
+                        }
+                        if (p1 >= 0) emit(p0, p1);
+                    }
+                }
+            }
+            std::vector<size_t> &cnt = count[worker];
+            cnt.assign(kKeys, 0);
+            for (const TripletItem &it : mine) ++cnt[kMaxItemBlocks - (it.c1 - it.c0)];
+        });
+        // Place of each (key, thread) group: keys in order, threads in A order.
+        size_t acc = out.size();
+        for (int k = 0; k < kKeys; ++k)
+            for (auto &cnt : count)
+                if (!cnt.empty()) {
+                    const size_t c = cnt[k];
+                    cnt[k] = acc;
+                    acc += c;
+                }
+        const size_t total = acc - out.size();
+        out.resize(acc);
+        const auto place = [&](int w) {
+            std::vector<size_t> &at = count[w];
+            for (const TripletItem &it : part[w]) out[at[kMaxItemBlocks - (it.c1 - it.c0)]++] = it;
+        };
+        // A thread each only for large terms (a thread start costs about as
+        // much as placing ten thousand items).
+        if (total < (size_t(1) << 17)) {
+            for (int w = 0; w < 16; ++w) place(w);
+        } else {
+            std::vector<std::thread> th;
+            for (int w = 0; w < 16; ++w)
+                if (!part[w].empty()) th.emplace_back(place, w);
+            for (auto &t : th) t.join();
+        }
+        return;
+    }
+    const size_t before = out.size();
+    for (int64_t A = a0; A < a1; ++A)
+        for (int64_t B = (tr[0] == tr[1] ? A : b0); B < b1; ++B) {
+            const int64_t cs = (tr[1] == tr[2]) ? B : cb0;
+            for (int64_t c = cs; c < cb1; c += kMaxItemBlocks) {
+                TripletItem it{};
+                it.term = term_slot;
+                it.A = (int)A;
+                it.B = (int)B;
+                it.c0 = (int)c;
+                it.c1 = (int)std::min(cb1, c + kMaxItemBlocks);
+                out.push_back(it);
+            }
+        }
+    sort_items_longest_first(out, before);
 }
 
 // Everything one job needs on the device, laid out in one region.
@@ -585,10 +653,8 @@ static void build_queue(const std::vector<const JobPlan *> &plans, const std::ve
             T.nblk = P.L.nblk;
             T.hist = reinterpret_cast<unsigned long long *>(bases[j] + P.o_hist) + (size_t)t * P.cells;
             T.n = P.n;
-            const size_t before = Q.items.size();
             term_items(P.L, jobs[j]->term[t], (int)Q.terms.size(), Q.items, P.cull ? &P.reach : nullptr,
                        kMergeGapBlocks);
-            sort_items_longest_first(Q.items, before);
             Q.terms.push_back(T);
             Q.max_n = std::max(Q.max_n, P.n);
         }
@@ -764,10 +830,8 @@ static int sqt_plan(tpcf_ctx *ctx, const std::vector<const JobPlan *> &plans,
         L.lds_cells = std::max(L.lds_cells, T.cells);
         for (int t = 0; t < P.nterms; ++t) {
             T.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist) + (size_t)t * P.cells;
-            const size_t before = L.items.size();
             term_items(P.L, jobs[j]->term[t], (int)L.terms.size(), L.items, P.cull ? &P.reach : nullptr,
                        kMergeGapBlocks);
-            sort_items_longest_first(L.items, before);
             L.terms.push_back(T);
         }
     }
