@@ -165,11 +165,7 @@ int sort_segments(tpcf_ctx *ctx, const tpcf_job *jobs, int njobs, std::vector<st
         const int n = (int)(std::get<2>(segs[i]) - lo);
         if (n == 0) continue;
         double blo[3], bhi[3];
-        for (int a = 0; a < 3; ++a) {
-            const double *v = c->h.data() + (size_t)a * c->n + lo;
-            blo[a] = *std::min_element(v, v + n);
-            bhi[a] = *std::max_element(v, v + n);
-        }
+        seg_bounds(c, lo, lo + n, blo, bhi);
         const double span = std::max({bhi[0] - blo[0], bhi[1] - blo[1], bhi[2] - blo[2]});
         double *sorted = reinterpret_cast<double *>(base + o_sorted[i]);
         double *box = reinterpret_cast<double *>(base + o_box[i]);
@@ -216,37 +212,45 @@ int sort_segments(tpcf_ctx *ctx, const tpcf_job *jobs, int njobs, std::vector<st
 // runs skip the chunks whose box gap to the i block's box reaches T[n].
 void term_pair_items(int slot, int na, int nb, bool same, int block, int chunk, const std::vector<double> *abox,
                      const std::vector<double> *bbox, double Tn, std::vector<PairHistItem> &out) {
-    for (int64_t i0 = 0; i0 < na; i0 += block) {
-        const int64_t js = same ? i0 + 1 : 0;
-        if (js >= nb) continue;
-        if (!abox) {
-            for (int64_t j0 = js; j0 < nb; j0 += chunk)
-                out.push_back({slot, (int)i0, (int)j0, (int)std::min<int64_t>(nb, j0 + chunk)});
-            continue;
-        }
-        double ib[6] = {HUGE_VAL, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-        for (int64_t c = i0 / kCellChunk; c * kCellChunk < std::min<int64_t>(na, i0 + block); ++c)
-            for (int a = 0; a < 3; ++a) {
-                ib[a] = std::min(ib[a], (*abox)[6 * c + a]);
-                ib[3 + a] = std::max(ib[3 + a], (*abox)[6 * c + 3 + a]);
+    // i blocks split over host threads, each thread's items appended in
+    // block order.
+    const int64_t nblocks = ((int64_t)na + block - 1) / block;
+    std::vector<std::vector<PairHistItem>> part(16);
+    parallel_ranges(nblocks, [&](int64_t k0, int64_t k1, int worker) {
+        std::vector<PairHistItem> &mine = part[worker];
+        for (int64_t i0 = k0 * block; i0 < std::min<int64_t>(na, k1 * block); i0 += block) {
+            const int64_t js = same ? i0 + 1 : 0;
+            if (js >= nb) continue;
+            if (!abox) {
+                for (int64_t j0 = js; j0 < nb; j0 += chunk)
+                    mine.push_back({slot, (int)i0, (int)j0, (int)std::min<int64_t>(nb, j0 + chunk)});
+                continue;
             }
-        int64_t r0 = -1, r1 = -1;
-        for (int64_t c = js / kCellChunk; c * kCellChunk < nb; ++c) {
-            const int64_t j0 = std::max<int64_t>(js, c * kCellChunk), j1 = std::min<int64_t>(nb, (c + 1) * kCellChunk);
-            if (box_gap2(ib, bbox->data() + 6 * c) < Tn) {
-                if (r0 < 0) r0 = j0;
-                r1 = j1;
-                if (r1 - r0 >= chunk) {
-                    out.push_back({slot, (int)i0, (int)r0, (int)r1});
+            double ib[6] = {HUGE_VAL, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+            for (int64_t c = i0 / kCellChunk; c * kCellChunk < std::min<int64_t>(na, i0 + block); ++c)
+                for (int a = 0; a < 3; ++a) {
+                    ib[a] = std::min(ib[a], (*abox)[6 * c + a]);
+                    ib[3 + a] = std::max(ib[3 + a], (*abox)[6 * c + 3 + a]);
+                }
+            int64_t r0 = -1, r1 = -1;
+            for (int64_t c = js / kCellChunk; c * kCellChunk < nb; ++c) {
+                const int64_t j0 = std::max<int64_t>(js, c * kCellChunk), j1 = std::min<int64_t>(nb, (c + 1) * kCellChunk);
+                if (box_gap2(ib, bbox->data() + 6 * c) < Tn) {
+                    if (r0 < 0) r0 = j0;
+                    r1 = j1;
+                    if (r1 - r0 >= chunk) {
+                        mine.push_back({slot, (int)i0, (int)r0, (int)r1});
+                        r0 = -1;
+                    }
+                } else if (r0 >= 0) {
+                    mine.push_back({slot, (int)i0, (int)r0, (int)r1});
                     r0 = -1;
                 }
-            } else if (r0 >= 0) {
-                out.push_back({slot, (int)i0, (int)r0, (int)r1});
-                r0 = -1;
             }
+            if (r0 >= 0) mine.push_back({slot, (int)i0, (int)r0, (int)r1});
         }
-        if (r0 >= 0) out.push_back({slot, (int)i0, (int)r0, (int)r1});
-    }
+    });
+    for (auto &p : part) out.insert(out.end(), p.begin(), p.end());
 }
 
 }  // namespace

@@ -133,21 +133,6 @@ struct Reach {
     bool has(int64_t X, int64_t Y) const { return (row(X)[Y >> 6] >> (Y & 63)) & 1u; }
 };
 
-// Run body(lo, hi, worker) over [0, n) split into contiguous ranges on up to
-// 16 host threads (one range per thread; small n stays on the caller).
-template <class F>
-static void parallel_ranges(int64_t n, F body) {
-    const int64_t nt = std::min<int64_t>(std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency())),
-                                         std::max<int64_t>(1, n / 64));
-    if (nt <= 1) {
-        body((int64_t)0, n, 0);
-        return;
-    }
-    std::vector<std::thread> th;
-    for (int64_t w = 0; w < nt; ++w) th.emplace_back(body, n * w / nt, n * (w + 1) / nt, (int)w);
-    for (auto &t : th) t.join();
-}
-
 static Reach block_reach(const std::vector<double> &boxes, int64_t nblk, double cut2) {
     Reach r;
     r.words = (nblk + 63) / 64;
@@ -422,13 +407,13 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     // Origin and extent of the union for the FP32 copy.
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int s = 0; s < job.nseg; ++s) {
-        const tpcf_catalog *c = job.seg[s];
-        for (int d = 0; d < 3; ++d)
-            for (int64_t i = job.seg_lo[s]; i < job.seg_hi[s]; ++i) {
-                const double v = c->h[d * c->n + i];
-                lo[d] = std::min(lo[d], v);
-                hi[d] = std::max(hi[d], v);
-            }
+        if (job.seg_hi[s] <= job.seg_lo[s]) continue;
+        double slo[3], shi[3];
+        seg_bounds(job.seg[s], job.seg_lo[s], job.seg_hi[s], slo, shi);
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], slo[d]);
+            hi[d] = std::max(hi[d], shi[d]);
+        }
     }
     double r = 0;
     for (int d = 0; d < 3; ++d) {
@@ -519,11 +504,7 @@ static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStrea
             // Morton order (the counts do not depend on the order) and the
             // box of every 64-point block.
             double lo[3], hi[3];
-            for (int d = 0; d < 3; ++d) {
-                const double *v = c->h.data() + (size_t)d * c->n + job.seg_lo[s];
-                lo[d] = *std::min_element(v, v + m);
-                hi[d] = *std::max_element(v, v + m);
-            }
+            seg_bounds(c, job.seg_lo[s], job.seg_lo[s] + m, lo, hi);
             const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
             const int64_t lo_i = job.seg_lo[s];
             HIP_TRY(sort_segment(c->d + lo_i, c->d + c->n + lo_i, c->d + 2 * c->n + lo_i, (int)m, lo, span,
@@ -1178,15 +1159,28 @@ int tpcf_ctx_set_memory_budget(tpcf_ctx *ctx, uint64_t bytes) {
 int tpcf_catalog_create(tpcf_ctx *ctx, const double *xyz, int64_t n, tpcf_catalog **out) {
     clear_error();
     if (!ctx || !out || n < 0 || (n > 0 && !xyz)) return set_error(TPCF_EINVAL, "bad catalog arguments");
-    for (int64_t i = 0; i < 3 * n; ++i)
-        if (!(std::fabs(xyz[i]) <= kMaxCoordinate))
-            return set_error(TPCF_EINVAL, "coordinate %lld of the catalog is not finite or exceeds %g in magnitude",
-                             (long long)i, kMaxCoordinate);
+    double blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+    for (int d = 0; d < 3 && n > 0; ++d) {
+        const double *v = xyz + (size_t)d * n;
+        blo[d] = bhi[d] = v[0];
+        for (int64_t i = 0; i < n; ++i) {
+            if (!(std::fabs(v[i]) <= kMaxCoordinate))
+                return set_error(TPCF_EINVAL,
+                                 "coordinate %lld of the catalog is not finite or exceeds %g in magnitude",
+                                 (long long)(d * n + i), kMaxCoordinate);
+            blo[d] = std::min(blo[d], v[i]);
+            bhi[d] = std::max(bhi[d], v[i]);
+        }
+    }
     HIP_TRY(hipSetDevice(ctx->device));
     auto *c = new tpcf_catalog();
     c->ctx = ctx;
     c->n = n;
     c->h.assign(xyz, xyz + 3 * n);
+    for (int d = 0; d < 3; ++d) {
+        c->lo[d] = blo[d];
+        c->hi[d] = bhi[d];
+    }
     if (n > 0) {
         if (hipMalloc(&c->d, sizeof(double) * 3 * n) != hipSuccess) {
             delete c;

@@ -1,6 +1,7 @@
 // tpcf_host.h -- host-side internals of libtpcf (not part of the C ABI).
 #pragma once
 
+#include <algorithm>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -79,11 +80,49 @@ struct tpcf_catalog {
     double *d = nullptr;   // device SoA [x|y|z]
     int64_t n = 0;
     std::vector<double> h; // host copy (origin / extent computations)
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // per-axis bounds (n > 0)
 };
 
 namespace tpcf {
 struct Ticket;
 struct Workspace;
+
+// Per-axis bounds of points [lo_i, hi_i) of a catalog (non-empty range):
+// the bounds kept at creation for the whole catalog, else one pass.
+inline void seg_bounds(const tpcf_catalog *c, int64_t lo_i, int64_t hi_i, double lo[3], double hi[3]) {
+    if (lo_i == 0 && hi_i == c->n) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = c->lo[d];
+            hi[d] = c->hi[d];
+        }
+        return;
+    }
+    for (int d = 0; d < 3; ++d) {
+        const double *v = c->h.data() + (size_t)d * c->n;
+        double a = v[lo_i], b = v[lo_i];
+        for (int64_t i = lo_i + 1; i < hi_i; ++i) {
+            a = std::min(a, v[i]);
+            b = std::max(b, v[i]);
+        }
+        lo[d] = a;
+        hi[d] = b;
+    }
+}
+
+// Run body(lo, hi, worker) over [0, n) split into contiguous ranges on up to
+// 16 host threads (one range per thread; small n stays on the caller).
+template <class F>
+void parallel_ranges(int64_t n, F body) {
+    const int64_t nt = std::min<int64_t>(std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency())),
+                                         std::max<int64_t>(1, n / 64));
+    if (nt <= 1) {
+        body((int64_t)0, n, 0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int64_t w = 0; w < nt; ++w) th.emplace_back(body, n * w / nt, n * (w + 1) / nt, (int)w);
+    for (auto &t : th) t.join();
+}
 }  // namespace tpcf
 
 namespace tpcf {
