@@ -492,7 +492,9 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
 }
 
 // Upload the small tables, gather the segments into the union layout and
-// clear histograms/counters.  All on `st`, asynchronously.
+// clear histograms/counters.  All on `st`, asynchronously; with culling the
+// block boxes are queued for download into P.boxes, and cull_job(), once the
+// stream has been synchronized, builds the reach rows from them.
 static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStream_t st) {
     double *u64 = reinterpret_cast<double *>(base + P.o_u64);
     HIP_TRY(hipMemsetAsync(u64, 0, sizeof(double) * 3 * P.L.npad, st));
@@ -520,10 +522,6 @@ static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStrea
         P.boxes.resize((size_t)6 * P.L.nblk);
         HIP_TRY(hipMemcpyAsync(P.boxes.data(), base + P.o_box, sizeof(double) * P.boxes.size(), hipMemcpyDeviceToHost,
                                st));
-        HIP_TRY(hipStreamSynchronize(st));
-        P.reach = block_reach(P.boxes, P.L.nblk, P.cut2);
-        HIP_TRY(hipMemcpyAsync(base + P.o_need, P.reach.bits.data(), sizeof(uint64_t) * P.reach.bits.size(),
-                               hipMemcpyHostToDevice, st));
     }
     HIP_TRY(hipMemcpyAsync(base + P.o_blk, P.blk_count.data(), P.blk_count.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(base + P.o_T64, P.T64.data(), sizeof(double) * P.T64.size(), hipMemcpyHostToDevice, st));
@@ -535,6 +533,32 @@ static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStrea
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(base + P.o_cnt), 16, 0u, st));
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(base + P.o_hist), 2 * P.cells * P.nterms, 0u, st));
     if (P.need_tiles && P.L.npad) HIP_TRY(launch_to_f32(u64, P.L.npad, P.origin, reinterpret_cast<float4 *>(base + P.o_u32), st));
+    return TPCF_OK;
+}
+
+// The reach rows of a staged culled job (its boxes downloaded), uploaded
+// for the pair stage.
+static int cull_job(const JobPlan &P, char *base, hipStream_t st) {
+    if (!P.cull || !P.L.nblk) return TPCF_OK;
+    P.reach = block_reach(P.boxes, P.L.nblk, P.cut2);
+    HIP_TRY(hipMemcpyAsync(base + P.o_need, P.reach.bits.data(), sizeof(uint64_t) * P.reach.bits.size(),
+                           hipMemcpyHostToDevice, st));
+    return TPCF_OK;
+}
+
+// Stage jobs[j] at bases[j]: every job's staging queued, one synchronization
+// when any job culls (not one per job), then the reach rows.
+static int stage_jobs(const tpcf_job *const *jobs, const JobPlan *const *plans, char *const *bases, int njobs,
+                      hipStream_t st) {
+    bool cull = false;
+    for (int j = 0; j < njobs; ++j) {
+        if (int e = stage_job(*jobs[j], *plans[j], bases[j], st)) return e;
+        cull = cull || (plans[j]->cull && plans[j]->L.nblk);
+    }
+    if (!cull) return TPCF_OK;
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int j = 0; j < njobs; ++j)
+        if (int e = cull_job(*plans[j], bases[j], st)) return e;
     return TPCF_OK;
 }
 
@@ -922,7 +946,11 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
     if (int e = ws.init()) return e;
     if (int e = ws.region.reserve(P.bytes)) return e;
     char *base = reinterpret_cast<char *>(ws.region.p);
-    if (int e = stage_job(job, P, base, st)) return e;
+    {
+        const tpcf_job *jp = &job;
+        const JobPlan *pp = &P;
+        if (int e = stage_jobs(&jp, &pp, &base, 1, st)) return e;
+    }
     HIP_TRY(hipEventRecord(ws.ev[0], st));
     HIP_TRY(hipEventRecord(ws.ev[1], st));
     HIP_TRY(hipEventRecord(ws.ev[3], st));
@@ -984,8 +1012,17 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     if (int e = ws.init()) return e;
     if (int e = ws.region.reserve(std::max<size_t>(bytes, 256))) return e;
     char *base = reinterpret_cast<char *>(ws.region.p);
-    for (int j = 0; j < njobs; ++j)
-        if (int e = stage_job(jobs[j], plans[j], base + off[j], st)) return e;
+    {
+        std::vector<const tpcf_job *> jp(njobs);
+        std::vector<const JobPlan *> pp(njobs);
+        std::vector<char *> bp(njobs);
+        for (int j = 0; j < njobs; ++j) {
+            jp[j] = &jobs[j];
+            pp[j] = &plans[j];
+            bp[j] = base + off[j];
+        }
+        if (int e = stage_jobs(jp.data(), pp.data(), bp.data(), njobs, st)) return e;
+    }
     HIP_TRY(hipEventRecord(ws.ev[0], st));
     HIP_TRY(hipEventRecord(ws.ev[3], st));
     // (s,q,theta) and JBJ jobs: one persistent launch of their own.
@@ -1393,8 +1430,18 @@ int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *s
     std::vector<const JobPlan *> pp, sp;
     std::vector<const tpcf_job *> jj, sj;
     std::vector<char *> bb, sb;
+    {
+        std::vector<const tpcf_job *> js(njobs);
+        std::vector<const JobPlan *> ps(njobs);
+        std::vector<char *> bs(njobs);
+        for (int j = 0; j < njobs; ++j) {
+            js[j] = &b->jobs[j];
+            ps[j] = &b->plans[j];
+            bs[j] = base + b->off[j];
+        }
+        if (int e = stage_jobs(js.data(), ps.data(), bs.data(), njobs, b->st)) return e;
+    }
     for (int j = 0; j < njobs; ++j) {
-        if (int e = stage_job(b->jobs[j], b->plans[j], base + b->off[j], b->st)) return e;
         if (!b->plans[j].L.nblk) continue;
         const bool r123 = b->plans[j].param == TPCF_R123;
         (r123 ? pp : sp).push_back(&b->plans[j]);
