@@ -394,9 +394,24 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
 // plain FP64 baseline
 // ---------------------------------------------------------------------------
 
+// Add the LDS histogram (n^3 sorted cells) into g and clear it.
+__device__ inline void r123_plain_flush(uint32_t *hist, int n3, unsigned long long *g) {
+    __syncthreads();
+    for (int c = threadIdx.x; c < n3; c += blockDim.x) {
+        const uint32_t v = hist[c];
+        if (v) {
+            atomicAdd(g + c, (unsigned long long)v);
+            hist[c] = 0u;
+        }
+    }
+    __syncthreads();
+}
+
 // Static round-robin over items (one job per launch).  Lane k owns point k of
 // block C; every triplet recomputes its three FP64 squared distances and
 // bins each against the squared thresholds, then sorts the three bins.
+// The 32-bit LDS counters are flushed to the uint64 histogram before the
+// items since the last flush could add more than P.flush_limit to a cell.
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void r123_plain_kernel(PlainArgs P) {
     extern __shared__ uint32_t hist[];
@@ -409,8 +424,15 @@ __global__ __launch_bounds__(WAVES * 64) void r123_plain_kernel(PlainArgs P) {
         for (int c = threadIdx.x; c < n3; c += blockDim.x) hist[c] = 0u;
     __syncthreads();
     const int64_t np = P.npad;
+    unsigned long long since = 0;
     for (int it = blockIdx.x; it < P.nitems; it += gridDim.x) {
         const TripletItem I = P.items[it];
+        const unsigned long long add = (unsigned long long)(I.c1 - I.c0) * (kBlock * kBlock * kBlock);
+        if (lds && since > 0 && since + add > P.flush_limit) {
+            r123_plain_flush(hist, n3, P.hist);
+            since = 0;
+        }
+        since += add;
         const int A = I.A, B = I.B;
         const int cntA = P.blk_count[A], cntB = P.blk_count[B];
         for (int C = I.c0; C < I.c1; ++C) {
@@ -441,10 +463,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_plain_kernel(PlainArgs P) {
             }
         }
     }
-    __syncthreads();
-    if (lds)
-        for (int c = threadIdx.x; c < n3; c += blockDim.x)
-            if (hist[c]) atomicAdd(P.hist + c, (unsigned long long)hist[c]);
+    if (lds) r123_plain_flush(hist, n3, P.hist);
 }
 
 // ---------------------------------------------------------------------------

@@ -180,8 +180,13 @@ static Reach block_reach(const std::vector<double> &boxes, int64_t nblk, double 
     return r;
 }
 
-// Longest C-range of one work item, in blocks.
+// Longest C-range of one work item, in blocks.  The persistent kernels flush
+// their 32-bit LDS counters before an item could take a cell past
+// 0xC0000000; an item adds at most 3 * 64^3 per C block to one cell (three
+// (s,q,theta) entries per triplet), so one item alone stays far below that.
 constexpr int64_t kMaxItemBlocks = 512;
+static_assert(3 * kMaxItemBlocks * kBlock * kBlock * kBlock < 0xC0000000ll,
+              "one work item could wrap a 32-bit LDS counter");
 // Culled items: runs of reachable C blocks at most this far apart share one
 // item (the pair stages mark the tiles they skip empty, and the triplet
 // kernels skip blocks with an empty tile).
@@ -718,6 +723,18 @@ static int launch_triplets(tpcf_ctx *ctx, Workspace &ws, const std::vector<const
     return launch_queue(ctx, D, triplet_shape(ctx, Q.max_n), st);
 }
 
+// Triplets a workgroup of the plain (r1,r2,r3) kernel adds to its 32-bit LDS
+// counters between flushes: far below 2^32.  TPCF_PLAIN_FLUSH_LIMIT lowers it
+// (tests run the flush path many times with a tiny limit).
+static unsigned long long plain_flush_limit() {
+    unsigned long long lim = 0xC0000000ull;
+    if (const char *e = std::getenv("TPCF_PLAIN_FLUSH_LIMIT")) {
+        const unsigned long long v = std::strtoull(e, nullptr, 10);
+        if (v > 0 && v < lim) lim = v;
+    }
+    return lim;
+}
+
 static int launch_plain(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tpcf_job &job, char *base,
                         hipStream_t st) {
     for (int t = 0; t < P.nterms; ++t) {
@@ -754,6 +771,7 @@ static int launch_plain(tpcf_ctx *ctx, Workspace &ws, const JobPlan &P, const tp
         A.n = P.n;
         A.npad = P.L.npad;
         A.hist = reinterpret_cast<unsigned long long *>(base + P.o_hist) + (size_t)t * P.cells;
+        A.flush_limit = plain_flush_limit();
         const int grid = std::min<int64_t>((int64_t)items.size(), (int64_t)ctx->num_cu * 4);
         HIP_TRY(launch_r123_plain(A, grid, st));
         // The item buffer is reused by the next term: finish this one first.

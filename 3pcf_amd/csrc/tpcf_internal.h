@@ -69,6 +69,10 @@ struct PlainArgs {
     int n;
     int64_t npad;
     unsigned long long *hist;  // sorted cells, n^3
+    // The workgroup's 32-bit LDS counters are flushed to hist before the
+    // items it has added since the last flush could exceed this many
+    // triplets (one item adds at most (c1 - c0) * 64^3 to a cell).
+    unsigned long long flush_limit;
 };
 
 // Parametrization codes (equal to TPCF_SQT / TPCF_JBJ of the C ABI).

@@ -5,14 +5,21 @@ Headline (default): 50k clustered data + 50k uniform randoms, all four terms
 (DDD, DDR, DRR, RRR) of sorted (r1,r2,r3) histograms, FAST path (FP32 pair
 stage with FP64 recheck, integer triplet kernel), histograms exact.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--budget-s S]
 
 One step = one tpcf_count() of the whole four-term job with the catalogs
-already resident in device memory.  The JSON line reports triplets/s (unique
-triplets classified per second of the timed steps), the roofline of the
-dominant kernel (the triplet kernel, timed with HIP events recorded by the
-library on its own stream), and a CPU baseline: the oracle on the host cores
-over a bounded sample of the same catalogs.
+already resident in device memory (tens of seconds).  At most one warmup
+step runs (nothing is compiled at run time), and timed steps stop early when
+one more would end past the --budget-s wall budget: "steps" is the number
+timed, "steps_requested" the number asked for.  Each step writes a progress
+line to stderr.  The JSON line reports triplets/s (unique triplets
+classified per second of the timed steps), the roofline of the dominant
+kernel (the triplet kernel, timed with HIP events recorded by the library on
+its own stream), the batched half of the metric (64 voxels of 150k + 150k in
+one prepared launch against the FP64 plain kernel one voxel at a time, every
+histogram compared), and CPU baselines timed on bounded samples of the same
+catalogs: the reference C path (one thread, -O1) and the clean64 oracle on
+the host's threads.
 """
 from __future__ import annotations
 
@@ -62,24 +69,60 @@ def measured_traffic(args):
     return (t["fetch_size_kib"] * t["fetch_factor"] + t["write_size_kib"] * t["write_factor"]) * 1024.0
 
 
-def cpu_baseline(d, r, seconds_hint: float = 20.0):
-    """Oracle (clean64, all host threads) on the first m data + m randoms."""
+def cpu_baseline(d, r, threads_max: int = 16):
+    """CPU rates on bounded samples of the headline catalogs, timed on this
+    host in this run.
+
+    value: the reference C path -- the oracle's refcompat32 restatement of
+    the reference's triangle loop (src/3pcf_C_version_triangle_parametrization.cc:477-533,
+    FP32 distances, acosf, the reference's bins S 50 [2,12), Q 16 [0.9,4.1),
+    theta/pi 32 [0,1)) built at the reference's -O1 and run on one thread
+    over DDD/DDR/DRR/RRR of the first m data and m randoms.  Also reported:
+    the clean64 oracle (FP64, the headline's (r1,r2,r3) spec) on all host
+    threads up to 16."""
     from oracle import oracle as orc
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    m = 1800
-    segs = [d[:m], r[:m]]
+    # reference C path, one thread
+    m = 420
+    cats = {"D": np.asarray(d[:m]), "R": np.asarray(r[:m])}
+    ref_axes = [(50, 2.0, 12.0), (16, 0.9, 4.1), (32, 0.0, 1.0)]
+    iters = 0
     t0 = time.perf_counter()
-    orc.clean_count(segs, tp.FOUR_TERMS, orc.R123, [orc.axis(20, 1.0, 350.0, "log")], nthreads=threads)
+    for names, which in (("DDD", 0), ("DDR", 1), ("DRR", 1), ("RRR", 0)):
+        _, nc = orc.ref_tri([cats[c] for c in names], which, 0, ref_axes, nthreads=1, o1=True)
+        iters += nc
+    ref_dt = time.perf_counter() - t0
+    ref_trip = math.comb(2 * m, 3)
+    sys.stderr.write(f"cpu baseline: reference C path {ref_trip / ref_dt:.3e} triplets/s ({ref_dt:.1f} s)\n")
+    sys.stderr.flush()
+
+    # clean64 oracle, all host threads
+    threads = max(1, min(threads_max, os.cpu_count() or 1))
+    mc = 1500
+    t0 = time.perf_counter()
+    orc.clean_count([d[:mc], r[:mc]], tp.FOUR_TERMS, orc.R123, [orc.axis(20, 1.0, 350.0, "log")], nthreads=threads)
     dt = time.perf_counter() - t0
-    trip = math.comb(2 * m, 3)
+    trip = math.comb(2 * mc, 3)
+    sys.stderr.write(f"cpu baseline: clean64 port {trip / dt:.3e} triplets/s on {threads} threads ({dt:.1f} s)\n")
+    sys.stderr.flush()
     return {
-        "value": trip / dt,
+        "value": ref_trip / ref_dt,
         "unit": "triplets/s",
-        "cores": threads,
+        "cores": 1,
         "kind": "port",
-        "sample": f"first {m} data + {m} randoms of the same catalogs, four terms ({trip} triplets), "
-                  f"clean64 oracle, {dt:.1f} s",
+        "sample": f"reference C path: refcompat32 restatement of the reference's triangle loop (FP32, acosf, its "
+                  f"S/Q/theta bins), gcc -O1, one thread, DDD/DDR/DRR/RRR of the first {m} data + {m} randoms "
+                  f"of the headline catalogs ({ref_trip} unique triplets, {iters} reference loop iterations, "
+                  f"{ref_dt:.1f} s)",
+        "reference_loop_iterations_per_s": iters / ref_dt,
+        "clean64_port": {
+            "value": trip / dt,
+            "unit": "triplets/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"clean64 oracle (FP64, the headline's 20 log bins on [1,350)), first {mc} data + {mc} "
+                      f"randoms, four terms ({trip} triplets), {dt:.1f} s",
+        },
     }
 
 
@@ -232,14 +275,9 @@ def bench_batch(args) -> int:
             "sqt_speedup_vs_plain": (nv / (sum(slat) / len(slat))) / (nv / splain_s),
             "sqt_histograms_identical_to_plain": sexact,
         },
-        "roofline": {
-            "bound": "valu",
-            "achieved": OPS_PER_TRIPLET * triplets / dev_s / 1e12,
-            "peak": VALU_PEAK / 1e12,
-            "unit": "TOP/s (VALU lane-ops, 6 per triplet)",
-            "frac": OPS_PER_TRIPLET * triplets / dev_s / VALU_PEAK,
-            "traffic": None,
-        },
+        # No roofline: the kernel skips the tiles, rows and C blocks with no
+        # pair in range, so a per-triplet figure over all the job's triplets
+        # would not describe the work done.
     }
     print(json.dumps(line))
     sys.stderr.write(info)
@@ -383,11 +421,6 @@ def bench_serve(args) -> int:
     return 0
 
 
-# (s,q,theta): nominal work per triplet of the entry arithmetic (3 entries x
-# {q ratio test 4, cosine-rule value 5, edge search 1, index 2} lane-ops).
-SQT_OPS_PER_TRIPLET = 36.0
-
-
 def bench_sqt(args) -> int:
     """Four-term (s,q,theta) histograms: 10k clustered data + 10k uniform
     randoms in a 200 Mpc/h cube, s in [2,20) 10 log bins, q in [1,3) 8 bins,
@@ -427,7 +460,6 @@ def bench_sqt(args) -> int:
     tri_s = sum(x.triplet_ms for x in stats) / len(stats) / 1e3
     value = triplets * reps / dt
     entries = int(out.sum())
-    achieved = SQT_OPS_PER_TRIPLET * triplets / tri_s
     line = {
         "metric": "(s,q,theta) four-term triplets/s, FAST vs plain FP64, exact",
         "value": value,
@@ -460,15 +492,9 @@ def bench_sqt(args) -> int:
             "cells_triplet_kernel_ms": stc.triplet_ms,
             "cells_equals_fast": bool(np.array_equal(out, outc)),
         },
-        "roofline": {
-            "bound": "valu",
-            "achieved": achieved / 1e12,
-            "peak": VALU_PEAK / 1e12,
-            "unit": "TOP/s (VALU lane-ops, 36 nominal per triplet; the bit-mask filter settles most "
-                    "triplets without the entry arithmetic, so the fraction can exceed 1)",
-            "frac": achieved / VALU_PEAK,
-            "traffic": None,
-        },
+        # No roofline: the bit-mask filter settles most triplets 64 at a time
+        # without entry arithmetic, so the nominal per-triplet figure does not
+        # describe the work done (the profiles give the measured counters).
     }
     print(json.dumps(line))
     return 0
@@ -660,7 +686,6 @@ def bench_precision(args) -> int:
         dtc = time.perf_counter() - t1
     triplets = stats[0].triplets
     tri_s = sum(x.triplet_ms for x in stats) / len(stats) / 1e3
-    achieved = OPS_PER_TRIPLET * triplets / tri_s
     line = {
         "metric": "precision form triplets/s (FP32 pair stage + FP64 recheck), exact vs FP64",
         "value": triplets * reps / dt,
@@ -690,14 +715,8 @@ def bench_precision(args) -> int:
             "cells_triplet_kernel_ms": stc.triplet_ms,
             "cells_equals_fast": bool(np.array_equal(out, outc)),
         },
-        "roofline": {
-            "bound": "valu",
-            "achieved": achieved / 1e12,
-            "peak": VALU_PEAK / 1e12,
-            "unit": "TOP/s (VALU lane-ops, 6 per triplet)",
-            "frac": achieved / VALU_PEAK,
-            "traffic": None,
-        },
+        # No roofline: most block triples of the wedge have a side out of
+        # range and are skipped by their tiles' flags.
     }
     print(json.dumps(line))
     return 0
@@ -712,6 +731,9 @@ def main(argv=None) -> int:
     ap.add_argument("--nr", type=int, default=50000)
     ap.add_argument("--mode", choices=sorted(tp.MODES), default="fast")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="headline: leave out the 64-voxel batch figures")
+    ap.add_argument("--budget-s", type=float, default=150.0,
+                    help="headline: no step is started that would end past this many seconds of timed steps")
     ap.add_argument("--workload", choices=["headline", "batch", "serve", "sqt", "precision", "pairs", "survey"],
                     default="headline",
                     help="headline: 50k+50k four terms (default); batch: 64 angular voxels of 150k+150k in one "
@@ -761,36 +783,116 @@ def main(argv=None) -> int:
     if args.workload == "precision":
         return bench_precision(args)
 
+    return bench_headline(args)
+
+
+def voxel_batch_figures(reps: int = 10) -> dict:
+    """The batched half of the metric: 64 angular voxels (8x8 cells of a
+    20x20 deg wedge, r 500-800 Mpc/h) of 150k clustered data + 150k randoms,
+    lognormal occupancy 500..8000, all four terms of every voxel, 20 log bins
+    on [1,50) Mpc/h.  FAST as one prepared batch (one persistent launch,
+    autotuned shape, HIP graph replay) against the FP64 plain kernel running
+    the voxel jobs one at a time; every voxel's histograms compared."""
+    vox = synth.angular_voxels(8, 150000, seed=0)
+    spec = tp.r123_spec(20, 1.0, 50.0, "log", 1e-5)
+    with tp.Context(0) as ctx:
+        cats = [(ctx.catalog(dv), ctx.catalog(rv)) for dv, rv in vox]
+        jobs = [tp.four_term_job(cd, cr) for cd, cr in cats]
+        t0 = time.perf_counter()
+        b = ctx.batch(jobs, [spec] * len(jobs), tp.MODE_FAST, autotune=True, graph=True)
+        prep_s = time.perf_counter() - t0
+        b.run()
+        lat, dev = [], []
+        for _ in range(reps):
+            t1 = time.perf_counter()
+            outs, st = b.run()
+            lat.append(time.perf_counter() - t1)
+            dev.append(st.device_ms)
+        fast = [o.copy() for o in outs]
+        launch = b.info.splitlines()[0] if b.info else ""
+        b.close()
+        sys.stderr.write(f"batch: prepared FAST replay {sum(lat) / len(lat) * 1e3:.1f} ms for {len(jobs)} voxels\n")
+        sys.stderr.flush()
+        plain_lat = []
+        exact = True
+        for j, job in enumerate(jobs):
+            t1 = time.perf_counter()
+            out, _ = ctx.count(job, spec, tp.MODE_PLAIN)
+            plain_lat.append(time.perf_counter() - t1)
+            exact &= bool(np.array_equal(out, fast[j]))
+            if j % 16 == 15:
+                sys.stderr.write(f"batch: plain FP64 {j + 1}/{len(jobs)} voxels, {sum(plain_lat):.1f} s\n")
+                sys.stderr.flush()
+    nv = len(jobs)
+    mean = sum(lat) / len(lat)
+    plain_rate = nv / sum(plain_lat)
+    return {
+        "workload": "64 angular voxels (8x8 of a 20x20 deg wedge, r 500-800 Mpc/h), 150k clustered data + 150k "
+                    "randoms, lognormal occupancy, four terms per voxel, 20 log bins on [1,50) Mpc/h",
+        "voxel_jobs_per_s": nv / mean,
+        "replays": reps,
+        "replay_p50_ms": pct(lat, 50) * 1e3,
+        "replay_p99_ms": pct(lat, 99) * 1e3,
+        "device_ms_per_replay": sum(dev) / len(dev),
+        "triplets_per_replay": st.triplets,
+        "prepare_s": prep_s,
+        "launch": launch,
+        "plain_fp64_voxel_jobs_per_s": plain_rate,
+        "plain_job_p50_ms": pct(plain_lat, 50) * 1e3,
+        "plain_job_p99_ms": pct(plain_lat, 99) * 1e3,
+        "speedup_vs_plain": (nv / mean) / plain_rate,
+        "histograms_identical_to_plain": exact,
+    }
+
+
+def bench_headline(args) -> int:
+    """50k + 50k, four terms, (r1,r2,r3): warmup (at most one step: nothing
+    is compiled at run time), then timed steps until --steps are done or the
+    --budget-s wall budget would be passed by one more step (each step takes
+    tens of seconds), then the batched voxel figures and the CPU baselines."""
     d, r = headline_catalogs(args.nd, args.nr)
     spec = headline_spec()
     mode = tp.MODES[args.mode]
+    warmup = min(args.warmup, 1)
     with tp.Context(0) as ctx:
         cd, cr = ctx.catalog(d), ctx.catalog(r)
         job = tp.four_term_job(cd, cr)
-        for _ in range(args.warmup):
+        for w in range(warmup):
+            t1 = time.perf_counter()
             ctx.count(job, spec, mode)
-        stats = []
+            sys.stderr.write(f"headline: warmup step {w + 1}/{warmup} {time.perf_counter() - t1:.2f} s\n")
+            sys.stderr.flush()
+        stats, times = [], []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for k in range(args.steps):
+            if times and (time.perf_counter() - t0) + max(times) > args.budget_s:
+                break
+            t1 = time.perf_counter()
             out, st = ctx.count(job, spec, mode)
+            times.append(time.perf_counter() - t1)
             stats.append(st)
+            sys.stderr.write(f"headline: step {k + 1}/{args.steps} {times[-1]:.2f} s "
+                             f"(triplet kernel {st.triplet_ms / 1e3:.2f} s)\n")
+            sys.stderr.flush()
         dt = time.perf_counter() - t0
         cd.close()
         cr.close()
-
+    steps = len(stats)
     triplets = stats[0].triplets
     counted = int(out.sum())
-    value = triplets * args.steps / dt
-    tri_s = sum(s.triplet_ms for s in stats) / len(stats) / 1e3
+    value = triplets * steps / dt
+    tri_s = sum(s.triplet_ms for s in stats) / steps / 1e3
     achieved = OPS_PER_TRIPLET * triplets / tri_s if tri_s > 0 else 0.0
     line = {
         "metric": METRIC,
         "value": value,
         "unit": "triplets/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt * 1e3 / args.steps,
+        "steps": steps,
+        "steps_requested": args.steps,
+        "warmup": warmup,
+        "warmup_requested": args.warmup,
+        "ms_per_step": dt * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -806,9 +908,11 @@ def main(argv=None) -> int:
             "mode": args.mode,
             "triplets_per_step": triplets,
             "triplets_in_range": counted,
+            "step_s": times,
+            "wall_budget_s": args.budget_s,
             "pairs_rechecked_fraction": (stats[0].pairs_rechecked / stats[0].pairs) if stats[0].pairs else 0.0,
             "pair_stage_ms": stats[0].pair_ms,
-            "triplet_kernel_ms": stats[0].triplet_ms,
+            "triplet_kernel_ms": tri_s * 1e3,
             # nominal LDS traffic of the pair-bin form: 8 B per triplet against
             # 128 B/clk/CU x 256 CUs x 2.4 GHz
             "lds_roofline_frac": (8.0 * triplets / tri_s) / (128 * 256 * 2.4e9) if tri_s > 0 else 0.0,
@@ -818,11 +922,13 @@ def main(argv=None) -> int:
             "bound": "valu",
             "achieved": achieved / 1e12,
             "peak": VALU_PEAK / 1e12,
-            "unit": "TOP/s (VALU lane-ops, 6 per triplet)",
+            "unit": "TOP/s (VALU lane-ops, 6 per triplet; every triplet of the step is in range and visited)",
             "frac": achieved / VALU_PEAK,
             "traffic": measured_traffic(args),
         },
     }
+    if not args.no_batch:
+        line["batch"] = voxel_batch_figures(reps=10)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(d, r)
     print(json.dumps(line))

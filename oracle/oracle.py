@@ -42,46 +42,61 @@ class OrcJob(C.Structure):
     ]
 
 
+LIB_O1 = os.path.join(HERE, "liboracle_o1.so")
+
 _lib = None
+_lib_o1 = None
 
 
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def load() -> C.CDLL:
-    global _lib
+def load(o1: bool = False) -> C.CDLL:
+    """The oracle library (-O2), or with o1=True the same source built at the
+    reference's -O1 (for timing the reference C path's loop)."""
+    global _lib, _lib_o1
+    if o1:
+        if _lib_o1 is None:
+            if not os.path.exists(LIB_O1):
+                build()
+            _lib_o1 = _declare(C.CDLL(LIB_O1))
+        return _lib_o1
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        lib = C.CDLL(LIB)
-        lib.orc_clean_count.argtypes = [C.POINTER(OrcJob), C.c_void_p, C.c_int]
-        lib.orc_clean_count.restype = C.c_int
-        lib.orc_hist_cells.argtypes = [C.POINTER(OrcJob)]
-        lib.orc_hist_cells.restype = C.c_int64
-        lib.orc_edges.argtypes = [C.POINTER(OrcAxis), C.c_void_p]
-        lib.orc_edges.restype = C.c_int
-        lib.orc_sq_thresholds.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
-        lib.orc_sq_thresholds.restype = None
-        lib.orc_bin_sq.argtypes = [C.c_double, C.c_void_p, C.c_int]
-        lib.orc_bin_sq.restype = C.c_int
-        lib.orc_ref_tri.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(OrcRefAxis),
-                                    C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
-        lib.orc_ref_tri.restype = C.c_int
-        lib.orc_ref_plain.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_void_p,
-                                      C.POINTER(C.c_uint64), C.c_int]
-        lib.orc_ref_plain.restype = C.c_int
-        lib.orc_ref_which.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
-        lib.orc_ref_which.restype = C.c_int
-        lib.orc_ref_vox2gal.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
-        lib.orc_ref_vox2gal.restype = C.c_int
-        lib.orc_clean_pairs.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(OrcAxis),
-                                        C.c_void_p, C.c_int]
-        lib.orc_clean_pairs.restype = C.c_int
-        lib.orc_ref_two.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
-        lib.orc_ref_two.restype = C.c_int
-        _lib = lib
+        _lib = _declare(C.CDLL(LIB))
     return _lib
+
+
+def _declare(lib: C.CDLL) -> C.CDLL:
+    """Declare the argument and result types of the oracle functions."""
+    lib.orc_clean_count.argtypes = [C.POINTER(OrcJob), C.c_void_p, C.c_int]
+    lib.orc_clean_count.restype = C.c_int
+    lib.orc_hist_cells.argtypes = [C.POINTER(OrcJob)]
+    lib.orc_hist_cells.restype = C.c_int64
+    lib.orc_edges.argtypes = [C.POINTER(OrcAxis), C.c_void_p]
+    lib.orc_edges.restype = C.c_int
+    lib.orc_sq_thresholds.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    lib.orc_sq_thresholds.restype = None
+    lib.orc_bin_sq.argtypes = [C.c_double, C.c_void_p, C.c_int]
+    lib.orc_bin_sq.restype = C.c_int
+    lib.orc_ref_tri.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(OrcRefAxis),
+                                C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
+    lib.orc_ref_tri.restype = C.c_int
+    lib.orc_ref_plain.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_void_p,
+                                  C.POINTER(C.c_uint64), C.c_int]
+    lib.orc_ref_plain.restype = C.c_int
+    lib.orc_ref_which.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
+    lib.orc_ref_which.restype = C.c_int
+    lib.orc_ref_vox2gal.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    lib.orc_ref_vox2gal.restype = C.c_int
+    lib.orc_clean_pairs.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(OrcAxis),
+                                    C.c_void_p, C.c_int]
+    lib.orc_clean_pairs.restype = C.c_int
+    lib.orc_ref_two.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    lib.orc_ref_two.restype = C.c_int
+    return lib
 
 
 def axis(n, lo, hi, scale="lin") -> OrcAxis:
@@ -130,9 +145,9 @@ def clean_count(segments: Sequence[np.ndarray], terms, param: int, axes: Sequenc
 
 
 def ref_tri(cats: Sequence[np.ndarray], which: int, variant: int, axes, kmin_legacy: int = 0,
-            nthreads: int = 8):
+            nthreads: int = 8, o1: bool = False):
     """refcompat32 (s,q,theta) histogram (uint32, wrapping) and numcalc."""
-    lib = load()
+    lib = load(o1)
     soas = [np.ascontiguousarray(np.asarray(c, dtype=np.float32).T) for c in cats]
     ptrs = (C.c_void_p * 3)(*[s.ctypes.data for s in soas])
     ns = (C.c_int * 3)(*[s.shape[1] for s in soas])

@@ -662,6 +662,9 @@ static void clean_body(void *vctx, int tid, long long ilo, long long ihi)
         int64_t j0 = c->same_ab ? i + 1 : 0;
         for (int64_t j = j0; j < c->nb; j++) {
             double sab = sqdist(c->A, c->na, i, c->B, c->nb, j);
+            /* (r1,r2,r3): clean_triplet drops every k when the (i,j) side has
+             * no bin, so the k loop is skipped (same counts, less time). */
+            if (c->job->param == ORC_R123 && orc_bin_sq(sab, c->T0, c->job->ax[0].n) < 0) continue;
             int64_t k0 = c->same_bc ? j + 1 : 0;
             for (int64_t k = k0; k < c->nc; k++) {
                 double sac = sqdist(c->A, c->na, i, c->C, c->nc, k);

@@ -193,3 +193,22 @@ def test_batch_equals_single(ctx):
     outs, st = ctx.count_batch(jobs, specs)
     for o, w in zip(outs, wants):
         assert np.array_equal(o, w)
+
+
+def test_plain_flush_path_many_flushes(ctx, monkeypatch):
+    """The plain kernel flushes its 32-bit LDS counters to the uint64
+    histogram between items; with the limit lowered to one triplet every
+    workgroup flushes before each of its items after the first (6000 points:
+    4465 items of one term over at most 1024 workgroups).  Same histograms as
+    with the default limit and as FAST."""
+    print("case plain flush path", flush=True)
+    d = synth.neyman_scott(6000, nclust=50, sigma=3.0, size=100.0, seed=31)
+    r = synth.uniform_cube(1500, 100.0, seed=32)
+    terms = [(0, 0, 0), (0, 0, 1)]
+    base, _ = run(ctx, [d, r], terms, 16, 0.0, 40.0, mode=tp.MODE_PLAIN)
+    fast, _ = run(ctx, [d, r], terms, 16, 0.0, 40.0, mode=tp.MODE_FAST)
+    monkeypatch.setenv("TPCF_PLAIN_FLUSH_LIMIT", "1")
+    flushed, _ = run(ctx, [d, r], terms, 16, 0.0, 40.0, mode=tp.MODE_PLAIN)
+    assert int(base.sum()) > 0
+    assert np.array_equal(flushed, base)
+    assert np.array_equal(fast, base)

@@ -260,3 +260,42 @@ def test_voxel_partition_sums_to_full(tmp_path):
     for a, b, c in itertools.product(range(3), repeat=3):
         tot += run(["-X", "3", "-x", f"{a:03d}{b:03d}{c:03d}"])
     assert np.array_equal(tot, full)
+
+
+def _read_tri(path):
+    tok = open(path).read().split()
+    nb = [int(t) for t in tok[3:6]]
+    return nb, np.array(tok[15:], dtype=np.int64).reshape(nb)
+
+
+def test_clean64_matches_reference_debbie_ddd():
+    """clean64 (FP64, unique triplets, edge tables, third angle from its own
+    cosine) against a histogram the reference itself produced: the Debbie
+    LRG DDD file (examples/sandbox/DDD_Debbie_LRG_CPU_10degx10deg_LRG_binning_SLAC_1k.dat,
+    a full run of src/3pcf_C_version_triangle_parametrization.cc with
+    THETA_NBINS 25 on sample_data/MICE_LRGs_10degx10deg.dat).  The reference
+    computes in FP32 with theta2 = 1 - theta0 - theta1 (:192-236); after its
+    zero-side drop it counts the same triplets.  So the two histograms hold
+    the same number of entries and differ only where an FP32 value and the
+    FP64 value fall on two sides of an edge: measured, 2 entries of 21569 (one
+    across a theta edge, one across a q edge).  A misread entry definition
+    (another vertex's sides, q or theta off by a bin) moves thousands."""
+    gold_dir = os.path.join(HERE, "golden")
+    # columns 5-7 of "id ra dec z x y z" (src/3pcf_C_version_triangle_parametrization.cc:366-394)
+    cat = np.loadtxt(os.path.join(gold_dir, "inputs", "MICE_LRGs_10degx10deg.dat"), usecols=(4, 5, 6))
+    assert cat.shape == (1316, 3)
+    nb, gold = _read_tri(os.path.join(gold_dir, "sandbox", "DDD_Debbie_LRG_CPU_10degx10deg_LRG_binning_SLAC_1k.dat"))
+    assert nb == [50, 16, 25]
+
+    def clean(q_lo, q_hi, t_lo, t_hi):
+        h = orc.clean_count([cat], [(0, 0, 0)], orc.SQT,
+                            [orc.axis(50, 2.0, 12.0), orc.axis(16, q_lo, q_hi), orc.axis(25, t_lo, t_hi)])
+        return h[0].reshape(nb).astype(np.int64)
+
+    h = clean(0.9, 4.1, 0.0, 1.0)
+    assert int(h.sum()) == int(gold.sum()) == 21569
+    assert int(np.abs(h - gold).sum()) <= 8  # at most four entries across an edge
+    # The check is sensitive: a q axis off by one bin, or theta bins counted
+    # from the other end, move thousands of entries.
+    assert int(np.abs(clean(1.1, 4.3, 0.0, 1.0) - gold).sum()) > 1000
+    assert int(np.abs(clean(0.9, 4.1, 0.0, 1.0)[:, :, ::-1] - gold).sum()) > 1000
