@@ -101,6 +101,8 @@ _SIGNATURES = {
     "tpcf_axis_edges": (C.c_int, [C.POINTER(Axis), C.POINTER(C.c_double)]),
     "tpcf_binspec_check": (C.c_int, [C.POINTER(BinSpec)]),
     "tpcf_count": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(BinSpec), C.c_int, C.c_void_p, C.POINTER(Stats)]),
+    "tpcf_count_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(BinSpec), C.c_int, C.c_void_p,
+                                     C.POINTER(Stats)]),
     "tpcf_count_batch": (
         C.c_int,
         [C.c_void_p, C.POINTER(Job), C.POINTER(BinSpec), C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(Stats)],
@@ -307,6 +309,16 @@ class Context:
         out = np.zeros((job.nterms, cells), dtype=np.uint64)
         st = Stats()
         _check(self.lib.tpcf_count(self.handle, C.byref(job), C.byref(spec), mode, out.ctypes.data, C.byref(st)))
+        return out, st
+
+    def count_noalloc(self, job: Job, spec: BinSpec, mode: int = MODE_FAST):
+        """count() through tpcf_count_noalloc: no device allocation in the
+        call (the workspace is the library's fixed 64 MiB scratch)."""
+        cells = hist_cells(spec)
+        out = np.zeros((job.nterms, cells), dtype=np.uint64)
+        st = Stats()
+        _check(self.lib.tpcf_count_noalloc(self.handle, C.byref(job), C.byref(spec), mode, out.ctypes.data,
+                                           C.byref(st)))
         return out, st
 
     def count_batch(self, jobs: Sequence[Job], specs: Sequence[BinSpec], mode: int = MODE_FAST):

@@ -39,11 +39,14 @@ int set_error(int code, const char *fmt, ...) {
 void clear_error() { g_err.clear(); }
 
 DevBuf::~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && !fixed) (void)hipFree(p);
 }
 
 int DevBuf::reserve(size_t need) {
     if (need <= bytes) return TPCF_OK;
+    if (fixed)
+        return set_error(TPCF_ENOMEM, "the job needs %zu bytes in a part of the fixed scratch that holds %zu", need,
+                         bytes);
     if (p) {
         (void)hipFree(p);
         p = nullptr;
@@ -1126,6 +1129,42 @@ int64_t plan_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode, uint
 Workspace *new_workspace() { return new Workspace(); }
 void delete_workspace(Workspace *w) { delete w; }
 
+// ---------------------------------------------------------------------------
+// fixed scratch of tpcf_count_noalloc
+// ---------------------------------------------------------------------------
+
+// Device memory of tpcf_count_noalloc: the job's region (staged points, pair
+// tiles, tables, histograms) at the start, its work queue after it.  Each GPU
+// holds its own copy of the array for the life of the process.  Host code
+// never passes `scratch` itself (on the host it is only a stand-in for the
+// symbol): it uses the device address hipGetSymbolAddress returns, kept per
+// device in g_scratch.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+// Room kept for the work queue (items, term table, counters) after the region.
+constexpr size_t kScratchQueueMin = size_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch[kMaxDevices];
+
+// Device address of the scratch array on `device` (the current device),
+// looked up once per device.
+static int scratch_address(int device, void **out) {
+    if (device < 0 || device >= kMaxDevices) return set_error(TPCF_EDEVICE, "device %d out of range", device);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(TPCF_EDEVICE, "hipGetSymbolAddress of the fixed scratch failed on device %d: %s", device,
+                             hipGetErrorString(e));
+        g_scratch[device] = p;
+    }
+    *out = g_scratch[device];
+    return TPCF_OK;
+}
+
 }  // namespace tpcf
 
 using namespace tpcf;
@@ -1162,6 +1201,17 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) c->budget = (uint64_t)(0.9 * (double)fr);
     c->ws.reset(new Workspace());
+    // The no-allocation entry point's scratch address and events, set up
+    // here so that the call itself creates nothing.
+    c->ws_fixed.reset(new Workspace());
+    if (int e = scratch_address(device, &c->scratch)) {
+        tpcf_ctx_destroy(c);
+        return e;
+    }
+    if (int e = c->ws_fixed->init()) {
+        tpcf_ctx_destroy(c);
+        return e;
+    }
     // Kernels with an LDS histogram raise their dynamic LDS limit once, to
     // what their static LDS leaves of the 160 KiB.
     std::vector<const void *> lds_kernels{sqt_plain_kernel_ptr(kSqt), sqt_plain_kernel_ptr(kJbj),
@@ -1198,6 +1248,7 @@ int tpcf_ctx_destroy(tpcf_ctx *ctx) {
     for (auto &t : ctx->workers) t.join();
     (void)hipSetDevice(ctx->device);
     ctx->ws.reset();
+    ctx->ws_fixed.reset();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TPCF_OK;
@@ -1291,6 +1342,31 @@ int tpcf_count(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int
     std::lock_guard<std::mutex> g(ctx->call_mu);
     HIP_TRY(hipSetDevice(ctx->device));
     return run_job(ctx, *ctx->ws, *job, *spec, mode, hist, stats, ctx->stream);
+}
+
+int tpcf_count_noalloc(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int mode, uint64_t *hist,
+                       tpcf_stats *stats) {
+    clear_error();
+    if (!ctx || !job || !spec || !hist) return set_error(TPCF_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> g(ctx->call_mu);
+    uint64_t triplets = 0;
+    const int64_t need = plan_bytes(*job, *spec, mode, &triplets);
+    if (need < 0) return (int)need;
+    const size_t region = ((size_t)need + 255) & ~size_t(255);
+    if (region + kScratchQueueMin > kScratchBytes)
+        return set_error(TPCF_ENOMEM,
+                         "tpcf_count_noalloc: the job needs %lld bytes of device workspace; the fixed scratch holds "
+                         "%zu (%zu kept for the work queue); use tpcf_count",
+                         (long long)need, kScratchBytes, kScratchQueueMin);
+    HIP_TRY(hipSetDevice(ctx->device));
+    // Region first, the queue after it; one job uses only one of the two
+    // queues (the (r1,r2,r3)/plain one or the (s,q,theta) one).
+    Workspace &ws = *ctx->ws_fixed;
+    char *base = reinterpret_cast<char *>(ctx->scratch);
+    ws.region.view(base, region);
+    ws.queue.view(base + region, kScratchBytes - region);
+    ws.sqt_queue.view(base + region, kScratchBytes - region);
+    return run_job(ctx, ws, *job, *spec, mode, hist, stats, ctx->stream);
 }
 
 int tpcf_count_batch(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode,

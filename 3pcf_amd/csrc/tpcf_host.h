@@ -40,12 +40,20 @@ void sq_thresholds(const double *e, int n, double *T);
 int binspec_check(const tpcf_binspec &s);
 int64_t hist_cells(const tpcf_binspec &s);
 
-// Device allocation that remembers its size.
+// Device allocation that remembers its size.  A fixed buffer is a view of
+// memory it does not own (the no-allocation entry point's scratch): it never
+// grows, reallocates or frees, and reserve() beyond its size is an error.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool fixed = false;
     ~DevBuf();
     int reserve(size_t need);  // grow-only
+    void view(void *at, size_t n) {
+        p = at;
+        bytes = n;
+        fixed = true;
+    }
 };
 
 // Union layout of a job: each segment padded to whole 64-point blocks.
@@ -147,6 +155,10 @@ struct tpcf_ctx {
     hipStream_t stream = nullptr;
     std::mutex call_mu;                    // serializes synchronous calls
     std::unique_ptr<tpcf::Workspace, tpcf::WorkspaceDeleter> ws;  // workspace of synchronous calls
+    // tpcf_count_noalloc: the device's fixed scratch array (its address from
+    // hipGetSymbolAddress) and a workspace of views into it, events included.
+    void *scratch = nullptr;
+    std::unique_ptr<tpcf::Workspace, tpcf::WorkspaceDeleter> ws_fixed;
     tpcf::LaunchShape shape;
     tpcf::PairWorkspace pairs;             // workspace of tpcf_count_pairs*
     // serving

@@ -185,6 +185,22 @@ int tpcf_binspec_check(const tpcf_binspec *spec);
 int tpcf_count(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int mode,
                uint64_t *hist, tpcf_stats *stats);
 
+/* tpcf_count without any device allocation in the call: the job's device
+ * workspace (staged points, pair tiles, tables, histograms, work queue) lives
+ * in a fixed 64 MiB scratch array in device memory that each GPU the library
+ * runs on holds until the process ends (its address is looked up, and the
+ * call's events created, by tpcf_ctx_create).  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, modes and
+ * results as tpcf_count.  A job whose workspace does not fit (the limit of
+ * tpcf_job_footprint() is 63 MiB, e.g. about 10k points in (r1,r2,r3) FAST
+ * mode, far more in PLAIN mode) fails with TPCF_ENOMEM and a message.  Every
+ * context on a GPU shares that GPU's scratch: calls that overlap in time on
+ * one GPU (from different contexts) race and must be serialized by the
+ * caller; calls on one context are serialized by the library. */
+int tpcf_count_noalloc(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int mode,
+                       uint64_t *hist, tpcf_stats *stats);
+
 /* Count many jobs (for instance every voxel and term of a survey) in one
  * persistent device launch.  hist[j] receives job j's histograms.  Replaces
  * the per-voxel process fan-out of examples/build_voxel_submission_scripts.py
