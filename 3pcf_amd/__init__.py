@@ -24,7 +24,9 @@ from typing import Iterable, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtpcf.so")
+# TPCF_LIBRARY names another build of the library (the host-sanitized one of
+# the Makefile's "san" target, for the CPU tests).
+LIB_PATH = os.environ.get("TPCF_LIBRARY") or os.path.join(HERE, "libtpcf.so")
 BIN_DIR = os.path.join(HERE, "bin")
 
 R123, SQT, JBJ = 0, 1, 2
