@@ -123,6 +123,14 @@ _SIGNATURES = {
     "tpcf_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(Stats)]),
     "tpcf_batch_info": (C.c_char_p, [C.c_void_p]),
     "tpcf_batch_destroy": (C.c_int, [C.c_void_p]),
+    "tpcf_spec_reach": (C.c_int, [C.POINTER(BinSpec), C.POINTER(C.c_double)]),
+    "tpcf_survey_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_double,
+                                     C.POINTER(C.c_void_p)]),
+    "tpcf_survey_cells": (C.c_int64, [C.c_void_p, C.POINTER(C.c_int)]),
+    "tpcf_survey_jobs": (C.c_int64, [C.c_void_p]),
+    "tpcf_survey_count": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BinSpec), C.c_int, C.c_void_p, C.c_void_p,
+                                    C.POINTER(Stats)]),
+    "tpcf_survey_destroy": (C.c_int, [C.c_void_p]),
     "tpcf_count_pairs": (C.c_int, [C.c_void_p, C.POINTER(Job), C.POINTER(Axis), C.c_int, C.c_void_p, C.POINTER(Stats)]),
     "tpcf_count_pairs_batch": (
         C.c_int,
@@ -350,6 +358,9 @@ class Context:
         _check(self.lib.tpcf_count_pairs_batch(self.handle, JA, AA, n, mode, ptrs, C.byref(st)))
         return outs, st
 
+    def survey(self, data: np.ndarray, randoms: np.ndarray, maxsep: float) -> "Survey":
+        return Survey(self, data, randoms, maxsep)
+
     def batch(self, jobs: Sequence[Job], specs: Sequence[BinSpec], mode: int = MODE_FAST,
               autotune: bool = True, graph: bool = True) -> "Batch":
         return Batch(self, jobs, specs, mode, (BATCH_AUTOTUNE if autotune else 0) | (BATCH_GRAPH if graph else 0))
@@ -391,6 +402,52 @@ class Context:
         st = Stats()
         _check(self.lib.tpcf_wait(self.handle, ticket, C.byref(st)))
         return st
+
+
+class Survey:
+    """A survey (data + randoms) split into spatial cells of side >= maxsep,
+    counted cell by cell (tpcf_survey_*); see include/tpcf.h."""
+
+    def __init__(self, ctx: "Context", data: np.ndarray, randoms: np.ndarray, maxsep: float):
+        self.ctx = ctx
+        d = np.ascontiguousarray(np.asarray(data, dtype=np.float64).T)
+        r = np.ascontiguousarray(np.asarray(randoms, dtype=np.float64).T)
+        self.handle = C.c_void_p()
+        _check(ctx.lib.tpcf_survey_create(ctx.handle, d.ctypes.data, d.shape[1], r.ctypes.data, r.shape[1],
+                                          float(maxsep), C.byref(self.handle)))
+        dims = (C.c_int * 3)()
+        self.ncells = _check(ctx.lib.tpcf_survey_cells(self.handle, dims))
+        self.dims = tuple(dims)
+        self.njobs = _check(ctx.lib.tpcf_survey_jobs(self.handle))
+
+    def count(self, spec: BinSpec, mode: int = MODE_FAST, per_cell: bool = False):
+        """(hist (4, cells): DDD, DDR, DRR, RRR of the survey, per-cell
+        histograms (ncells, 4, cells) or None, Stats)."""
+        cells = hist_cells(spec)
+        hist = np.zeros((4, cells), dtype=np.uint64)
+        pc = np.zeros((self.ncells, 4, cells), dtype=np.uint64) if per_cell else None
+        st = Stats()
+        _check(self.ctx.lib.tpcf_survey_count(self.ctx.handle, self.handle, C.byref(spec), mode, hist.ctypes.data,
+                                              pc.ctypes.data if pc is not None else None, C.byref(st)))
+        return hist, pc, st
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.tpcf_survey_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spec_reach(spec: BinSpec) -> float:
+    """Longest side a counted triangle of spec can have (tpcf_spec_reach)."""
+    out = C.c_double()
+    _check(load_library().tpcf_spec_reach(C.byref(spec), C.byref(out)))
+    return out.value
 
 
 class Batch:

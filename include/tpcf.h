@@ -86,6 +86,7 @@ enum {
 
 typedef struct tpcf_ctx tpcf_ctx;
 typedef struct tpcf_catalog tpcf_catalog;
+typedef struct tpcf_survey tpcf_survey;
 
 typedef struct {
     int n;        /* bins, 1..255 (pair counts: 1..4096) */
@@ -243,6 +244,40 @@ int tpcf_wait(tpcf_ctx *ctx, int64_t ticket, tpcf_stats *stats);
 
 /* Device bytes the job's workspace needs (admission footprint). */
 int64_t tpcf_job_footprint(const tpcf_job *job, const tpcf_binspec *spec, int mode);
+
+/* ---- surveys split into spatial cells ------------------------------------ */
+
+/* Longest side a counted triangle of spec can have: r123 the last edge;
+ * (s,q,theta) s_hi (1 + max(q_hi, 1)); JBJ s_hi (1/q_lo + v_hi) (TPCF_EINVAL
+ * when q_lo = 0: no bound), the last two with a 1e-6 relative margin. */
+int tpcf_spec_reach(const tpcf_binspec *spec, double *reach);
+
+/* A survey (data and randoms, FP64 SoA [x|y|z]) split into spatial cells of
+ * side >= maxsep over their common bounding box, with the jobs that count
+ * its four terms cell by cell: each counted triangle (every side shorter
+ * than maxsep) belongs to the lowest-index cell among its points' cells and
+ * is counted by that cell's jobs, over the cell's own points and copies of
+ * the points of its 13 higher-index neighbour cells.  The cell jobs'
+ * histograms add up to the survey's, bit for bit.  Replaces the index-chunk
+ * voxels of src/3pcf_C_version_triangle_parametrization.cc:52-80 and the
+ * per-voxel process fan-out of examples/build_voxel_submission_scripts.py:9-41
+ * with the cell division of examples/test_predivision_of_distances.py:11-35
+ * (define_ranges) and its neighbour-cell loop (:123-205).  The survey holds
+ * four device catalogs until tpcf_survey_destroy(). */
+int tpcf_survey_create(tpcf_ctx *ctx, const double *d_xyz, int64_t nd, const double *r_xyz, int64_t nr,
+                       double maxsep, tpcf_survey **out);
+/* Number of cells (and the cells per axis in dims, if given). */
+int64_t tpcf_survey_cells(const tpcf_survey *survey, int dims[3]);
+/* Number of cell jobs (at most six per cell). */
+int64_t tpcf_survey_jobs(const tpcf_survey *survey);
+/* Count every cell job (FAST, FAST64, CELLS: one batched launch; PLAIN: one
+ * job at a time).  hist: 4 x cells uint64 (DDD, DDR, DRR, RRR of the whole
+ * survey); per_cell (optional): ncells x 4 x cells, the triangles each cell
+ * owns (what sum_voxels.py adds up).  spec must reach at most the survey's
+ * maxsep (TPCF_EINVAL otherwise). */
+int tpcf_survey_count(tpcf_ctx *ctx, const tpcf_survey *survey, const tpcf_binspec *spec, int mode,
+                      uint64_t *hist, uint64_t *per_cell, tpcf_stats *stats);
+int tpcf_survey_destroy(tpcf_survey *survey);
 
 /* ---- pair counts (two-point function) ---------------------------------- */
 

@@ -187,3 +187,63 @@ def test_program_rejects_bad_input(tmp_path):
     assert r.returncode != 0 and "out of range" in r.stderr
     with pytest.raises(tp.TpcfError):
         tp.triplet_norm([10, 10, 10], 7)
+
+
+# ---- reduced amplitude: hand-worked cases -------------------------------------
+# Q = zeta / (xi12 xi23 + xi23 xi31 + xi31 xi12) with the sides of the triangle
+# (s, q, theta) -> (s, s q, s sqrt(1 + q^2 - 2 q cos theta)) and xi looked up at
+# the nearest table abscissa (examples/plot_MICE_3pcf.py:43-51, :9-30, :196-221).
+# Each case below has sides and a denominator known exactly.
+
+def _table(xs, ys):
+    return np.asarray(xs, dtype=float), np.asarray(ys, dtype=float)
+
+
+@pytest.mark.parametrize("case", [
+    # (s, q, theta_lo, theta_width, table xs, table ys, want denominator)
+    # 3-4-5 right triangle: theta = pi (0 + 1/2) = pi/2, sides 3, 4, 5
+    (3.0, 4.0 / 3.0, 0.0, 1.0, [1, 3, 4, 5, 9], [7, 2, 3, 5, 11], 2 * 3 + 3 * 5 + 5 * 2),
+    # equilateral: theta = pi (0 + 1/3) = pi/3, q = 1, every side 2
+    (2.0, 1.0, 0.0, 2.0 / 3.0, [1, 2, 3], [5, 4, 9], 3 * 4 * 4),
+    # collinear: theta = pi (0 + 1) = pi, sides 1, 2, 3
+    (1.0, 2.0, 0.0, 2.0, [1, 2, 3, 4], [1, 10, 100, 1000], 1 * 10 + 10 * 100 + 100 * 1),
+    # a side halfway between two abscissae takes the first (sides 1, 2.5, ~1.5)
+    (1.0, 2.5, 0.0, 2.0 / 3.0, [1, 2, 3], [2, 3, 7], None),
+])
+def test_reduced_q_hand_worked_triangles(case):
+    s, q, lo, width, xs, ys, want = case
+    xs, ys = _table(xs, ys)
+    zeta_row = np.array([6.2])
+    th, den, qred = tp.reduced_q(zeta_row, s, q, lo, width, xs, ys)
+    if want is None:
+        # sides 1, 2.5, sqrt(1 + 6.25 - 2.5) = 2.179...: xi 2, 3 (2.5 is as far
+        # from 2 as from 3: the first, 2), 3 -> 2*3 + 3*3 + 3*2 = 21
+        want = 21.0
+    assert th[0] == lo
+    assert den[0] == want
+    assert qred[0] == 6.2 / want
+    # the oracle's restatement of the script's steps gives the same
+    s12, s23, s31 = est.sqtheta2s1s2s3(s, q, np.pi * (np.array([lo]) + width / 2.0))
+    c12, c23, c31 = (est.sxx2cf(v, xs, ys) for v in (np.atleast_1d(s12), s23, s31))
+    assert (c12 * c23 + c23 * c31 + c31 * c12)[0] == want
+
+
+def test_reduced_q_constant_xi(tmp_path):
+    """With xi constant the denominator is 3 xi^2 for every triangle, so Q is
+    zeta / (3 xi^2) along the whole theta row: checked through the oracle's
+    file-level restatement (est.reduced) and the library on the same row."""
+    cells = [[12, 0, 5, 8], [30, 6, 20, 0], [95, 30, 10, 40], [114, 57, 20, 80]]
+    ddd = four_files(tmp_path, cells)
+    table = tmp_path / "xi_const.dat"
+    table.write_text("".join(f"{x:f} {0.5:f}\n" for x in np.arange(0.1, 70.0, 0.1)))
+    z, _, nbins, hv = est.zeta(ddd)
+    for sbin in range(nbins[0]):
+        for q, x, zrow, den, qred in est.reduced(ddd, sbin, [0], str(table)):
+            assert np.all(den == 0.75)
+            np.testing.assert_array_equal(qred, zrow / 0.75)
+            tab = np.loadtxt(table)
+            th, d2, q2 = tp.reduced_q(zrow, est.bin2val(sbin, hv[0][0], hv[0][1], nbins[0]),
+                                      est.bin2val(q, hv[1][0], hv[1][1], nbins[1]), hv[2][0], hv[2][2],
+                                      tab[:, 0], tab[:, 1])
+            assert np.all(d2 == 0.75)
+            np.testing.assert_array_equal(q2, zrow / 0.75)
