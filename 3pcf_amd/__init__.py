@@ -81,6 +81,7 @@ class Stats(C.Structure):
         ("wall_ms", C.c_double),
         ("pair_ms", C.c_double),
         ("triplet_ms", C.c_double),
+        ("queue_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -116,6 +117,7 @@ _SIGNATURES = {
     "tpcf_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Stats)]),
     "tpcf_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Stats)]),
     "tpcf_job_footprint": (C.c_int64, [C.POINTER(Job), C.POINTER(BinSpec), C.c_int]),
+    "tpcf_ctx_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "tpcf_batch_create": (
         C.c_int,
         [C.c_void_p, C.POINTER(Job), C.POINTER(BinSpec), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)],
@@ -364,6 +366,12 @@ class Context:
     def batch(self, jobs: Sequence[Job], specs: Sequence[BinSpec], mode: int = MODE_FAST,
               autotune: bool = True, graph: bool = True) -> "Batch":
         return Batch(self, jobs, specs, mode, (BATCH_AUTOTUNE if autotune else 0) | (BATCH_GRAPH if graph else 0))
+
+    def memory(self) -> dict:
+        """Serving workers' device memory: held now, peak, budget (bytes)."""
+        out = (C.c_uint64 * 3)()
+        _check(self.lib.tpcf_ctx_memory(self.handle, out))
+        return {"in_use": out[0], "peak": out[1], "budget": out[2]}
 
     def footprint(self, job: Job, spec: BinSpec, mode: int = MODE_FAST) -> int:
         return _check(self.lib.tpcf_job_footprint(C.byref(job), C.byref(spec), mode))

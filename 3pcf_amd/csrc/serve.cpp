@@ -37,7 +37,7 @@ struct Ticket {
     int rc = TPCF_OK;
     std::string err;
     tpcf_stats stats{};
-    std::chrono::steady_clock::time_point t_submit;
+    std::chrono::steady_clock::time_point t_submit, t_admit;
 };
 
 // Jobs that do not join a batched persistent launch.
@@ -52,7 +52,15 @@ static constexpr double kStarveSeconds = 60.0;
 static constexpr double kBandFactor = 4.0;
 static constexpr double kBandFloor = 1e9;
 
-static void worker_main(tpcf_ctx *ctx) {
+// Sum of the workers' held bytes into in_use, and the peak (under ctx->mu).
+static void account(tpcf_ctx *ctx) {
+    uint64_t s = 0;
+    for (uint64_t h : ctx->held) s += h;
+    ctx->in_use = s;
+    ctx->peak = std::max(ctx->peak, s);
+}
+
+static void worker_main(tpcf_ctx *ctx, int w) {
     (void)hipSetDevice(ctx->device);
     hipStream_t st = nullptr;
     (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
@@ -60,11 +68,16 @@ static void worker_main(tpcf_ctx *ctx) {
     for (;;) {
         std::vector<std::shared_ptr<Ticket>> batch;
         std::vector<int64_t> ids;
-        int64_t reserved = 0;
+        bool release_first = false;
         {
             std::unique_lock<std::mutex> lk(ctx->mu);
             for (;;) {
                 if (ctx->stopping && ctx->pending.empty()) break;
+                // This worker holds `cap` bytes; a batch of total footprint R
+                // runs in max(cap, R), or in R once it gives cap back.
+                const uint64_t cap = ctx->held[w];
+                const uint64_t others = ctx->in_use - cap;
+                uint64_t need = 0;
                 // This worker's share of the queued work (triplets).
                 uint64_t queued = 0, taken = 0;
                 for (int64_t id : ctx->pending) queued += ctx->tickets[id]->work;
@@ -102,8 +115,8 @@ static void worker_main(tpcf_ctx *ctx) {
                         ctx->cv.notify_all();
                         continue;
                     }
-                    if (ctx->in_use + reserved + (uint64_t)t->footprint <= ctx->budget) {
-                        reserved += t->footprint;
+                    if (others + need + (uint64_t)t->footprint <= ctx->budget) {
+                        need += t->footprint;
                         taken += t->work;
                         t->state = 1;
                         batch.push_back(t);
@@ -112,12 +125,27 @@ static void worker_main(tpcf_ctx *ctx) {
                         if (plain) break;
                     }
                 }
-                if (!batch.empty()) break;
+                if (!batch.empty()) {
+                    const auto now = std::chrono::steady_clock::now();
+                    for (auto &t : batch) t->t_admit = now;
+                    release_first = others + std::max(cap, need) > ctx->budget;
+                    ctx->held[w] = release_first ? need : std::max(cap, need);
+                    account(ctx);
+                    break;
+                }
+                if (!ctx->pending.empty() && cap > 0) {
+                    // Idle while queued jobs do not fit: give the memory back.
+                    workspace_release(ws.get());
+                    ctx->held[w] = 0;
+                    account(ctx);
+                    ctx->cv.notify_all();
+                    continue;
+                }
                 ctx->cv.wait(lk);
             }
             if (batch.empty()) break;
-            ctx->in_use += reserved;
         }
+        if (release_first) workspace_release(ws.get());
         std::vector<int> rcs(batch.size(), TPCF_OK);
         std::vector<std::string> errs(batch.size());
         if (solo(*batch[0])) {
@@ -151,12 +179,21 @@ static void worker_main(tpcf_ctx *ctx) {
                 batch[k]->state = rcs[k] ? 3 : 2;
                 batch[k]->stats.wall_ms =
                     std::chrono::duration<double, std::milli>(now - batch[k]->t_submit).count();
+                batch[k]->stats.queue_ms =
+                    std::chrono::duration<double, std::milli>(batch[k]->t_admit - batch[k]->t_submit).count();
             }
-            ctx->in_use -= reserved;
+            // what the workspace really holds now (at most what was admitted)
+            ctx->held[w] = workspace_held(ws.get());
+            account(ctx);
         }
         ctx->cv.notify_all();
     }
     ws.reset();
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->held[w] = 0;
+        account(ctx);
+    }
     if (st) (void)hipStreamDestroy(st);
 }
 
@@ -171,7 +208,7 @@ int tpcf_submit(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, in
     clear_error();
     if (!ctx || !job || !spec || !hist || !ticket) return set_error(TPCF_EINVAL, "NULL argument");
     uint64_t work = 0;
-    const int64_t fp = plan_bytes(*job, *spec, mode, &work);
+    const int64_t fp = footprint_bytes(*job, *spec, mode, &work);
     if (fp < 0) return (int)fp;
     auto t = std::make_shared<Ticket>();
     t->job = *job;
@@ -183,8 +220,10 @@ int tpcf_submit(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, in
     t->t_submit = std::chrono::steady_clock::now();
     {
         std::lock_guard<std::mutex> g(ctx->mu);
-        if (ctx->workers.empty())
-            for (int w = 0; w < kWorkers; ++w) ctx->workers.emplace_back(worker_main, ctx);
+        if (ctx->workers.empty()) {
+            ctx->held.assign(kWorkers, 0);
+            for (int w = 0; w < kWorkers; ++w) ctx->workers.emplace_back(worker_main, ctx, w);
+        }
         const int64_t id = ctx->next_ticket++;
         ctx->tickets[id] = t;
         ctx->pending.push_back(id);

@@ -42,6 +42,13 @@ DevBuf::~DevBuf() {
     if (p && !fixed) (void)hipFree(p);
 }
 
+void DevBuf::release() {
+    if (fixed) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
 int DevBuf::reserve(size_t need) {
     if (need <= bytes) return TPCF_OK;
     if (fixed)
@@ -1128,6 +1135,31 @@ int64_t plan_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode, uint
 
 Workspace *new_workspace() { return new Workspace(); }
 void delete_workspace(Workspace *w) { delete w; }
+size_t workspace_held(const Workspace *w) { return w->region.bytes + w->queue.bytes + w->sqt_queue.bytes; }
+void workspace_release(Workspace *w) {
+    w->region.release();
+    w->queue.release();
+    w->sqt_queue.release();
+}
+
+int64_t footprint_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode, uint64_t *triplets) {
+    JobPlan P;
+    if (int e = plan_job(job, spec, mode, P)) return e;
+    if (triplets) *triplets = P.triplets;
+    // Work items: at most one per (A, B) block pair and run of up to
+    // kMaxItemBlocks C blocks, per term; plus the term table and counters.
+    uint64_t items = 0;
+    for (int t = 0; t < job.nterms; ++t) {
+        const int *tr = job.term[t];
+        const uint64_t a = P.L.seg_nblk[tr[0]], b = P.L.seg_nblk[tr[1]], c = P.L.seg_nblk[tr[2]];
+        // culled items end where reachable C runs are more than
+        // kMergeGapBlocks apart
+        items += a * b * (mode == TPCF_MODE_CELLS ? c / kMergeGapBlocks + c / kMaxItemBlocks + 2
+                                                   : (c + kMaxItemBlocks - 1) / kMaxItemBlocks + 1);
+    }
+    const size_t queue = align256(sizeof(TripletItem) * items) + align256(256 * 4) + 256;
+    return (int64_t)(P.bytes + queue);
+}
 
 // ---------------------------------------------------------------------------
 // fixed scratch of tpcf_count_noalloc
@@ -1330,9 +1362,17 @@ int tpcf_binspec_check(const tpcf_binspec *spec) {
 
 int64_t tpcf_job_footprint(const tpcf_job *job, const tpcf_binspec *spec, int mode) {
     if (!job || !spec) return set_error(TPCF_EINVAL, "NULL argument");
-    JobPlan P;
-    if (int e = plan_job(*job, *spec, mode, P)) return e;
-    return (int64_t)P.bytes;
+    return footprint_bytes(*job, *spec, mode);
+}
+
+int tpcf_ctx_memory(tpcf_ctx *ctx, uint64_t out[3]) {
+    clear_error();
+    if (!ctx || !out) return set_error(TPCF_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    out[0] = ctx->in_use;
+    out[1] = ctx->peak;
+    out[2] = ctx->budget;
+    return TPCF_OK;
 }
 
 int tpcf_count(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int mode, uint64_t *hist,

@@ -49,6 +49,7 @@ struct DevBuf {
     bool fixed = false;
     ~DevBuf();
     int reserve(size_t need);  // grow-only
+    void release();            // free the memory (owned buffers only)
     void view(void *at, size_t n) {
         p = at;
         bytes = n;
@@ -143,6 +144,13 @@ int run_single(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf_bin
 int64_t plan_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode, uint64_t *triplets = nullptr);
 Workspace *new_workspace();
 void delete_workspace(Workspace *w);
+// Device bytes a workspace holds, and giving them back (the workspace must
+// be idle: its stream synchronized).
+size_t workspace_held(const Workspace *w);
+void workspace_release(Workspace *w);
+// Admission footprint: the job's region plus an upper bound of its work
+// queue (negative: error).
+int64_t footprint_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode, uint64_t *triplets = nullptr);
 struct WorkspaceDeleter {
     void operator()(Workspace *w) const { delete_workspace(w); }
 };
@@ -162,7 +170,11 @@ struct tpcf_ctx {
     tpcf::LaunchShape shape;
     tpcf::PairWorkspace pairs;             // workspace of tpcf_count_pairs*
     // serving
-    uint64_t budget = 0, in_use = 0;
+    // Device bytes of the serving workers' workspaces: held[w] is what worker
+    // w holds, or has been admitted to grow to, in_use their sum, peak the
+    // largest in_use so far; admission keeps in_use <= budget.
+    uint64_t budget = 0, in_use = 0, peak = 0;
+    std::vector<uint64_t> held;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<int64_t> pending;

@@ -121,6 +121,7 @@ typedef struct {
     double wall_ms;            /* host wall time of the call (serving: submit to done) */
     double pair_ms;            /* pair stage (FAST modes) */
     double triplet_ms;         /* triplet kernel alone */
+    double queue_ms;           /* serving: submit to admission */
 } tpcf_stats;
 
 /* ---- library / context ------------------------------------------------- */
@@ -242,8 +243,14 @@ int tpcf_submit(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, in
 int tpcf_poll(tpcf_ctx *ctx, int64_t ticket, tpcf_stats *stats);
 int tpcf_wait(tpcf_ctx *ctx, int64_t ticket, tpcf_stats *stats);
 
-/* Device bytes the job's workspace needs (admission footprint). */
+/* Device bytes the job's workspace needs (admission footprint): its region
+ * plus an upper bound of its work queue. */
 int64_t tpcf_job_footprint(const tpcf_job *job, const tpcf_binspec *spec, int mode);
+/* Device memory of the serving workers: out[0] bytes held now (the
+ * workspaces of running and idle workers, counted at what they were admitted
+ * to grow to), out[1] the most held at any time, out[2] the budget.  A worker
+ * gives its workspace back when a queued job would not fit beside it. */
+int tpcf_ctx_memory(tpcf_ctx *ctx, uint64_t out[3]);
 
 /* ---- surveys split into spatial cells ------------------------------------ */
 
