@@ -9,7 +9,7 @@
 // i < j (no self pairs), (a, b) terms the full product.
 //
 // FAST (pair_hist_kernel<.., CULL = false>): every pair's squared distance
-// in FP64 over Morton-ordered copies of the segments (the order does not
+// in FP64 over Hilbert-ordered copies of the segments (the order does not
 // change the counts); persistent workgroups pull items -- an i block of
 // WAVES*64*IPL points of segment a against a j range of segment b -- from an
 // atomic counter.  Each lane keeps IPL i points in registers; the j point is
@@ -22,7 +22,7 @@
 // copies), flushed with 64-bit atomics on a term change, before any copy
 // could wrap, and at exit.
 //
-// CELLS (pair_hist_kernel<.., CULL = true>): the same over Morton-ordered
+// CELLS (pair_hist_kernel<.., CULL = true>): the same over Hilbert-ordered
 // copies of the segments; each wave takes the box of its own points and
 // runs only the 64-point j runs whose box gap reaches below the last squared
 // threshold (a lower bound of every pair's squared distance).
@@ -69,7 +69,7 @@ __device__ __noinline__ int bin_walk(double s, int k, const double *D, int n) {
 // all squared distances first; if none of the wave's pairs is below the last
 // squared threshold Tn (out of range by the FP64 definition itself) the
 // batch ends there; otherwise all guesses, then all table reads, so the LDS
-// latency is paid once per batch.  (Segments are in Morton order, so a
+// latency is paid once per batch.  (Segments are in Hilbert order, so a
 // wave's batch is mostly all in or all out of range.)
 template <int IPL, int U, bool LOG, bool DIAG>
 __device__ __forceinline__ void pair_batch(const double (&xi)[IPL], const double (&yi)[IPL],
@@ -126,7 +126,7 @@ __device__ __forceinline__ void pair_item(CPairTerm *T, int i0, int j0, int j1, 
     double xi[IPL], yi[IPL], zi[IPL];
     int jmin[IPL];  // DIAG: the lane counts the pairs with j > jmin
     // The lane's points are consecutive, so a wave's 64 * IPL points are one
-    // run of the segment (a compact box in Morton order).
+    // run of the segment (a compact box in Hilbert order).
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
         const int i = i0 + (int)threadIdx.x * IPL + q;

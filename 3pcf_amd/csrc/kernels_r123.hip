@@ -8,7 +8,7 @@
 //                         error bound (or the requested relative window) of an
 //                         edge recomputed in FP64; or FP64 throughout.
 //   r123_triplet_kernel   persistent triplet stage over a work queue of
-//                         (term, A, B, C-range) block triples of Morton-ordered
+//                         (term, A, B, C-range) block triples of Hilbert-ordered
 //                         points: runs of equal cells along a tile row added
 //                         as one LDS atomic each (or one atomic per triplet on
 //                         rows with out-of-range pairs) into a privatized
@@ -234,7 +234,7 @@ __device__ __forceinline__ void r123_column(uint32_t *hist, uint32_t &w, int j, 
 // LDS histogram copies.  Lane k owns point k of block C: z = bin(j,k) is
 // preloaded for the 64 columns j of block B, y = bin(i,k) is per lane and
 // fixed for a row i of block A, and x = bin(i,j) is uniform, read with scalar
-// loads.  The blocks hold spatially ordered (Morton) points, so along a row a
+// loads.  The blocks hold spatially ordered (Hilbert) points, so along a row a
 // lane's cell (x, y, z) often stays the same for several consecutive
 // columns: each lane adds one run of equal cells at a time (the run's length
 // at its last column).  A run ends where x changes along the row (one ballot

@@ -111,7 +111,7 @@ PairShape pair_shape(const tpcf_ctx *ctx, int max_n) {
 struct SegView {
     const double *x, *y, *z;
     int n;
-    int sorted;          // index of its Morton-ordered copy (TPCF_MODE_CELLS), or -1
+    int sorted;          // index of its Hilbert-ordered copy (TPCF_MODE_CELLS), or -1
     const double *jbox;  // device boxes of its kPairBoxPoints runs (TPCF_MODE_CELLS)
 };
 
@@ -128,7 +128,7 @@ double box_gap2(const double *a, const double *b) {
     return (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2];
 }
 
-// Morton-ordered copies of the distinct segments of the jobs (on the
+// Hilbert-ordered copies of the distinct segments of the jobs (on the
 // context's stream) and their chunk boxes (on the host).
 int sort_segments(tpcf_ctx *ctx, const tpcf_job *jobs, int njobs, std::vector<std::array<SegView, 3>> &views,
                   std::vector<std::vector<double>> &boxes) {
@@ -300,7 +300,7 @@ static int run_pairs(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_axis *axes,
         if (cells) chunk = std::min(chunk, 2048);
     }
 
-    // Segments as the kernel reads them: Morton-ordered copies with chunk
+    // Segments as the kernel reads them: Hilbert-ordered copies with chunk
     // boxes (FAST and CELLS: a wave's pairs are then mostly all in or all out
     // of range), or the catalogs (PLAIN).
     std::vector<std::array<SegView, 3>> views(njobs);

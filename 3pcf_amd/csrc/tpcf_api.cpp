@@ -372,10 +372,10 @@ struct JobPlan {
     double far_s = 0;  // SQT FAST: squared sides above this cannot be in any entry
     double mid_s = 0;  // SQT FAST: squared q sides above this cannot be in any entry
     tpcf_axis axq{}, axt{};  // SQT: q and theta/pi axes
-    // TPCF_MODE_CELLS: segments staged in Morton order, 64-point block boxes
+    // TPCF_MODE_CELLS: segments staged in Hilbert order, 64-point block boxes
     // (filled by stage_job), and the squared side no counted triplet reaches.
     bool cull = false;
-    bool morton = false;  // segments staged in Morton order (cull, or the (s,q,theta)/JBJ FAST paths)
+    bool spatial = false;  // segments staged in Hilbert order (cull, or the (s,q,theta)/JBJ FAST paths)
     double cut2 = HUGE_VAL;
     mutable std::vector<double> boxes;
     mutable Reach reach;  // per block: the blocks within the cut
@@ -485,8 +485,8 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     // four terms, s [2,20) in a 200 Mpc/h cube: 50.8 -> 42.8 ms).  The
     // (r1,r2,r3) triplet kernel adds runs of equal cells along a tile row,
     // which spatial order makes long.
-    P.morton = mode != TPCF_MODE_PLAIN;
-    if (P.morton) {
+    P.spatial = mode != TPCF_MODE_PLAIN;
+    if (P.spatial) {
         // (r1,r2,r3): every side of a counted triplet is below T[n].
         // (s,q,theta), JBJ: no side of an entry's triangle exceeds far_s.
         if (P.cull) P.cut2 = spec.param == TPCF_R123 ? P.T64[P.n] : std::nextafter(P.far_s, HUGE_VAL);
@@ -517,8 +517,8 @@ static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStrea
         const tpcf_catalog *c = job.seg[s];
         const int64_t m = P.L.seg_n[s];
         if (!m) continue;
-        if (P.morton) {
-            // Morton order (the counts do not depend on the order) and the
+        if (P.spatial) {
+            // Hilbert order (the counts do not depend on the order) and the
             // box of every 64-point block.
             double lo[3], hi[3];
             seg_bounds(c, job.seg_lo[s], job.seg_lo[s] + m, lo, hi);
@@ -1553,7 +1553,7 @@ int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *s
         bytes = align256(bytes + b->plans[j].bytes);
         if (specs[j].param == TPCF_R123) max_n = std::max(max_n, b->plans[j].n);
     }
-    // CELLS: the plans hold the culling (Morton staging, reach, culled
+    // CELLS: the plans hold the culling (Hilbert staging, reach, culled
     // items, all done once below); the replayed kernels are FAST's.
     if (mode == TPCF_MODE_CELLS) b->mode = TPCF_MODE_FAST;
     b->shift = max_n <= kMaxLdsR123Bins ? 2 : 0;

@@ -198,7 +198,7 @@ size_t pair_hist_lds_bytes(int hist_words, int max_n);
 const void *pair_hist_kernel_ptr(int waves, bool cull);  // 8 waves per workgroup
 const void *pair_plain_kernel_ptr();
 
-// Morton ordering of a segment (kernels_cells.hip): sorted SoA copy
+// Hilbert ordering of a segment (kernels_cells.hip): sorted SoA copy
 // (x at sorted[0..n), y at sorted[stride..], z at sorted[2*stride..]) and the
 // box (lo xyz, hi xyz) of every chunk of consecutive sorted points.
 // scratch: sort_segment_scratch_bytes(n) bytes.
