@@ -117,25 +117,6 @@ __device__ inline int guess_bin(float v, float lo, float inv, int lg, int n) {
     return (int)__builtin_floorf(fminf(fmaxf(g, -1.f), (float)n));
 }
 
-// arccos(x)/pi for the theta hint, after Abramowitz & Stegun 4.4.46
-// (Hastings): arccos(x) = sqrt(1-x) * (a0 + a1 x + ... + a7 x^7) on [0,1],
-// absolute error below 2e-8; arccos(-x) = pi - arccos(x).
-// Only a hint (confirmed against the edge table), so fused multiply-adds and
-// the bare v_sqrt_f32 are fine.
-__device__ inline float acos_over_pi(float c) {
-    const float x = fminf(fabsf(c), 1.f);
-    float p = -0.0012624911f;
-    p = __builtin_fmaf(p, x, 0.0066700901f);
-    p = __builtin_fmaf(p, x, -0.0170881256f);
-    p = __builtin_fmaf(p, x, 0.0308918810f);
-    p = __builtin_fmaf(p, x, -0.0501743046f);
-    p = __builtin_fmaf(p, x, 0.0889789874f);
-    p = __builtin_fmaf(p, x, -0.2145988016f);
-    p = __builtin_fmaf(p, x, 1.5707963050f);
-    const float a = __builtin_amdgcn_sqrtf(1.f - x) * p * 0.318309886f;
-    return c < 0.f ? 1.f - a : a;
-}
-
 // Bin m of v against ascending FP32 edges e[0..n] given the hint m in
 // [-1, n]: returns m (or -1 out of range) and sets *unsure unless v is more
 // than its error d (plus the edge's own FP32 rounding) away from every edge
@@ -143,44 +124,25 @@ __device__ inline float acos_over_pi(float c) {
 // Branch-free: the hint m is in [-1, n]; the lower edge e[m] decides unless
 // m < 0, the upper edge e[m+1] unless m >= n (the tests of the three cases
 // m < 0, m >= n and in range, unified).
+// Both edges are read whatever m is, and the conditions combined with
+// bitwise operators, so the compiler keeps the test free of exec-mask
+// branches.
 __device__ inline int bracket_asc(float v, float d, const float *e, int n, int m, bool *unsure) {
-    const float a = e[m > 0 ? m : 0], b = e[m < n ? m + 1 : n];
-    const bool lo_ok = m < 0 || v - a > d + kU * fabsf(a);
-    const bool hi_ok = m >= n || b - v > d + kU * fabsf(b);
-    *unsure = *unsure || !(lo_ok && hi_ok);
-    return m >= 0 && m < n ? m : -1;
+    const float a = e[max(m, 0)], b = e[min(m + 1, n)];
+    const bool lo_ok = (m < 0) | (v - a > d + kU * fabsf(a));
+    const bool hi_ok = (m >= n) | (b - v > d + kU * fabsf(b));
+    *unsure = *unsure | !(lo_ok & hi_ok);
+    return ((m >= 0) & (m < n)) ? m : -1;
 }
 
 // Same for the descending cosine table (bin m iff C[m+1] < c <= C[m]); d
 // includes the table's rounding.
 __device__ inline int bracket_desc(float c, float d, const float *C, int n, int m, bool *unsure) {
-    const float a = C[m > 0 ? m : 0], b = C[m < n ? m + 1 : n];
-    const bool up_ok = m < 0 || a - c > d;
-    const bool lo_ok = m >= n || c - b > d;
-    *unsure = *unsure || !(up_ok && lo_ok);
-    return m >= 0 && m < n ? m : -1;
-}
-
-// One entry in FP32 from sides a <= b (rounded FP32 lengths of the FP64
-// sides, each within u of it), ra = 1/a and rden = 1/den from v_rcp_f32
-// (1 ulp = at most 2u).  Error bounds against the FP64 definition:
-//   q = b * ra: within (2u sides + 3u quotient) q, 6u q used;
-//   c = num * rden with num = (sa + sb) - sc, den = 2 a b (squares within 3u):
-//   |c - c64| <= 4u (sa+sb+sc)/den + 7u |c|; 5u, 7u |c| and 2u for the
-//   table's rounding used.
-// Returns the histogram cell, -1 for "out of range" (certain), and sets
-// *unsure when some value lies within its error bound of a deciding edge.
-__device__ inline int entry32(int sbin, float b, float ra, float num, float rden, float ssum, const float *fQ,
-                              const float *fA, CTerm &T, bool *unsure) {
-    const int nq = T.nq, nt = T.nt;
-    const float q = b * ra;
-    const int mq = bracket_asc(q, 6.f * kU * q, fQ, nq, guess_bin(q, T.q_lo, T.q_inv, T.q_log, nq), unsure);
-    const float c = num * rden;
-    const float dc = 5.f * kU * ssum * rden + 7.f * kU * fabsf(c) + 2.f * kU;
-    const int mt =
-        bracket_desc(c, dc, fA, nt, guess_bin(acos_over_pi(c), T.t_lo, T.t_inv, T.t_log, nt), unsure);
-    if (mq < 0 || mt < 0) return -1;
-    return (sbin * nq + mq) * nt + mt;
+    const float a = C[max(m, 0)], b = C[min(m + 1, n)];
+    const bool up_ok = (m < 0) | (a - c > d);
+    const bool lo_ok = (m >= n) | (c - b > d);
+    *unsure = *unsure | !(up_ok & lo_ok);
+    return ((m >= 0) & (m < n)) ? m : -1;
 }
 
 __device__ inline void sqt_flush64(uint32_t *h, int cells, unsigned long long *g) {
@@ -278,55 +240,110 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A queued candidate is k | j << 6 | i << 12 | (C - c0) << 18 for point i of
+// block A, j of B and k of C.  With t = e >> 6 the byte offsets of its sides
+// in the item's tiles are t & 0xFFF = i*64 + j in (A,B), and two bit-field
+// inserts: (C - c0)*4096 + i*64 + k in the tiles of row A from (A, c0) on, and
+// (C - c0)*4096 + j*64 + k in those of row B from (B, c0) on (the tiles of one
+// row are consecutive in C).
 struct SqtItemCtx {
-    const uint8_t *cAB;
-    const float *dAB;
+    const uint8_t *cAB, *cAC, *cBC;  // codes of tiles (A,B), (A,c0), (B,c0)
+    const float *dAB, *dAC, *dBC;    // FP32 side lengths of the same tiles
     int A, B, c0;
 };
 
 // A queued candidate triangle with the codes and FP32 lengths of its sides
 // (loads in flight until first used).
 struct SqtPending {
-    uint32_t e;  // i | j << 6 | k << 12 | (C - c0) << 18 (rows of blocks A, B, C)
+    uint32_t e;
     uint32_t xc, yc, zc;
     float xd, yd, zd;
     bool live;
 };
 
 // Starts loading the sides of ring entries head .. head+cnt-1, one per lane.
-__device__ inline SqtPending sqt_gather(CTerm &T, const SqtItemCtx &X, const uint32_t *Q,
-                                        uint32_t head, uint32_t qmask, int cnt, int lane) {
+__device__ inline SqtPending sqt_gather(const SqtItemCtx &X, const uint32_t *Q, uint32_t head, uint32_t qmask,
+                                        int cnt, int lane) {
     SqtPending p;
     p.live = lane < cnt;
     p.e = p.live ? Q[(head + (uint32_t)lane) & qmask] : 0u;
     p.xc = p.yc = p.zc = kCodeFar;
     p.xd = p.yd = p.zd = 0.f;
     if (p.live) {
-        const uint32_t e = p.e;
-        const int i = e & 63, j = (e >> 6) & 63, k = (e >> 12) & 63, C = X.c0 + (int)(e >> 18);
-        const int64_t tAC = tile_index(X.A, C, T.nblk) * kTileBytes;
-        const int64_t tBC = tile_index(X.B, C, T.nblk) * kTileBytes;
-        p.xc = X.cAB[i * kBlock + j];
-        p.yc = T.codes[tAC + i * kBlock + k];
-        p.zc = T.codes[tBC + j * kBlock + k];
-        p.xd = X.dAB[i * kBlock + j];
-        p.yd = T.dists[tAC + i * kBlock + k];
-        p.zd = T.dists[tBC + j * kBlock + k];
+        const uint32_t e = p.e, t = e >> 6;
+        const uint32_t oAB = t & 0xFFFu;
+        const uint32_t oAC = (e & 0x3Fu) | (t & ~0x3Fu);
+        const uint32_t oBC = (e & 0xFFFu) | (t & ~0xFFFu);
+        p.xc = X.cAB[oAB];
+        p.yc = X.cAC[oAC];
+        p.zc = X.cBC[oBC];
+        p.xd = X.dAB[oAB];
+        p.yd = X.dAC[oAC];
+        p.zd = X.dBC[oBC];
     }
     return p;
 }
 
-// Histogram entries of a gathered candidate (FP32 with FP64 recheck).
-__device__ inline void sqt_entries(CTerm &T, const SqtItemCtx &X, const SqtPending &p,
-                                   const double *sT, const double *sQ, const double *sA, const float *fQ,
-                                   const float *fA, uint32_t *hist, unsigned long long &nre) {
+// Orders (a, ka), (b, kb) by length without branches.
+__device__ __forceinline__ void sort2(float &a, float &b, uint32_t &ka, uint32_t &kb) {
+    const bool sw = a > b;
+    const float lo = sw ? b : a, hi = sw ? a : b;
+    const uint32_t klo = sw ? kb : ka, khi = sw ? ka : kb;
+    a = lo;
+    b = hi;
+    ka = klo;
+    kb = khi;
+}
+
+// Cells of the theta hint table over the cosine-rule value c in [-1, 1].
+constexpr int kCosCells = 2048;
+
+// Bin hint of the cosine-rule value c: the table holds, per cell of
+// [-1, 1], 1 + the bin of the cell's upper end (-1 above the range, nt below
+// it); with at most one edge in the cell, c is in that bin or the next, and
+// one comparison with the next edge decides.  Only a hint: bracket_desc
+// confirms it.
+__device__ __forceinline__ int cos_hint(float c, const uint8_t *tcos, const float *fA, int nt) {
+    const int idx = __builtin_amdgcn_fmed3f(0.f, (c + 1.f) * (0.5f * kCosCells), (float)(kCosCells - 1));
+    const int m0 = (int)tcos[idx] - 1;
+    const float next = fA[min(m0 + 1, nt)];
+    return m0 + (int)((m0 < nt) & (c <= next));
+}
+
+// One entry in FP32 from sides a <= b (rounded FP32 lengths of the FP64
+// sides, each within u of it), ra = 1/a and rden = 1/den from v_rcp_f32
+// (1 ulp = at most 2u).  Error bounds against the FP64 definition:
+//   q = b * ra: within (2u sides + 3u quotient) q, 6u q used;
+//   c = num * rden with num = (sa + sb) - sc, den = 2 a b (squares within 3u):
+//   |c - c64| <= 4u (sa+sb+sc)/den + 7u |c|; 5u, 7u |c| and 2u for the
+//   table's rounding used.
+// The theta bin is hinted from the cosine table.  Returns the histogram
+// cell, -1 for "out of range" (certain), and sets *unsure when some value
+// lies within its error bound of a deciding edge.
+__device__ inline int entry32(int sbin, float b, float ra, float num, float rden, float ssum, const float *fQ,
+                                  const float *fA, const uint8_t *tcos, CTerm &T, bool *unsure) {
+    const int nq = T.nq, nt = T.nt;
+    const float q = b * ra;
+    const int mq = bracket_asc(q, 6.f * kU * q, fQ, nq, guess_bin(q, T.q_lo, T.q_inv, T.q_log, nq), unsure);
+    const float c = num * rden;
+    const float dc = 5.f * kU * ssum * rden + 7.f * kU * fabsf(c) + 2.f * kU;
+    const int mt = bracket_desc(c, dc, fA, nt, cos_hint(c, tcos, fA, nt), unsure);
+    return ((mq < 0) | (mt < 0)) ? -1 : (int)__umul24(__umul24((uint32_t)sbin, nq) + mq, nt) + mt;
+}
+
+// Histogram entries of a gathered candidate (FP32 with FP64 recheck), for
+// PARAM = kSqt or kJbj.
+template <int PARAM>
+__device__ inline void sqt_entries(CTerm &T, const SqtItemCtx &X, const SqtPending &p, const double *sT,
+                                   const double *sQ, const double *sA, const float *fQ, const float *fA,
+                                   const uint8_t *tcos, uint32_t *hist, unsigned long long &nre) {
     if (!p.live) return;
     // Sort the sides by FP32 length, carrying their codes.
     float l1 = p.xd, l2 = p.yd, l3 = p.zd;
     uint32_t k1 = p.xc, k2 = p.yc, k3 = p.zc;
-    if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
-    if (l2 > l3) { float t = l2; l2 = l3; l3 = t; uint32_t u = k2; k2 = k3; k3 = u; }
-    if (l1 > l2) { float t = l1; l1 = l2; l2 = t; uint32_t u = k1; k1 = k2; k2 = u; }
+    sort2(l1, l2, k1, k2);
+    sort2(l2, l3, k2, k3);
+    sort2(l1, l2, k1, k2);
     const int nq = T.nq, nt = T.nt;
     bool unsure = T.force_fp64 != 0;
     // Side order certain only when lengths differ by more than their FP32
@@ -334,7 +351,7 @@ __device__ inline void sqt_entries(CTerm &T, const SqtItemCtx &X, const SqtPendi
     // underflow or overflow below).
     if (!(l2 - l1 > 4.f * kU * l2) || !(l3 - l2 > 4.f * kU * l3) || !(l1 > 1e-18f) || !(l3 < 1e18f)) unsure = true;
     int cell[3] = {-1, -1, -1};
-    if (!unsure && T.param == kJbj) {
+    if (!unsure && PARAM == kJbj) {
         // One entry: the shortest side's s bin, q = l1/l2 and v =
         // (l3 - l2)/l1.  With every FP32 side within u of the FP64 one:
         // q = l1 * (1/l2) is within (2u + 3u) q (6u q used); the FP32
@@ -349,21 +366,21 @@ __device__ inline void sqt_entries(CTerm &T, const SqtItemCtx &X, const SqtPendi
             const float v = (l3 - l2) * r1;
             const float dv = 2.f * kU * (l2 + l3) * r1 + 6.f * kU * fabsf(v);
             const int mv = bracket_asc(v, dv, fA, nt, guess_bin(v, T.t_lo, T.t_inv, T.t_log, nt), &unsure);
-            if (mq >= 0 && mv >= 0) cell[0] = ((int)k1 * nq + mq) * nt + mv;
+            if (mq >= 0 && mv >= 0) cell[0] = (int)__umul24(__umul24(k1, nq) + mq, nt) + mv;
         }
     } else if (!unsure) {
         const float s1 = l1 * l1, s2 = l2 * l2, s3 = l3 * l3;
         const float ssum = (s1 + s2) + s3;
         if (k1 < kCodeNear) {
             const float r1 = __builtin_amdgcn_rcpf(l1);
-            cell[0] = entry32((int)k1, l2, r1, (s1 + s2) - s3, __builtin_amdgcn_rcpf((2.f * l1) * l2), ssum, fQ, fA,
-                              T, &unsure);
-            cell[1] = entry32((int)k1, l3, r1, (s1 + s3) - s2, __builtin_amdgcn_rcpf((2.f * l1) * l3), ssum, fQ, fA,
-                              T, &unsure);
+            cell[0] = entry32((int)k1, l2, r1, (s1 + s2) - s3, __builtin_amdgcn_rcpf((2.f * l1) * l2), ssum, fQ,
+                                  fA, tcos, T, &unsure);
+            cell[1] = entry32((int)k1, l3, r1, (s1 + s3) - s2, __builtin_amdgcn_rcpf((2.f * l1) * l3), ssum, fQ,
+                                  fA, tcos, T, &unsure);
         }
         if (k2 < kCodeNear)
             cell[2] = entry32((int)k2, l3, __builtin_amdgcn_rcpf(l2), (s2 + s3) - s1,
-                              __builtin_amdgcn_rcpf((2.f * l2) * l3), ssum, fQ, fA, T, &unsure);
+                                  __builtin_amdgcn_rcpf((2.f * l2) * l3), ssum, fQ, fA, tcos, T, &unsure);
     }
     if (!unsure) {
 #pragma unroll
@@ -372,11 +389,11 @@ __device__ inline void sqt_entries(CTerm &T, const SqtItemCtx &X, const SqtPendi
     } else {
         ++nre;
         const uint32_t e = p.e;
-        const int i = e & 63, j = (e >> 6) & 63, k = (e >> 12) & 63, C = X.c0 + (int)(e >> 18);
+        const int k = e & 63, j = (e >> 6) & 63, i = (e >> 12) & 63, C = X.c0 + (int)(e >> 18);
         const int64_t gi = (int64_t)X.A * kBlock + i, gj = (int64_t)X.B * kBlock + j, gk = (int64_t)C * kBlock + k;
         const double sij = sq64f(T.u64, T.npad, gi, gj), sik = sq64f(T.u64, T.npad, gi, gk),
                      sjk = sq64f(T.u64, T.npad, gj, gk);
-        if (T.param == kJbj)
+        if (PARAM == kJbj)
             entry_jbj_fp64(sij, sik, sjk, sT, sQ, sA, T.ns, nq, nt, hist);
         else
             entries_fp64(sij, sik, sjk, sT, sQ, sA, T.ns, nq, nt, hist);
@@ -412,6 +429,105 @@ __device__ inline unsigned long long row_candidates(unsigned long long xn, unsig
     return xn & zn & shortx & mid2;
 }
 
+namespace {
+
+// LDS tables of the current term and the (A,B) row masks of the current item.
+struct SqtShared {
+    double *sT, *sQ, *sA;
+    float *fQ, *fA;
+    uint8_t *tcos;
+    const unsigned long long *xnear, *xshort, *xmid;
+};
+
+// The candidates of one item (term, A, B, C-range) for PARAM, found with the
+// pair stage's bit masks and appended to this wave's ring (Q, qcap words),
+// drained 64 at a time.
+template <int PARAM, int WAVES>
+__device__ inline void sqt_item(CTerm &T, const SqtItemCtx &X, int c1, const SqtShared &S, uint32_t *hist,
+                                uint32_t *Q, uint32_t qcap, int lane, int wave, unsigned long long &nre) {
+    const uint32_t qmask = qcap - 1u;
+    const int64_t nblk = T.nblk;
+    // This wave's ring of queued candidates: entries [qhead, qtail)
+    // (indices mod qcap).
+    uint32_t qhead = 0u, qtail = 0u;
+    // The last 64 queued entries, their sides in flight: their entries are
+    // computed at the next drain (or the item's end), so the loads overlap
+    // the scan in between.
+    SqtPending pend;
+    pend.live = false;
+    auto drain_ready = [&]() {
+        while (qtail - qhead >= (uint32_t)kBlock) {
+            wave_sync();
+            sqt_entries<PARAM>(T, X, pend, S.sT, S.sQ, S.sA, S.fQ, S.fA, S.tcos, hist, nre);
+            pend = sqt_gather(X, Q, qhead, qmask, kBlock, lane);
+            qhead += kBlock;
+        }
+    };
+    for (int C = X.c0 + wave; C < c1; C += WAVES) {
+        // Lane k's column of tiles (B,C) and (A,C): bits j of side (j,k)
+        // and bits i of side (i,k).
+        if (!(T.tile_any[tile_index(X.A, C, nblk)] & T.tile_any[tile_index(X.B, C, nblk)])) continue;
+        const unsigned long long *mBC = T.masks + tile_index(X.B, C, nblk) * kMaskWords;
+        const unsigned long long zn = mBC[kMaskColNear + lane], zs = mBC[kMaskColShort + lane],
+                                 zm = mBC[kMaskColMid + lane];
+        if (__ballot(zn != 0ull) == 0ull) continue;
+        const unsigned long long *mAC = T.masks + tile_index(X.A, C, nblk) * kMaskWords;
+        const unsigned long long yn = mAC[kMaskColNear + lane], ys = mAC[kMaskColShort + lane],
+                                 ym = mAC[kMaskColMid + lane];
+        const uint32_t lbase = (uint32_t)lane | (uint32_t)(C - X.c0) << 18;
+        for (int i = 0; i < kBlock; ++i) {
+            const unsigned long long xn = S.xnear[i];
+            if (xn == 0ull) continue;
+            unsigned long long m = row_candidates(xn, S.xshort[i], S.xmid[i], i, yn, ys, ym, zn, zs, zm);
+            const uint32_t cnt = (uint32_t)__popcll(m);
+            const uint32_t incl = wave_inclusive_sum(cnt, lane);
+            const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+            if (total == 0u) continue;
+            const uint32_t base = lbase | (uint32_t)i << 12;
+            if (qtail - qhead + total <= qcap) {
+                // Compaction: every lane writes its candidates at its offset
+                // in the row's block of the ring.
+                uint32_t pos = qtail + incl - cnt;
+                while (m != 0ull) {
+                    const int j = __builtin_ctzll(m);
+                    m &= m - 1ull;
+                    Q[pos & qmask] = base | (uint32_t)j << 6;
+                    ++pos;
+                }
+                qtail += total;
+            } else {
+                // The row does not fit: one entry per lane per round,
+                // draining in between.
+                for (;;) {
+                    const bool has = m != 0ull;
+                    const unsigned long long act = __ballot(has);
+                    if (act == 0ull) break;
+                    if (has) {
+                        const int j = __builtin_ctzll(m);
+                        m &= m - 1ull;
+                        const uint32_t slot = qtail + __builtin_amdgcn_mbcnt_hi(
+                                                          (uint32_t)(act >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+                        Q[slot & qmask] = base | (uint32_t)j << 6;
+                    }
+                    qtail += (uint32_t)__popcll(act);
+                    drain_ready();
+                }
+            }
+            drain_ready();
+        }
+    }
+    sqt_entries<PARAM>(T, X, pend, S.sT, S.sQ, S.sA, S.fQ, S.fA, S.tcos, hist, nre);
+    if (qtail != qhead) {
+        wave_sync();
+        sqt_entries<PARAM>(T, X, sqt_gather(X, Q, qhead, qmask, (int)(qtail - qhead), lane), S.sT, S.sQ, S.sA, S.fQ,
+                           S.fA, S.tcos, hist, nre);
+        wave_sync();
+    }
+}
+
+}  // namespace
+
 // Persistent over (term, A, B, C-range) items.  Candidate triangles (no far
 // or zero side, at least one side with an s bin) are found with the pair
 // stage's bit masks: per item, for every row i of block A, the columns j whose
@@ -421,18 +537,22 @@ __device__ inline unsigned long long row_candidates(unsigned long long xn, unsig
 //   X_near[i] & Z_near & Y_near(i) & (X_short[i] | Z_short | Y_short(i)).
 // Each wave takes the C blocks c0 + wave, c0 + wave + WAVES, ... and appends
 // its candidates to an LDS queue, drained 64 at a time with every lane busy.
+// The per-item work is compiled once per parametrization ((s,q,theta) and
+// JBJ jobs may share a launch).
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
     extern __shared__ uint32_t hist[];
     __shared__ double sT[kMaxBins + 1], sQ[kMaxBins + 1], sA[kMaxBins + 1];
     __shared__ float fQ[kMaxBins + 1], fA[kMaxBins + 1];
+    __shared__ uint8_t tcos[kCosCells];
     __shared__ unsigned long long xnear[kBlock], xshort[kBlock], xmid[kBlock];
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int c = threadIdx.x; c < P.lds_cells; c += blockDim.x) hist[c] = 0u;
-    const uint32_t qcap = (uint32_t)P.qcap, qmask = qcap - 1u;
+    const uint32_t qcap = (uint32_t)P.qcap;
     uint32_t *Q = hist + ((P.lds_cells + 3) & ~3) + wave * qcap;
+    const SqtShared S{sT, sQ, sA, fQ, fA, tcos, xnear, xshort, xmid};
     // The current term (its job's tables in LDS); flushing writes and clears
     // the cells the term used, so the next term starts from zero.
     int cur = -1;
@@ -471,6 +591,29 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
                     }
                 }
                 __syncthreads();
+                // theta hint table: 1 + the bin (descending cosine edges fA)
+                // of each cell's upper end
+                if (T.param == kSqt) {
+                    const int nt = T.nt;
+                    for (int c = threadIdx.x; c < kCosCells; c += blockDim.x) {
+                        const float v = -1.f + (float)(c + 1) * (2.f / kCosCells);
+                        int m;
+                        if (v > fA[0]) {
+                            m = -1;
+                        } else if (v <= fA[nt]) {
+                            m = nt;
+                        } else {
+                            int lo = 0, hi = nt;  // fA[lo] >= v > fA[hi]
+                            while (hi - lo > 1) {
+                                const int mid = (lo + hi) >> 1;
+                                if (v <= fA[mid]) lo = mid;
+                                else hi = mid;
+                            }
+                            m = lo;
+                        }
+                        tcos[c] = (uint8_t)(m + 1);
+                    }
+                }
             }
             cur = term;
             since = 0;
@@ -478,12 +621,18 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         since += add;
         CTerm &T = *Tp;
         const int64_t nblk = T.nblk;
-        const int64_t tAB = tile_index(X.A, X.B, nblk) * kTileBytes;
-        X.cAB = T.codes + tAB;
-        X.dAB = T.dists + tAB;
         // No candidate in the item when its (A, B) tile has no near pair
         // (or was not computed).
         if (!T.tile_any[tile_index(X.A, X.B, nblk)]) continue;
+        const int64_t tAB = tile_index(X.A, X.B, nblk) * kTileBytes;
+        const int64_t tAC = tile_index(X.A, X.c0, nblk) * kTileBytes;
+        const int64_t tBC = tile_index(X.B, X.c0, nblk) * kTileBytes;
+        X.cAB = T.codes + tAB;
+        X.cAC = T.codes + tAC;
+        X.cBC = T.codes + tBC;
+        X.dAB = T.dists + tAB;
+        X.dAC = T.dists + tAC;
+        X.dBC = T.dists + tBC;
         const unsigned long long *mAB = T.masks + tile_index(X.A, X.B, nblk) * kMaskWords;
         for (int i = threadIdx.x; i < kBlock; i += blockDim.x) {
             xnear[i] = mAB[kMaskRowNear + i];
@@ -491,83 +640,10 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
             xmid[i] = mAB[kMaskRowMid + i];
         }
         __syncthreads();
-        // This wave's ring of queued candidates: entries [qhead, qtail)
-        // (indices mod qcap); each packs i | j << 6 | k << 12 | (C - c0) << 18.
-        uint32_t qhead = 0u, qtail = 0u;
-        // The last 64 queued entries, their sides in flight: their entries
-        // are computed at the next drain (or the item's end), so the loads
-        // overlap the scan in between.
-        SqtPending pend;
-        pend.live = false;
-        auto drain_ready = [&]() {
-            while (qtail - qhead >= (uint32_t)kBlock) {
-                wave_sync();
-                sqt_entries(T, X, pend, sT, sQ, sA, fQ, fA, hist, nre);
-                pend = sqt_gather(T, X, Q, qhead, qmask, kBlock, lane);
-                qhead += kBlock;
-            }
-        };
-        for (int C = X.c0 + wave; C < c1; C += WAVES) {
-            // Lane k's column of tiles (B,C) and (A,C): bits j of side (j,k)
-            // and bits i of side (i,k).
-            if (!(T.tile_any[tile_index(X.A, C, nblk)] & T.tile_any[tile_index(X.B, C, nblk)])) continue;
-            const unsigned long long *mBC = T.masks + tile_index(X.B, C, nblk) * kMaskWords;
-            const unsigned long long zn = mBC[kMaskColNear + lane], zs = mBC[kMaskColShort + lane],
-                                     zm = mBC[kMaskColMid + lane];
-            if (__ballot(zn != 0ull) == 0ull) continue;
-            const unsigned long long *mAC = T.masks + tile_index(X.A, C, nblk) * kMaskWords;
-            const unsigned long long yn = mAC[kMaskColNear + lane], ys = mAC[kMaskColShort + lane],
-                                     ym = mAC[kMaskColMid + lane];
-            const uint32_t coff = (uint32_t)(C - X.c0) << 18;
-            for (int i = 0; i < kBlock; ++i) {
-                const unsigned long long xn = xnear[i];
-                if (xn == 0ull) continue;
-                unsigned long long m = row_candidates(xn, xshort[i], xmid[i], i, yn, ys, ym, zn, zs, zm);
-                const uint32_t cnt = (uint32_t)__popcll(m);
-                const uint32_t incl = wave_inclusive_sum(cnt, lane);
-                const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-                if (total == 0u) continue;
-                const uint32_t base = (uint32_t)i | (uint32_t)lane << 12 | coff;
-                if (qtail - qhead + total <= qcap) {
-                    // Compaction: every lane writes its candidates at its
-                    // offset in the row's block of the ring.
-                    uint32_t pos = qtail + incl - cnt;
-                    while (m != 0ull) {
-                        const int j = __builtin_ctzll(m);
-                        m &= m - 1ull;
-                        Q[pos & qmask] = base | (uint32_t)j << 6;
-                        ++pos;
-                    }
-                    qtail += total;
-                } else {
-                    // The row does not fit: one entry per lane per round,
-                    // draining in between.
-                    for (;;) {
-                        const bool has = m != 0ull;
-                        const unsigned long long act = __ballot(has);
-                        if (act == 0ull) break;
-                        if (has) {
-                            const int j = __builtin_ctzll(m);
-                            m &= m - 1ull;
-                            const uint32_t slot = qtail + __builtin_amdgcn_mbcnt_hi(
-                                                              (uint32_t)(act >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
-                            Q[slot & qmask] = base | (uint32_t)j << 6;
-                        }
-                        qtail += (uint32_t)__popcll(act);
-                        drain_ready();
-                    }
-                }
-                drain_ready();
-            }
-        }
-        sqt_entries(T, X, pend, sT, sQ, sA, fQ, fA, hist, nre);
-        if (qtail != qhead) {
-            wave_sync();
-            sqt_entries(T, X, sqt_gather(T, X, Q, qhead, qmask, (int)(qtail - qhead), lane), sT, sQ, sA, fQ, fA,
-                        hist, nre);
-            wave_sync();
-        }
+        if (T.param == kJbj)
+            sqt_item<kJbj, WAVES>(T, X, c1, S, hist, Q, qcap, lane, wave, nre);
+        else
+            sqt_item<kSqt, WAVES>(T, X, c1, S, hist, Q, qcap, lane, wave, nre);
     }
     if (cur >= 0) {
         sqt_flush64(hist, Tp->cells, Tp->hist);

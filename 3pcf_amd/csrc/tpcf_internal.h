@@ -138,9 +138,10 @@ struct SqtTerm {
     int param;  // kSqt or kJbj
     int ns, nq, nt, cells;
     int force_fp64;  // every candidate triangle through the FP64 definition
-    // Bin guesses of the q and theta/pi axes: lin (v - lo) * inv, log
+    // Bin guesses of the q axis and (JBJ) the v axis: lin (v - lo) * inv, log
     // (log2(v) - lo) * inv with lo = log2(axis lo); only a hint, every bin is
-    // confirmed against the edge tables.
+    // confirmed against the edge tables.  (The (s,q,theta) theta bin is
+    // hinted from a table of the cosine edges the kernel builds in LDS.)
     float q_lo, q_inv, t_lo, t_inv;
     int q_log, t_log;
 };
