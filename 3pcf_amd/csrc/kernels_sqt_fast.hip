@@ -310,23 +310,24 @@ __device__ __forceinline__ int cos_hint(float c, const uint8_t *tcos, const floa
     return m0 + (int)((m0 < nt) & (c <= next));
 }
 
-// One entry in FP32 from sides a <= b (rounded FP32 lengths of the FP64
-// sides, each within u of it), ra = 1/a and rden = 1/den from v_rcp_f32
-// (1 ulp = at most 2u).  Error bounds against the FP64 definition:
-//   q = b * ra: within (2u sides + 3u quotient) q, 6u q used;
-//   c = num * rden with num = (sa + sb) - sc, den = 2 a b (squares within 3u):
-//   |c - c64| <= 4u (sa+sb+sc)/den + 7u |c|; 5u, 7u |c| and 2u for the
-//   table's rounding used.
-// The theta bin is hinted from the cosine table.  Returns the histogram
-// cell, -1 for "out of range" (certain), and sets *unsure when some value
-// lies within its error bound of a deciding edge.
-__device__ inline int entry32(int sbin, float b, float ra, float num, float rden, float ssum, const float *fQ,
-                                  const float *fA, const uint8_t *tcos, CTerm &T, bool *unsure) {
+// One entry in FP32 from q = b * (1/a) and c = num * rden, with a <= b the
+// FP32 lengths (each within u of the FP64 side), 1/a from v_rcp_f32 (1 ulp =
+// at most 2u), num = (sa + sb) - sc and rden = (0.5 * (1/a)) * (1/b), two
+// v_rcp_f32 values and a rounding: within 5u of 1/(2ab).  Error bounds
+// against the FP64 definition:
+//   q: within (2u sides + 3u quotient) q, 6u q used;
+//   c: the squares are within 3u, so num is within 4u (sa+sb+sc) + u |num|
+//   of the FP64 value, and 2ab within 2u: |c - c64| <= 4u (sa+sb+sc)/(2ab)
+//   + 9u |c|; 5u (sa+sb+sc) rden, 10u |c| and 2u for the table's rounding
+//   used.
+// Returns the histogram cell, -1 for "out of range" (certain), and sets
+// *unsure when some value lies within its error bound of a deciding edge.
+__device__ inline int entry32(int sbin, float q, float num, float rden, float ssum, const float *fQ,
+                              const float *fA, const uint8_t *tcos, CTerm &T, bool *unsure) {
     const int nq = T.nq, nt = T.nt;
-    const float q = b * ra;
     const int mq = bracket_asc(q, 6.f * kU * q, fQ, nq, guess_bin(q, T.q_lo, T.q_inv, T.q_log, nq), unsure);
     const float c = num * rden;
-    const float dc = 5.f * kU * ssum * rden + 7.f * kU * fabsf(c) + 2.f * kU;
+    const float dc = 5.f * kU * ssum * rden + 10.f * kU * fabsf(c) + 2.f * kU;
     const int mt = bracket_desc(c, dc, fA, nt, cos_hint(c, tcos, fA, nt), unsure);
     return ((mq < 0) | (mt < 0)) ? -1 : (int)__umul24(__umul24((uint32_t)sbin, nq) + mq, nt) + mt;
 }
@@ -371,16 +372,15 @@ __device__ inline void sqt_entries(CTerm &T, const SqtItemCtx &X, const SqtPendi
     } else if (!unsure) {
         const float s1 = l1 * l1, s2 = l2 * l2, s3 = l3 * l3;
         const float ssum = (s1 + s2) + s3;
+        // three reciprocals serve the three entries' q and 1/(2ab)
+        const float r1 = __builtin_amdgcn_rcpf(l1), r2 = __builtin_amdgcn_rcpf(l2), r3 = __builtin_amdgcn_rcpf(l3);
+        const float h1 = 0.5f * r1, h2 = 0.5f * r2;
         if (k1 < kCodeNear) {
-            const float r1 = __builtin_amdgcn_rcpf(l1);
-            cell[0] = entry32((int)k1, l2, r1, (s1 + s2) - s3, __builtin_amdgcn_rcpf((2.f * l1) * l2), ssum, fQ,
-                                  fA, tcos, T, &unsure);
-            cell[1] = entry32((int)k1, l3, r1, (s1 + s3) - s2, __builtin_amdgcn_rcpf((2.f * l1) * l3), ssum, fQ,
-                                  fA, tcos, T, &unsure);
+            cell[0] = entry32((int)k1, l2 * r1, (s1 + s2) - s3, h1 * r2, ssum, fQ, fA, tcos, T, &unsure);
+            cell[1] = entry32((int)k1, l3 * r1, (s1 + s3) - s2, h1 * r3, ssum, fQ, fA, tcos, T, &unsure);
         }
         if (k2 < kCodeNear)
-            cell[2] = entry32((int)k2, l3, __builtin_amdgcn_rcpf(l2), (s2 + s3) - s1,
-                                  __builtin_amdgcn_rcpf((2.f * l2) * l3), ssum, fQ, fA, tcos, T, &unsure);
+            cell[2] = entry32((int)k2, l3 * r2, (s2 + s3) - s1, h2 * r3, ssum, fQ, fA, tcos, T, &unsure);
     }
     if (!unsure) {
 #pragma unroll

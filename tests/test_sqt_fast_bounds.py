@@ -128,17 +128,18 @@ def _entries(points, axes, rng, qscale=1, cscale=1):
     fA = Et.astype(np.float32)
     s1, s2, s3 = l1 * l1, l2 * l2, l3 * l3
     ssum = (s1 + s2) + s3
-    def entry32(sb, b, ra, num, rden):
-        q = b * ra
+    def entry32(sb, q, num, rden):
         mq, okq = _bracket_asc32(q, F(6) * U * q, fQ, _fp32_hint_asc(q, fQ), qscale)
         c = num * rden
-        dc = (F(5) * U * ssum * rden + F(7) * U * np.abs(c) + F(2) * U) * F(cscale)
+        dc = (F(5) * U * ssum * rden + F(10) * U * np.abs(c) + F(2) * U) * F(cscale)
         mt, okt = _bracket_desc32(c, dc, fA, _fp32_hint_desc(c, fA))
         return cell(sb, mq, mt), okq & okt
-    r1 = _faithful_rcp(l1, rng)
-    e0, ok0 = entry32(k[:, 0], l2, r1, (s1 + s2) - s3, _faithful_rcp((F(2) * l1) * l2, rng))
-    e1, ok1 = entry32(k[:, 0], l3, r1, (s1 + s3) - s2, _faithful_rcp((F(2) * l1) * l3, rng))
-    e2, ok2 = entry32(k[:, 1], l3, _faithful_rcp(l2, rng), (s2 + s3) - s1, _faithful_rcp((F(2) * l2) * l3, rng))
+    # three reciprocals; 1/(2ab) = (0.5 * (1/a)) * (1/b)
+    r1, r2, r3 = _faithful_rcp(l1, rng), _faithful_rcp(l2, rng), _faithful_rcp(l3, rng)
+    h1, h2 = F(0.5) * r1, F(0.5) * r2
+    e0, ok0 = entry32(k[:, 0], l2 * r1, (s1 + s2) - s3, h1 * r2)
+    e1, ok1 = entry32(k[:, 0], l3 * r1, (s1 + s3) - s2, h1 * r3)
+    e2, ok2 = entry32(k[:, 1], l3 * r2, (s2 + s3) - s1, h2 * r3)
     # Entries without an s bin are not computed (cell -1), their values do
     # not make the triangle unsure.
     has0 = k[:, 0] >= 0
