@@ -247,7 +247,8 @@ __device__ __forceinline__ void r123_column(uint32_t *hist, uint32_t &w, int j, 
 // adds one triplet per valid column, the invalid ones skipped by a scalar
 // test.
 template <int WAVES, int COPIES>
-__global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletItem *__restrict__ items,
+__global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticCtl ctl,
+                                                                  const TripletItem *__restrict__ items,
                                                                   int nitems, int *__restrict__ counter,
                                                                   const TripletTerm *__restrict__ terms,
                                                                   int lds_words) {
@@ -258,6 +259,8 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
     // Clear the whole allocation once; every flush clears what its term used,
     // so terms of any bin count start from zero.
     for (int c = threadIdx.x; c < lds_words; c += blockDim.x) hist[c] = 0u;
+    if (threadIdx.x == 0 && ctl.active) atomicAdd(ctl.active, 1);
+    bool left = false;
     int cur = -1, cur_n = 0;
     unsigned long long since_flush = 0;
     unsigned long long *cur_hist = nullptr;
@@ -266,7 +269,10 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
     const unsigned long long flush_limit = 0xC0000000ull;
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) {
+            left = elastic_leave(ctl);
+            s_item = left ? nitems : atomicAdd(counter, 1);
+        }
         __syncthreads();
         const int it = __builtin_amdgcn_readfirstlane(s_item);
         if (it >= nitems) break;
@@ -388,6 +394,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const TripletI
         }
     }
     if (cur >= 0) r123_flush<COPIES>(hist, cur_n, cur_hist);
+    if (threadIdx.x == 0 && ctl.active && !left) atomicSub(ctl.active, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -545,14 +552,15 @@ const void *r123_triplet_kernel_ptr(int waves, int copies) {
 
 int r123_lds_bytes(int n, int copies) { return copies * r123_copy_stride(n, copies) * 4; }
 
-hipError_t launch_r123_triplets(const TripletItem *items, int nitems, int *counter, const TripletTerm *terms,
+hipError_t launch_r123_triplets(const ElasticCtl &ctl, const TripletItem *items, int nitems, int *counter,
+                                const TripletTerm *terms,
                                 int max_n, int waves, int copies, int grid, hipStream_t st) {
     if (max_n < 1 || max_n > kMaxLdsR123Bins) return hipErrorInvalidValue;
     const int words = copies * r123_copy_stride(max_n, copies);
     const size_t lds = (size_t)words * 4;
 #define TPCF_LAUNCH(W, K)                                                                                    \
     if (waves == W && copies == K) {                                                                         \
-        hipLaunchKernelGGL((r123_triplet_kernel<W, K>), dim3(grid), dim3(W * 64), lds, st, items, nitems, counter, \
+        hipLaunchKernelGGL((r123_triplet_kernel<W, K>), dim3(grid), dim3(W * 64), lds, st, ctl, items, nitems, counter, \
                            terms, words);                                                                    \
         return hipGetLastError();                                                                            \
     }

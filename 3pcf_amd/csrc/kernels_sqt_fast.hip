@@ -558,9 +558,14 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
     int cur = -1;
     CTerm *Tp = (CTerm *)(P.terms);
     unsigned long long since = 0, nre = 0;
+    if (threadIdx.x == 0 && P.ctl.active) atomicAdd(P.ctl.active, 1);
+    bool left = false;
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
+        if (threadIdx.x == 0) {
+            left = elastic_leave(P.ctl);
+            s_item = left ? P.nitems : atomicAdd(P.counter, 1);
+        }
         __syncthreads();
         const int it = __builtin_amdgcn_readfirstlane(s_item);
         if (it >= P.nitems) break;
@@ -649,6 +654,7 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         sqt_flush64(hist, Tp->cells, Tp->hist);
         if (nre) atomicAdd(Tp->rechecked, nre);
     }
+    if (threadIdx.x == 0 && P.ctl.active && !left) atomicSub(P.ctl.active, 1);
 }
 
 hipError_t launch_sqt_pairs(const SqtPairJob &J, hipStream_t st) {

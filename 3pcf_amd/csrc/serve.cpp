@@ -20,6 +20,7 @@
 // PLAIN-mode jobs (the FP64 baseline) run one per launch.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 
 #include "tpcf_host.h"
@@ -33,6 +34,7 @@ struct Ticket {
     uint64_t *hist = nullptr;
     int64_t footprint = 0;
     uint64_t work = 0;  // unique triplets
+    bool r123 = false;  // batches hold (r1,r2,r3) jobs or triangle-parametrization jobs
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
     int rc = TPCF_OK;
     std::string err;
@@ -44,6 +46,232 @@ struct Ticket {
 static bool solo(const Ticket &t) { return t.mode == TPCF_MODE_PLAIN; }
 
 static constexpr int kWorkers = 3;
+
+// A stream with a hardware queue of its own.  HIP maps ordinary streams
+// onto a small pool of hardware queues (four by default) shared round robin,
+// and kernels of streams that share a queue run one after the other; a
+// stream with a CU mask gets a queue of its own, so the serving streams use
+// a mask of every CU.  Such streams synchronize with the null stream.
+// Falls back to an ordinary non-blocking stream.
+static hipError_t queue_stream(int num_cu, hipStream_t *st) {
+    std::vector<uint32_t> mask((num_cu + 31) / 32, 0u);
+    for (int c = 0; c < num_cu; ++c) mask[c / 32] |= 1u << (c % 32);
+    if (hipExtStreamCreateWithCUMask(st, (uint32_t)mask.size(), mask.data()) == hipSuccess) return hipSuccess;
+    (void)hipGetLastError();
+    return hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+}
+
+// Device shares of the workers' persistent launches.  A persistent launch
+// fills every CU, so a second worker's kernels would wait for the first
+// launch to end; instead each batch holds a share of the CUs.  Per worker
+// slot the launch's workgroups read their desired number from pinned host
+// memory before every claim and count themselves in device memory
+// (ElasticCtl): lowering the desired number makes the surplus workgroups
+// leave at their next claim; raising it comes with more workgroups of the
+// same launch over the same work queue, enqueued on an idle helper stream
+// (one of a small pool of streams with hardware queues of their own).
+//
+// Shares, shortest work first: every claimed batch keeps a floor (1/16 of
+// an equal split, so a long batch keeps progressing), then the batches in
+// order of their work (unique triplets) take what they can use of the rest:
+// the batch with the least work gets the device and finishes first, and two
+// large batches mostly run one after the other instead of dividing the CUs
+// and the L2s between them.  A launch can use no more CUs than its items
+// need; a batch still staging (host planning, uploads, pair stages) asks for
+// an eighth of the device.
+struct ElasticScheduler {
+    struct Slot {
+        bool claimed = false;   // a batch is staging or running
+        bool launched = false;  // its persistent launch is enqueued
+        double work = 1.0;      // unique triplets of the batch
+        int per_cu = 1, nitems = 0, share = 0;
+        ElasticHook::Relaunch relaunch;
+        hipEvent_t ready = nullptr;   // the launch's inputs are in place
+        std::vector<hipEvent_t> ext;  // after each extra launch (reused)
+        size_t next_ext = 0;
+    };
+    static constexpr int kMaxHelpers = 8;
+    int num_cu = 0;
+    int *desired = nullptr;  // pinned host memory, one per slot
+    int *active = nullptr;   // device memory, one per slot
+    bool ok = false;         // false: set-up failed, launches keep their full grids
+    Slot slots[kWorkers];
+    std::vector<hipStream_t> helpers;
+    std::mutex mu;
+
+    explicit ElasticScheduler(const tpcf_ctx *ctx) : num_cu(ctx->num_cu) {
+        if (hipSetDevice(ctx->device) != hipSuccess) return;
+        if (hipHostMalloc(reinterpret_cast<void **>(&desired), sizeof(int) * kWorkers, hipHostMallocCoherent) !=
+                hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&active), sizeof(int) * kWorkers) != hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
+        for (Slot &s : slots)
+            if (hipEventCreateWithFlags(&s.ready, hipEventDisableTiming) != hipSuccess) {
+                (void)hipGetLastError();
+                return;
+            }
+        ok = true;
+    }
+    ~ElasticScheduler() {
+        for (hipStream_t h : helpers) (void)hipStreamDestroy(h);
+        for (Slot &s : slots) {
+            if (s.ready) (void)hipEventDestroy(s.ready);
+            for (hipEvent_t e : s.ext) (void)hipEventDestroy(e);
+        }
+        if (active) (void)hipFree(active);
+        if (desired) (void)hipHostFree(desired);
+    }
+    ElasticCtl ctl(int w) { return ElasticCtl{active + w, desired + w}; }
+    void set_desired(int w, int v) { __atomic_store_n(desired + w, v, __ATOMIC_RELEASE); }
+
+    // CUs of every claimed batch (under mu): floor, then by work.
+    void plan(double cus[kWorkers]) const {
+        int k = 0;
+        for (const Slot &s : slots) k += s.claimed;
+        double demand[kWorkers], left = num_cu;
+        int order[kWorkers], n = 0;
+        for (int w = 0; w < kWorkers; ++w) {
+            const Slot &s = slots[w];
+            cus[w] = 0.0;
+            if (!s.claimed) continue;
+            demand[w] = s.launched ? std::min<double>(num_cu, std::ceil((double)s.nitems / s.per_cu)) : num_cu / 8.0;
+            cus[w] = std::min(demand[w], num_cu / (16.0 * k));
+            left -= cus[w];
+            order[n++] = w;
+        }
+        std::sort(order, order + n, [&](int a, int b) { return slots[a].work < slots[b].work; });
+        for (int i = 0; i < n && left > 0.0; ++i) {
+            const int w = order[i];
+            const double give = std::min(demand[w] - cus[w], left);
+            cus[w] += give;
+            left -= give;
+        }
+    }
+    int target(int w, const double cus[kWorkers], int cap) const {
+        const Slot &s = slots[w];
+        const int t = (int)(cus[w] * s.per_cu);
+        return std::max(1, std::min({t, s.nitems, cap}));
+    }
+    // A helper stream with nothing enqueued, or a new one (null: none free).
+    hipStream_t idle_helper() {
+        for (hipStream_t h : helpers)
+            if (hipStreamQuery(h) == hipSuccess) return h;
+        (void)hipGetLastError();
+        if ((int)helpers.size() >= kMaxHelpers) return nullptr;
+        hipStream_t h = nullptr;
+        if (queue_stream(num_cu, &h) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        helpers.push_back(h);
+        return h;
+    }
+    // Every running launch but skip's to its share (under mu): shrinking
+    // only lowers the desired number; growing raises it first, so the new
+    // workgroups do not leave at once, then enqueues them after the
+    // launch's inputs.
+    void rebalance(int skip) {
+        double cus[kWorkers];
+        plan(cus);
+        for (int w = 0; w < kWorkers; ++w) {
+            Slot &s = slots[w];
+            if (w == skip || !s.launched) continue;
+            const int t = target(w, cus, 1 << 30);
+            if (t < s.share) {
+                set_desired(w, t);
+                s.share = t;
+                continue;
+            }
+            if (t == s.share) continue;
+            hipStream_t h = idle_helper();
+            if (!h) continue;
+            if (s.next_ext == s.ext.size()) {
+                hipEvent_t e = nullptr;
+                if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                    (void)hipGetLastError();
+                    continue;
+                }
+                s.ext.push_back(e);
+            }
+            set_desired(w, t);
+            if (hipStreamWaitEvent(h, s.ready, 0) != hipSuccess || s.relaunch(t - s.share, h, ctl(w)) != TPCF_OK ||
+                hipEventRecord(s.ext[s.next_ext], h) != hipSuccess) {
+                // whatever was enqueued before the error runs over the same
+                // queue; the owner still waits for this helper's work
+                (void)hipGetLastError();
+                set_desired(w, s.share);
+                (void)hipStreamSynchronize(h);
+                continue;
+            }
+            ++s.next_ext;
+            s.share = t;
+        }
+    }
+};
+
+// One worker's view of the scheduler, handed to run_batch.
+struct WorkerShare final : ElasticHook {
+    ElasticScheduler &S;
+    int w;
+    WorkerShare(ElasticScheduler &s, int w_) : S(s), w(w_) {}
+    void claim(double work) override {
+        std::lock_guard<std::mutex> g(S.mu);
+        S.slots[w].claimed = true;
+        S.slots[w].work = std::max(work, 1e6);
+        S.rebalance(-1);
+    }
+    int control(int per_cu, int nitems, Relaunch relaunch, hipStream_t st, ElasticCtl *ctl, int *grid) override {
+        std::lock_guard<std::mutex> g(S.mu);
+        ElasticScheduler::Slot &s = S.slots[w];
+        if (!S.ok || s.launched || nitems < 1) return TPCF_OK;  // not elastic: the full grid
+        s.per_cu = std::max(1, per_cu);
+        s.nitems = nitems;
+        s.launched = true;
+        double cus[kWorkers];
+        S.plan(cus);
+        const int t = S.target(w, cus, *grid);
+        if (hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t *>(S.active + w), 1, 0u, st);
+            e != hipSuccess || (e = hipEventRecord(s.ready, st)) != hipSuccess) {
+            s.launched = false;
+            return set_error(TPCF_EDEVICE, "elastic launch set-up failed: %s", hipGetErrorString(e));
+        }
+        S.set_desired(w, t);
+        s.share = t;
+        s.relaunch = std::move(relaunch);
+        *ctl = S.ctl(w);
+        *grid = t;
+        S.rebalance(w);  // the others, now that this launch's demand is known
+        return TPCF_OK;
+    }
+    void finish() override {
+        // The launch's own workgroups have ended, but all of them may have
+        // left at a shrink with the rest of the queue to its extra
+        // workgroups: the slot keeps its share, and keeps growing, until
+        // every extra launch made for it has ended -- then the queue is
+        // drained, since the last active workgroup of a launch never leaves.
+        size_t waited = 0;
+        for (;;) {
+            std::vector<hipEvent_t> wait;
+            {
+                std::lock_guard<std::mutex> g(S.mu);
+                ElasticScheduler::Slot &s = S.slots[w];
+                if (waited == s.next_ext) {
+                    s.claimed = s.launched = false;
+                    s.share = 0;
+                    s.next_ext = 0;
+                    s.relaunch = nullptr;
+                    S.rebalance(-1);
+                    return;
+                }
+                wait.assign(s.ext.begin() + waited, s.ext.begin() + s.next_ext);
+                waited = s.next_ext;
+            }
+            for (hipEvent_t e : wait) (void)hipEventSynchronize(e);
+        }
+    }
+};
 static constexpr int kMaxBatch = 64;
 static constexpr double kStarveSeconds = 60.0;
 // A batch holds jobs of at most kBandFactor times its first job's work (or
@@ -63,8 +291,10 @@ static void account(tpcf_ctx *ctx) {
 static void worker_main(tpcf_ctx *ctx, int w) {
     (void)hipSetDevice(ctx->device);
     hipStream_t st = nullptr;
-    (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    (void)queue_stream(ctx->num_cu, &st);
     std::unique_ptr<Workspace, WorkspaceDeleter> ws(new_workspace());
+    workspace_order(ws.get(), st);
+    WorkerShare share(*ctx->elastic, w);
     for (;;) {
         std::vector<std::shared_ptr<Ticket>> batch;
         std::vector<int64_t> ids;
@@ -103,7 +333,7 @@ static void worker_main(tpcf_ctx *ctx, int w) {
                     const bool plain = solo(*t);
                     if (plain && !batch.empty()) continue;
                     if (!batch.empty() && solo(*batch[0])) break;
-                    if (!batch.empty() && t->mode != batch[0]->mode) continue;
+                    if (!batch.empty() && (t->mode != batch[0]->mode || t->r123 != batch[0]->r123)) continue;
                     if (!batch.empty() &&
                         (double)t->work > kBandFactor * std::max((double)batch[0]->work, kBandFloor))
                         continue;
@@ -163,7 +393,7 @@ static void worker_main(tpcf_ctx *ctx, int w) {
                 hists.push_back(t->hist);
             }
             const int rc = run_batch(ctx, *ws, jobs.data(), specs.data(), (int)jobs.size(), batch[0]->mode,
-                                     hists.data(), per.data(), nullptr, st);
+                                     hists.data(), per.data(), nullptr, st, &share);
             for (size_t k = 0; k < batch.size(); ++k) {
                 rcs[k] = rc;
                 if (rc) errs[k] = tpcf_last_error();
@@ -194,7 +424,10 @@ static void worker_main(tpcf_ctx *ctx, int w) {
         ctx->held[w] = 0;
         account(ctx);
     }
-    if (st) (void)hipStreamDestroy(st);
+    if (st) {
+        (void)hipStreamSynchronize(st);  // the workspace's stream-ordered frees
+        (void)hipStreamDestroy(st);
+    }
 }
 
 }  // namespace tpcf
@@ -217,13 +450,15 @@ int tpcf_submit(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, in
     t->hist = hist;
     t->footprint = fp;
     t->work = work;
-    t->t_submit = std::chrono::steady_clock::now();
+    t->r123 = spec->param == TPCF_R123;
     {
         std::lock_guard<std::mutex> g(ctx->mu);
         if (ctx->workers.empty()) {
             ctx->held.assign(kWorkers, 0);
+            ctx->elastic = std::make_shared<ElasticScheduler>(ctx);
             for (int w = 0; w < kWorkers; ++w) ctx->workers.emplace_back(worker_main, ctx, w);
         }
+        t->t_submit = std::chrono::steady_clock::now();  // after the first call's set-up
         const int64_t id = ctx->next_ticket++;
         ctx->tickets[id] = t;
         ctx->pending.push_back(id);

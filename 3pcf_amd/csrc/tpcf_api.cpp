@@ -38,13 +38,20 @@ int set_error(int code, const char *fmt, ...) {
 
 void clear_error() { g_err.clear(); }
 
+static void dev_free(void *p, hipStream_t ordered) {
+    if (ordered)
+        (void)hipFreeAsync(p, ordered);
+    else
+        (void)hipFree(p);
+}
+
 DevBuf::~DevBuf() {
-    if (p && !fixed) (void)hipFree(p);
+    if (p && !fixed) dev_free(p, ordered);
 }
 
 void DevBuf::release() {
     if (fixed) return;
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p, ordered);
     p = nullptr;
     bytes = 0;
 }
@@ -55,11 +62,11 @@ int DevBuf::reserve(size_t need) {
         return set_error(TPCF_ENOMEM, "the job needs %zu bytes in a part of the fixed scratch that holds %zu", need,
                          bytes);
     if (p) {
-        (void)hipFree(p);
+        dev_free(p, ordered);
         p = nullptr;
         bytes = 0;
     }
-    hipError_t e = hipMalloc(&p, need);
+    hipError_t e = ordered ? hipMallocAsync(&p, need, ordered) : hipMalloc(&p, need);
     if (e != hipSuccess) {
         p = nullptr;
         return set_error(TPCF_ENOMEM, "device allocation of %zu bytes failed: %s", need, hipGetErrorString(e));
@@ -618,6 +625,10 @@ struct Workspace {
     }
 };
 
+void workspace_order(Workspace *ws, hipStream_t st) {
+    ws->region.ordered = ws->queue.ordered = ws->sqt_queue.ordered = st;
+}
+
 // Launch shape of the LDS triplet kernel for bins up to max_n: as many
 // privatized histogram copies (4, 2 or 1) as fit the LDS budget, and enough
 // waves per workgroup for four waves per SIMD at the resulting occupancy.
@@ -709,7 +720,8 @@ static int upload_queue(DevBuf &buf, const Queue &Q, hipStream_t st, QueueDev &D
 
 // Reset the queue counter and launch the triplet kernel (kernels only, so
 // the sequence can be captured in a graph).
-static int launch_queue(tpcf_ctx *ctx, const QueueDev &D, const TripletShape &sh, hipStream_t st) {
+static int launch_queue(tpcf_ctx *ctx, const QueueDev &D, const TripletShape &sh, hipStream_t st,
+                        ElasticHook *hook = nullptr) {
     if (!D.nitems) return TPCF_OK;
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(D.counter), 1, 0u, st));
     if (D.max_n > kMaxLdsR123Bins) {
@@ -717,20 +729,34 @@ static int launch_queue(tpcf_ctx *ctx, const QueueDev &D, const TripletShape &sh
         HIP_TRY(launch_r123_triplets_wide(D.items, D.nitems, D.terms, ctx->num_cu * 8, st));
         return TPCF_OK;
     }
-    HIP_TRY(launch_r123_triplets(D.items, D.nitems, D.counter, D.terms, D.max_n, sh.waves, sh.copies, sh.grid, st));
+    ElasticCtl ctl{};
+    int grid = sh.grid;
+    if (hook) {
+        const QueueDev d = D;
+        const TripletShape s = sh;
+        auto relaunch = [d, s](int wgs, hipStream_t other, const ElasticCtl &c) -> int {
+            HIP_TRY(launch_r123_triplets(c, d.items, d.nitems, d.counter, d.terms, d.max_n, s.waves, s.copies, wgs,
+                                         other));
+            return TPCF_OK;
+        };
+        if (int e = hook->control(std::max(1, sh.grid / ctx->num_cu), D.nitems, relaunch, st, &ctl, &grid))
+            return e;
+    }
+    HIP_TRY(launch_r123_triplets(ctl, D.items, D.nitems, D.counter, D.terms, D.max_n, sh.waves, sh.copies, grid,
+                                 st));
     return TPCF_OK;
 }
 
 // Launch the triplet stage of several jobs as one persistent kernel.
 static int launch_triplets(tpcf_ctx *ctx, Workspace &ws, const std::vector<const JobPlan *> &plans,
                            const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases,
-                           hipStream_t st) {
+                           hipStream_t st, ElasticHook *hook = nullptr) {
     Queue Q;
     build_queue(plans, jobs, bases, Q);
     QueueDev D;
     if (int e = upload_queue(ws.queue, Q, st, D)) return e;
     HIP_TRY(hipEventRecord(ws.ev[3], st));
-    return launch_queue(ctx, D, triplet_shape(ctx, Q.max_n), st);
+    return launch_queue(ctx, D, triplet_shape(ctx, Q.max_n), st, hook);
 }
 
 // Triplets a workgroup of the plain (r1,r2,r3) kernel adds to its 32-bit LDS
@@ -921,7 +947,7 @@ static int sqt_upload(const SqtLaunch &L, DevBuf &buf, hipStream_t st) {
 // recorded after the pair stages and ev_kernel before the candidate launch
 // (stage timings).
 static int sqt_enqueue(tpcf_ctx *ctx, const SqtLaunch &L, const DevBuf &buf, hipStream_t st, hipEvent_t ev_pairs,
-                       hipEvent_t ev_kernel, bool pairs = true) {
+                       hipEvent_t ev_kernel, bool pairs = true, ElasticHook *hook = nullptr) {
     if (pairs)
         for (const SqtPairJob &J : L.pairs) HIP_TRY(launch_sqt_pairs(J, st));
     if (ev_pairs) HIP_TRY(hipEventRecord(ev_pairs, st));
@@ -936,13 +962,25 @@ static int sqt_enqueue(tpcf_ctx *ctx, const SqtLaunch &L, const DevBuf &buf, hip
     A.counter = reinterpret_cast<int *>(q + L.o_ctr);
     A.lds_cells = L.lds_cells;
     A.qcap = L.qcap;
-    HIP_TRY(launch_sqt_fast(A, L.waves, ctx->num_cu * L.per_cu, st));
+    int grid = ctx->num_cu * L.per_cu;
+    if (hook) {
+        const SqtFastArgs a0 = A;
+        const int waves = L.waves;
+        auto relaunch = [a0, waves](int wgs, hipStream_t other, const ElasticCtl &c) -> int {
+            SqtFastArgs a = a0;
+            a.ctl = c;
+            HIP_TRY(launch_sqt_fast(a, waves, wgs, other));
+            return TPCF_OK;
+        };
+        if (int e = hook->control(L.per_cu, A.nitems, relaunch, st, &A.ctl, &grid)) return e;
+    }
+    HIP_TRY(launch_sqt_fast(A, L.waves, grid, st));
     return TPCF_OK;
 }
 
 static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<const JobPlan *> &plans,
                                 const std::vector<const tpcf_job *> &jobs, const std::vector<char *> &bases, int mode,
-                                hipStream_t st) {
+                                hipStream_t st, ElasticHook *hook = nullptr) {
     // The pair stages first: the device runs them while the host plans the
     // items.
     for (size_t j = 0; j < plans.size(); ++j)
@@ -951,7 +989,7 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
     SqtLaunch L;
     if (int e = sqt_plan(ctx, plans, jobs, bases, mode, L)) return e;
     if (int e = sqt_upload(L, ws.sqt_queue, st)) return e;
-    return sqt_enqueue(ctx, L, ws.sqt_queue, st, nullptr, ws.ev[3], false);
+    return sqt_enqueue(ctx, L, ws.sqt_queue, st, nullptr, ws.ev[3], false, hook);
 }
 
 // Pair-stage and triplet-kernel times from the workspace events.
@@ -1019,12 +1057,41 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
     return TPCF_OK;
 }
 
+// Work items of a planned job: at most one per (A, B) block pair and run of
+// up to kMaxItemBlocks C blocks, per term (term_items).
+static uint64_t items_bound(const JobPlan &P, const tpcf_job &job) {
+    uint64_t items = 0;
+    for (int t = 0; t < job.nterms; ++t) {
+        const int *tr = job.term[t];
+        const uint64_t a = P.L.seg_nblk[tr[0]], b = P.L.seg_nblk[tr[1]], c = P.L.seg_nblk[tr[2]];
+        // culled items end where reachable C runs are more than
+        // kMergeGapBlocks apart
+        items += a * b * (P.cull ? c / kMergeGapBlocks + c / kMaxItemBlocks + 2
+                                 : (c + kMaxItemBlocks - 1) / kMaxItemBlocks + 1);
+    }
+    return items;
+}
+
+// Work-queue bytes (items, term table, counter) that upload_queue and
+// sqt_upload need at most for these jobs.
+static size_t queue_bound(const std::vector<JobPlan> &plans, const tpcf_job *jobs, bool r123) {
+    uint64_t items = 0, terms = 0;
+    for (size_t j = 0; j < plans.size(); ++j)
+        if ((plans[j].param == TPCF_R123) == r123) {
+            items += items_bound(plans[j], jobs[j]);
+            terms += (uint64_t)jobs[j].nterms;
+        }
+    if (!items) return 0;
+    const size_t tsz = r123 ? sizeof(TripletTerm) : sizeof(SqtTerm);
+    return align256(align256(sizeof(TripletItem) * items) + tsz * terms) + 256;
+}
+
 // Run several jobs as one batch on stream st: per-job staging and pair
 // stage, then one persistent triplet launch over every job's work items.
 // per[j] receives job j's counters; total the batch figures (device_ms is
 // the batch's kernel time).
 int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode,
-              uint64_t *const *hist, tpcf_stats *per, tpcf_stats *total, hipStream_t st) {
+              uint64_t *const *hist, tpcf_stats *per, tpcf_stats *total, hipStream_t st, ElasticHook *hook) {
     if (mode != TPCF_MODE_FAST && mode != TPCF_MODE_FAST64 && mode != TPCF_MODE_CELLS)
         return set_error(TPCF_EINVAL, "batched counting runs the FAST paths only");
     const auto t0 = std::chrono::steady_clock::now();
@@ -1036,9 +1103,35 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
         off[j] = bytes;
         bytes = align256(bytes + plans[j].bytes);
     }
+    if (hook)
+        for (int j = 1; j < njobs; ++j)
+            if ((plans[j].param == TPCF_R123) != (plans[0].param == TPCF_R123))
+                return set_error(TPCF_EINVAL, "an elastic batch holds jobs of one family");
+    // The hook's share is given back on every exit, once st has drained.
+    struct Share {
+        ElasticHook *h;
+        hipStream_t st;
+        void done() {
+            if (!h) return;
+            (void)hipStreamSynchronize(st);
+            h->finish();
+            h = nullptr;
+        }
+        ~Share() { done(); }
+    } share{hook, st};
     if (mode == TPCF_MODE_CELLS) mode = TPCF_MODE_FAST;  // the plans hold the culling
     if (int e = ws.init()) return e;
+    // An elastic batch reserves its workspace, work-queue buffers included,
+    // before it claims its share: nothing after the claim allocates.
     if (int e = ws.region.reserve(std::max<size_t>(bytes, 256))) return e;
+    if (hook) {
+        // and so do the work-queue buffers, to their bound
+        if (int e = ws.queue.reserve(queue_bound(plans, jobs, true))) return e;
+        if (int e = ws.sqt_queue.reserve(queue_bound(plans, jobs, false))) return e;
+        double work = 0.0;
+        for (int j = 0; j < njobs; ++j) work += (double)plans[j].triplets;
+        hook->claim(work);
+    }
     char *base = reinterpret_cast<char *>(ws.region.p);
     {
         std::vector<const tpcf_job *> jp(njobs);
@@ -1064,7 +1157,7 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
             sb.push_back(base + off[j]);
         }
     if (!sp.empty())
-        if (int e = launch_sqt_fast_jobs(ctx, ws, sp, sj, sb, mode, st)) return e;
+        if (int e = launch_sqt_fast_jobs(ctx, ws, sp, sj, sb, mode, st, hook)) return e;
     // (r1,r2,r3) jobs: pair stages, then one persistent launch.
     int max_n = 1;
     for (int j = 0; j < njobs; ++j)
@@ -1082,8 +1175,11 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     }
     if (!pp.empty() || sp.empty()) {
         HIP_TRY(hipEventRecord(ws.ev[1], st));
-        if (int e = launch_triplets(ctx, ws, pp, jj, bb, st)) return e;
+        if (int e = launch_triplets(ctx, ws, pp, jj, bb, st, hook)) return e;
     }
+    // The extra workgroups of an elastic launch ran on another stream: the
+    // histograms are complete once the hook has waited for them.
+    share.done();
     HIP_TRY(hipEventRecord(ws.ev[2], st));
     std::vector<uint64_t> rech(njobs, 0);
     for (int j = 0; j < njobs; ++j) {
@@ -1146,18 +1242,9 @@ int64_t footprint_bytes(const tpcf_job &job, const tpcf_binspec &spec, int mode,
     JobPlan P;
     if (int e = plan_job(job, spec, mode, P)) return e;
     if (triplets) *triplets = P.triplets;
-    // Work items: at most one per (A, B) block pair and run of up to
-    // kMaxItemBlocks C blocks, per term; plus the term table and counters.
-    uint64_t items = 0;
-    for (int t = 0; t < job.nterms; ++t) {
-        const int *tr = job.term[t];
-        const uint64_t a = P.L.seg_nblk[tr[0]], b = P.L.seg_nblk[tr[1]], c = P.L.seg_nblk[tr[2]];
-        // culled items end where reachable C runs are more than
-        // kMergeGapBlocks apart
-        items += a * b * (mode == TPCF_MODE_CELLS ? c / kMergeGapBlocks + c / kMaxItemBlocks + 2
-                                                   : (c + kMaxItemBlocks - 1) / kMaxItemBlocks + 1);
-    }
-    const size_t queue = align256(sizeof(TripletItem) * items) + align256(256 * 4) + 256;
+    // plus the term table and counters
+    constexpr size_t kTerm = sizeof(SqtTerm) > sizeof(TripletTerm) ? sizeof(SqtTerm) : sizeof(TripletTerm);
+    const size_t queue = align256(sizeof(TripletItem) * items_bound(P, job)) + align256(kTerm * 4) + 256;
     return (int64_t)(P.bytes + queue);
 }
 
@@ -1226,7 +1313,9 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
     c->lds_per_block = prop.sharedMemPerBlock;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->upload, hipStreamNonBlocking) != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return set_error(TPCF_EDEVICE, "stream creation failed");
     }
@@ -1282,6 +1371,7 @@ int tpcf_ctx_destroy(tpcf_ctx *ctx) {
     ctx->ws.reset();
     ctx->ws_fixed.reset();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->upload) (void)hipStreamDestroy(ctx->upload);
     delete ctx;
     return TPCF_OK;
 }
@@ -1324,7 +1414,8 @@ int tpcf_catalog_create(tpcf_ctx *ctx, const double *xyz, int64_t n, tpcf_catalo
             delete c;
             return set_error(TPCF_ENOMEM, "device allocation for %lld points failed", (long long)n);
         }
-        if (hipMemcpy(c->d, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMemcpyAsync(c->d, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->upload) != hipSuccess ||
+            hipStreamSynchronize(ctx->upload) != hipSuccess) {
             (void)hipFree(c->d);
             delete c;
             return set_error(TPCF_EDEVICE, "catalog upload failed");

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -47,6 +48,10 @@ struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
     bool fixed = false;
+    // Set: allocations are stream-ordered on this stream (hipMallocAsync /
+    // hipFreeAsync), so growing or releasing the buffer does not wait for
+    // every kernel on the device, as hipFree does.
+    hipStream_t ordered = nullptr;
     ~DevBuf();
     int reserve(size_t need);  // grow-only
     void release();            // free the memory (owned buffers only)
@@ -95,6 +100,7 @@ struct tpcf_catalog {
 namespace tpcf {
 struct Ticket;
 struct Workspace;
+struct ElasticScheduler;
 
 // Per-axis bounds of points [lo_i, hi_i) of a catalog (non-empty range):
 // the bounds kept at creation for the whole catalog, else one pass.
@@ -135,8 +141,32 @@ void parallel_ranges(int64_t n, F body) {
 }  // namespace tpcf
 
 namespace tpcf {
+// A serving batch's share of the device.  claim(work) before staging (work:
+// the batch's unique triplets): the running batches' persistent launches
+// shrink to make room.  control() when the batch's persistent launch of
+// nitems work items is about to be enqueued on st (everything it reads
+// already enqueued there): fills *ctl and lowers *grid to the batch's share
+// of workgroups (per_cu per CU at full occupancy); relaunch(wgs, stream,
+// ctl) enqueues wgs more workgroups of the same launch, which the hook does
+// on a stream of its own when the batch's share grows.  finish() once st
+// has drained: no more growth, the extra workgroups are waited for and the
+// CUs go to the others.
+struct ElasticHook {
+    using Relaunch = std::function<int(int, hipStream_t, const ElasticCtl &)>;
+    virtual ~ElasticHook() = default;
+    virtual void claim(double work) = 0;
+    virtual int control(int per_cu, int nitems, Relaunch relaunch, hipStream_t st, ElasticCtl *ctl, int *grid) = 0;
+    virtual void finish() = 0;
+};
+// hook: the batch's persistent launch is elastic; the batch then holds jobs
+// of one family ((r1,r2,r3), or (s,q,theta) and JBJ).
+// Make the workspace's buffers stream-ordered on st (a serving worker's
+// stream: a worker growing its workspace then does not wait for the other
+// workers' batches).
+void workspace_order(Workspace *ws, hipStream_t st);
 int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode,
-              uint64_t *const *hist, tpcf_stats *per, tpcf_stats *total, hipStream_t st);
+              uint64_t *const *hist, tpcf_stats *per, tpcf_stats *total, hipStream_t st,
+              ElasticHook *hook = nullptr);
 int run_single(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf_binspec &spec, int mode, uint64_t *hist,
                tpcf_stats *stats, hipStream_t st);
 // Device workspace bytes of a job (negative: error); *triplets (if given)
@@ -161,6 +191,9 @@ struct tpcf_ctx {
     int num_cu = 0;
     size_t lds_per_block = 0;
     hipStream_t stream = nullptr;
+    // Catalog uploads (non-blocking: an upload does not wait for work on the
+    // serving workers' streams, which synchronize with the null stream).
+    hipStream_t upload = nullptr;
     std::mutex call_mu;                    // serializes synchronous calls
     std::unique_ptr<tpcf::Workspace, tpcf::WorkspaceDeleter> ws;  // workspace of synchronous calls
     // tpcf_count_noalloc: the device's fixed scratch array (its address from
@@ -175,6 +208,8 @@ struct tpcf_ctx {
     // largest in_use so far; admission keeps in_use <= budget.
     uint64_t budget = 0, in_use = 0, peak = 0;
     std::vector<uint64_t> held;
+    // Device shares of the workers' persistent launches (serve.cpp).
+    std::shared_ptr<tpcf::ElasticScheduler> elastic;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<int64_t> pending;

@@ -30,6 +30,33 @@ struct TripletTerm {
     int pad;
 };
 
+// Elastic persistent launches (serving): the workgroups of a launch count
+// themselves in *active (device memory) and, before claiming their next
+// item, leave while more of them are active than *desired (pinned host
+// memory the host lowers to make room for another launch and raises again,
+// launching more workgroups over the same queue, when one finishes).  A
+// workgroup leaves before it claims anything, so no item is lost.  Both
+// pointers null: not elastic.
+struct ElasticCtl {
+    int *active;
+    const int *desired;
+};
+
+// One workgroup's step of the protocol, by one thread, before a claim: true
+// when the workgroup is to leave (its count already taken off *active).
+__device__ inline bool elastic_leave(const ElasticCtl &c) {
+    if (!c.active) return false;
+    int a = __hip_atomic_load(c.active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        const int want = __hip_atomic_load(c.desired, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (a <= want) return false;
+        // a failed exchange reloads a: another workgroup came or left
+        if (__hip_atomic_compare_exchange_strong(c.active, &a, a - 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return true;
+    }
+}
+
 // A unit of triplet work: blocks A <= B <= C, C in [c0, c1).
 struct TripletItem {
     int term;  // index into the TripletTerm table
@@ -146,6 +173,7 @@ struct SqtTerm {
     int q_log, t_log;
 };
 struct SqtFastArgs {
+    ElasticCtl ctl;  // serving launches; null pointers otherwise
     const SqtTerm *terms;
     const TripletItem *items;  // sorted by term
     int nitems;
@@ -210,7 +238,8 @@ hipError_t sort_segment(const double *x, const double *y, const double *z, int n
 
 // Launchers (kernels_r123.hip).
 hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st);
-hipError_t launch_r123_triplets(const TripletItem *items, int nitems, int *counter, const TripletTerm *terms,
+hipError_t launch_r123_triplets(const ElasticCtl &ctl, const TripletItem *items, int nitems, int *counter,
+                                const TripletTerm *terms,
                                 int max_n, int waves, int copies, int grid, hipStream_t st);
 hipError_t launch_r123_plain(const PlainArgs &P, int grid, hipStream_t st);
 hipError_t launch_r123_triplets_wide(const TripletItem *items, int nitems, const TripletTerm *terms, int grid,
