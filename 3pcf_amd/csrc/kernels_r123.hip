@@ -246,7 +246,7 @@ __device__ __forceinline__ void r123_column(uint32_t *hist, uint32_t &w, int j, 
 // geometries, where the run bookkeeping per column costs more than it saves)
 // adds one triplet per valid column, the invalid ones skipped by a scalar
 // test.
-template <int WAVES, int COPIES>
+template <int WAVES, int COPIES, bool ELASTIC>
 __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticCtl ctl,
                                                                   const TripletItem *__restrict__ items,
                                                                   int nitems, int *__restrict__ counter,
@@ -259,8 +259,9 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticC
     // Clear the whole allocation once; every flush clears what its term used,
     // so terms of any bin count start from zero.
     for (int c = threadIdx.x; c < lds_words; c += blockDim.x) hist[c] = 0u;
-    if (threadIdx.x == 0 && ctl.active) atomicAdd(ctl.active, 1);
     bool left = false;
+    if constexpr (ELASTIC)
+        if (threadIdx.x == 0) atomicAdd(ctl.active, 1);
     int cur = -1, cur_n = 0;
     unsigned long long since_flush = 0;
     unsigned long long *cur_hist = nullptr;
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticC
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) {
-            left = elastic_leave(ctl);
+            if constexpr (ELASTIC) left = elastic_leave(ctl);
             s_item = left ? nitems : atomicAdd(counter, 1);
         }
         __syncthreads();
@@ -394,7 +395,8 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticC
         }
     }
     if (cur >= 0) r123_flush<COPIES>(hist, cur_n, cur_hist);
-    if (threadIdx.x == 0 && ctl.active && !left) atomicSub(ctl.active, 1);
+    if constexpr (ELASTIC)
+        if (threadIdx.x == 0 && !left) atomicSub(ctl.active, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -542,9 +544,11 @@ hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st) {
     X(2, 1) X(4, 1) X(8, 1) X(16, 1) X(2, 2) X(4, 2) X(8, 2) X(16, 2) X(2, 4) X(4, 4) X(8, 4) X(16, 4) X(2, 8)   \
         X(4, 8) X(8, 8) X(16, 8)
 
-const void *r123_triplet_kernel_ptr(int waves, int copies) {
-#define TPCF_PTR(W, K) \
-    if (waves == W && copies == K) return reinterpret_cast<const void *>(&r123_triplet_kernel<W, K>);
+const void *r123_triplet_kernel_ptr(int waves, int copies, bool elastic) {
+#define TPCF_PTR(W, K)                                                                                   \
+    if (waves == W && copies == K)                                                                       \
+        return elastic ? reinterpret_cast<const void *>(&r123_triplet_kernel<W, K, true>)                 \
+                       : reinterpret_cast<const void *>(&r123_triplet_kernel<W, K, false>);
     TPCF_R123_SHAPES(TPCF_PTR)
 #undef TPCF_PTR
     return nullptr;
@@ -560,8 +564,12 @@ hipError_t launch_r123_triplets(const ElasticCtl &ctl, const TripletItem *items,
     const size_t lds = (size_t)words * 4;
 #define TPCF_LAUNCH(W, K)                                                                                    \
     if (waves == W && copies == K) {                                                                         \
-        hipLaunchKernelGGL((r123_triplet_kernel<W, K>), dim3(grid), dim3(W * 64), lds, st, ctl, items, nitems, counter, \
-                           terms, words);                                                                    \
+        if (ctl.active)                                                                                      \
+            hipLaunchKernelGGL((r123_triplet_kernel<W, K, true>), dim3(grid), dim3(W * 64), lds, st, ctl, items, \
+                               nitems, counter, terms, words);                                               \
+        else                                                                                                 \
+            hipLaunchKernelGGL((r123_triplet_kernel<W, K, false>), dim3(grid), dim3(W * 64), lds, st, ctl, items, \
+                               nitems, counter, terms, words);                                               \
         return hipGetLastError();                                                                            \
     }
     TPCF_R123_SHAPES(TPCF_LAUNCH)

@@ -539,7 +539,7 @@ __device__ inline void sqt_item(CTerm &T, const SqtItemCtx &X, int c1, const Sqt
 // its candidates to an LDS queue, drained 64 at a time with every lane busy.
 // The per-item work is compiled once per parametrization ((s,q,theta) and
 // JBJ jobs may share a launch).
-template <int WAVES>
+template <int WAVES, bool ELASTIC>
 __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
     extern __shared__ uint32_t hist[];
     __shared__ double sT[kMaxBins + 1], sQ[kMaxBins + 1], sA[kMaxBins + 1];
@@ -558,12 +558,13 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
     int cur = -1;
     CTerm *Tp = (CTerm *)(P.terms);
     unsigned long long since = 0, nre = 0;
-    if (threadIdx.x == 0 && P.ctl.active) atomicAdd(P.ctl.active, 1);
     bool left = false;
+    if constexpr (ELASTIC)
+        if (threadIdx.x == 0) atomicAdd(P.ctl.active, 1);
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) {
-            left = elastic_leave(P.ctl);
+            if constexpr (ELASTIC) left = elastic_leave(P.ctl);
             s_item = left ? P.nitems : atomicAdd(P.counter, 1);
         }
         __syncthreads();
@@ -654,7 +655,8 @@ __global__ __launch_bounds__(WAVES * 64) void sqt_fast_kernel(SqtFastArgs P) {
         sqt_flush64(hist, Tp->cells, Tp->hist);
         if (nre) atomicAdd(Tp->rechecked, nre);
     }
-    if (threadIdx.x == 0 && P.ctl.active && !left) atomicSub(P.ctl.active, 1);
+    if constexpr (ELASTIC)
+        if (threadIdx.x == 0 && !left) atomicSub(P.ctl.active, 1);
 }
 
 hipError_t launch_sqt_pairs(const SqtPairJob &J, hipStream_t st) {
@@ -662,9 +664,13 @@ hipError_t launch_sqt_pairs(const SqtPairJob &J, hipStream_t st) {
     return hipGetLastError();
 }
 
-const void *sqt_fast_kernel_ptr(int waves) {
-    if (waves == 8) return reinterpret_cast<const void *>(&sqt_fast_kernel<8>);
-    if (waves == 16) return reinterpret_cast<const void *>(&sqt_fast_kernel<16>);
+const void *sqt_fast_kernel_ptr(int waves, bool elastic) {
+    if (waves == 8)
+        return elastic ? reinterpret_cast<const void *>(&sqt_fast_kernel<8, true>)
+                       : reinterpret_cast<const void *>(&sqt_fast_kernel<8, false>);
+    if (waves == 16)
+        return elastic ? reinterpret_cast<const void *>(&sqt_fast_kernel<16, true>)
+                       : reinterpret_cast<const void *>(&sqt_fast_kernel<16, false>);
     return nullptr;
 }
 
@@ -675,10 +681,15 @@ size_t sqt_fast_lds_bytes(int cells, int waves, int qcap) {
 hipError_t launch_sqt_fast(const SqtFastArgs &P, int waves, int grid, hipStream_t st) {
     if (P.qcap < 128 || (P.qcap & (P.qcap - 1))) return hipErrorInvalidValue;
     const size_t lds = sqt_fast_lds_bytes(P.lds_cells, waves, P.qcap);
-    if (waves == 8)
-        hipLaunchKernelGGL(sqt_fast_kernel<8>, dim3(grid), dim3(512), lds, st, P);
+    const bool el = P.ctl.active != nullptr;
+    if (waves == 8 && el)
+        hipLaunchKernelGGL((sqt_fast_kernel<8, true>), dim3(grid), dim3(512), lds, st, P);
+    else if (waves == 8)
+        hipLaunchKernelGGL((sqt_fast_kernel<8, false>), dim3(grid), dim3(512), lds, st, P);
+    else if (waves == 16 && el)
+        hipLaunchKernelGGL((sqt_fast_kernel<16, true>), dim3(grid), dim3(1024), lds, st, P);
     else if (waves == 16)
-        hipLaunchKernelGGL(sqt_fast_kernel<16>, dim3(grid), dim3(1024), lds, st, P);
+        hipLaunchKernelGGL((sqt_fast_kernel<16, false>), dim3(grid), dim3(1024), lds, st, P);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

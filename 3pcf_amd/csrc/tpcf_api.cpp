@@ -1336,11 +1336,14 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     // Kernels with an LDS histogram raise their dynamic LDS limit once, to
     // what their static LDS leaves of the 160 KiB.
     std::vector<const void *> lds_kernels{sqt_plain_kernel_ptr(kSqt), sqt_plain_kernel_ptr(kJbj),
-                                          sqt_fast_kernel_ptr(8), sqt_fast_kernel_ptr(16),
                                           pair_hist_kernel_ptr(8, false), pair_hist_kernel_ptr(8, true)};
-    for (int w : {2, 4, 8, 16})
-        for (int k : {1, 2, 4, 8})
-            if (const void *fn = r123_triplet_kernel_ptr(w, k)) lds_kernels.push_back(fn);
+    for (bool el : {false, true}) {
+        lds_kernels.push_back(sqt_fast_kernel_ptr(8, el));
+        lds_kernels.push_back(sqt_fast_kernel_ptr(16, el));
+        for (int w : {2, 4, 8, 16})
+            for (int k : {1, 2, 4, 8})
+                if (const void *fn = r123_triplet_kernel_ptr(w, k, el)) lds_kernels.push_back(fn);
+    }
     for (const void *fn : lds_kernels) {
         hipFuncAttributes fa{};
         if (hipFuncGetAttributes(&fa, fn) != hipSuccess ||

@@ -186,7 +186,7 @@ struct SqtFastArgs {
 hipError_t launch_sqt_pairs(const SqtPairJob &J, hipStream_t st);
 hipError_t launch_sqt_fast(const SqtFastArgs &P, int waves, int grid, hipStream_t st);
 size_t sqt_fast_lds_bytes(int cells, int waves, int qcap);
-const void *sqt_fast_kernel_ptr(int waves);  // 8 or 16 waves per workgroup
+const void *sqt_fast_kernel_ptr(int waves, bool elastic = false);  // 8 or 16 waves per workgroup
 
 // Pair counts (kernels_pairs.hip).  Bins per axis up to kMaxPairBins.
 constexpr int kMaxPairBins = 4096;
@@ -247,6 +247,6 @@ hipError_t launch_r123_triplets_wide(const TripletItem *items, int nitems, const
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_to_f32(const double *u64, int64_t npad, const double o[3], float4 *u32, hipStream_t st);
 int r123_lds_bytes(int n, int copies);
-const void *r123_triplet_kernel_ptr(int waves, int copies);
+const void *r123_triplet_kernel_ptr(int waves, int copies, bool elastic = false);
 
 }  // namespace tpcf
