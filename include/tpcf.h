@@ -234,8 +234,12 @@ int tpcf_batch_destroy(tpcf_batch *batch);
 
 /* Serving: enqueue a job (admitted when its device workspace fits the
  * memory budget; otherwise it waits in the queue), returns a ticket.  hist
- * must stay valid until the ticket completes.  No reference counterpart
- * (the reference runs one process per job). */
+ * must stay valid until the ticket completes.  Three workers run batches at
+ * the same time, their persistent launches sharing the CUs (shortest work
+ * first); their streams have hardware queues of their own and synchronize
+ * with the legacy null stream, so null-stream work of the caller waits for
+ * running batches.  No reference counterpart (the reference runs one
+ * process per job). */
 int tpcf_submit(tpcf_ctx *ctx, const tpcf_job *job, const tpcf_binspec *spec, int mode,
                 uint64_t *hist, int64_t *ticket);
 /* Poll: 1 = done (the ticket is then released), 0 = pending, < 0 = the
