@@ -82,6 +82,7 @@ class Stats(C.Structure):
         ("pair_ms", C.c_double),
         ("triplet_ms", C.c_double),
         ("queue_ms", C.c_double),
+        ("mask_ops", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

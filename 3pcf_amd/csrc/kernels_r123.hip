@@ -2,19 +2,24 @@
 //
 //   pair_tiles_kernel     pair stage: every pair of a job's union layout gets
 //                         a u8 bin (or the sentinel) in 64x64 tiles, and every
-//                         tile a byte saying whether any pair of it has a bin.
-//                         FP32 squared distances against squared edges, with
-//                         every pair whose FP32 value lies within a rigorous
-//                         error bound (or the requested relative window) of an
-//                         edge recomputed in FP64; or FP64 throughout.
+//                         tile a byte saying whether any pair of it has a bin;
+//                         for the bit-mask kernel also the tile's set of bins,
+//                         its non-empty rows and the row bit masks of its
+//                         lowest bins.  FP32 squared distances against squared
+//                         edges, with every pair whose FP32 value lies within
+//                         a rigorous error bound (or the requested relative
+//                         window) of an edge recomputed in FP64; or FP64
+//                         throughout.
 //   r123_triplet_kernel   persistent triplet stage over a work queue of
-//                         (term, A, B, C-range) block triples of Hilbert-ordered
-//                         points: runs of equal cells along a tile row added
-//                         as one LDS atomic each (or one atomic per triplet on
-//                         rows with out-of-range pairs) into a privatized
-//                         unsorted-cube histogram, folded into sorted uint64
-//                         cells on flush; block pairs and C blocks whose tiles
-//                         have no pair in range are skipped.
+//                         (term, A, B, C-range) block triples: for each row i
+//                         of A and point j of B (a lane), the points k of C
+//                         with bins y = bin(i,k) and z = bin(j,k) are counted
+//                         64 at a time as popcount(row mask of y & row mask of
+//                         z), bin pair by bin pair, into per-lane accumulators
+//                         of x = bin(i,j); wave sums go to an unsorted-cube LDS
+//                         histogram folded into sorted uint64 cells on flush.
+//                         Block pairs, C blocks and rows with no pair in range
+//                         are skipped.
 //   r123_plain_kernel     the FP64 baseline: all three distances of every
 //                         triplet recomputed in FP64 and binned, like the
 //                         reference's distance() (src/3pcf_C_version.cc:54-142).
@@ -43,9 +48,9 @@ __device__ inline int search_le(F s, const F *T, int n) {
     return lo;
 }
 
-// Stored pair value: bin << shift, or the sentinel outside [0, n).
-__device__ inline uint32_t bin_of(int m, int n, int shift) {
-    return (m >= 0 && m < n) ? ((uint32_t)m << shift) : kSentinel;
+// Stored pair value: the bin, or the sentinel outside [0, n).
+__device__ inline uint32_t bin_of(int m, int n) {
+    return (m >= 0 && m < n) ? (uint32_t)m : kSentinel;
 }
 
 __device__ inline double sq64(const double *u, int64_t npad, int64_t a, int64_t b) {
@@ -67,14 +72,23 @@ __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
     if (C < R) return;
     if (J.need && !((J.need[R * J.need_words + (C >> 6)] >> (C & 63)) & 1ull)) {
         // Not computed: marked empty, so work items may span it.
-        if (threadIdx.x == 0 && J.tile_any) J.tile_any[tile_index(R, C, J.nblk)] = 0;
+        if (threadIdx.x == 0) {
+            if (J.tile_any) J.tile_any[tile_index(R, C, J.nblk)] = 0;
+            if (J.meta) J.meta[tile_index(R, C, J.nblk)] = TileMeta{0u, 0u, 0ull};
+        }
         return;
     }
     __shared__ float4 sR[kBlock], sC[kBlock];
     __shared__ double sT64[kMaxBins + 1];
     __shared__ float sT32[kMaxBins + 1];
+    __shared__ uint32_t s_bins;
+    __shared__ unsigned long long s_rows;
     const int t = threadIdx.x;
     const int n = J.n;
+    if (t == 0) {
+        s_bins = 0u;
+        s_rows = 0ull;
+    }
     for (int m = t; m <= n; m += 256) {
         sT64[m] = J.T64[m];
         sT32[m] = J.T32[m];
@@ -127,131 +141,239 @@ __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
                 }
                 if (nearb) {
                     ++nrecheck;
-                    b = bin_of(search_le(sq64(J.u64, J.npad, gi, gj), sT64, n), n, J.shift);
+                    b = bin_of(search_le(sq64(J.u64, J.npad, gi, gj), sT64, n), n);
                 } else {
-                    b = bin_of(m, n, J.shift);
+                    b = bin_of(m, n);
                 }
             } else {
-                b = bin_of(search_le(sq64(J.u64, J.npad, gi, gj), sT64, n), n, J.shift);
+                b = bin_of(search_le(sq64(J.u64, J.npad, gi, gj), sT64, n), n);
             }
         }
         packed[q >> 2] |= (b & 0xFFu) << (8 * (q & 3));
         any |= b != kSentinel;
     }
-    uint4 *dst = reinterpret_cast<uint4 *>(J.tiles + tile_index(R, C, J.nblk) * kTileBytes + i * kBlock + jb);
+    const int64_t tile = tile_index(R, C, J.nblk);
+    uint4 *dst = reinterpret_cast<uint4 *>(J.tiles + tile * kTileBytes + i * kBlock + jb);
     *dst = make_uint4(packed[0], packed[1], packed[2], packed[3]);
     if (F32 && nrecheck) atomicAdd(J.rechecked, nrecheck);
     // Whether any pair of the tile has a bin: a triplet with a side in this
     // tile counts only if that side does, so the triplet stage skips the
     // block pairs (and C blocks) whose tiles have none.
-    const int tile_has = __syncthreads_or(any ? 1 : 0);
-    if (t == 0 && J.tile_any) J.tile_any[tile_index(R, C, J.nblk)] = tile_has ? 1 : 0;
+    if (!J.meta) {
+        const int tile_has = __syncthreads_or(any ? 1 : 0);
+        if (t == 0 && J.tile_any) J.tile_any[tile] = tile_has ? 1 : 0;
+        return;
+    }
+    // Bit-mask kernel (pair values are the bins themselves, n <= 32): the
+    // tile's bins and non-empty rows, then the row masks of its lowest
+    // kRowMaskSlots bins.
+    uint32_t bins = 0u;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const uint32_t b = (packed[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+        if (b != kSentinel) bins |= 1u << b;
+    }
+    if (bins) {
+        atomicOr(&s_bins, bins);
+        atomicOr(&s_rows, 1ull << i);
+    }
+    __syncthreads();
+    const uint32_t tb = s_bins;
+    if (t == 0) {
+        if (J.tile_any) J.tile_any[tile] = tb ? 1 : 0;
+        J.meta[tile] = TileMeta{tb, 0u, s_rows};
+    }
+    uint32_t rest = tb;
+    for (int slot = 0; slot < kRowMaskSlots && rest; ++slot) {
+        const uint32_t bin = (uint32_t)__builtin_ctz(rest);
+        rest &= rest - 1u;
+        uint32_t part = 0u;
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            part |= (uint32_t)(((packed[q >> 2] >> (8 * (q & 3))) & 0xFFu) == bin) << q;
+        // The four threads of row i hold its columns 0-15, 16-31, 32-47, 48-63.
+        unsigned long long m = (unsigned long long)part << jb;
+        m |= __shfl_xor(m, 1);
+        m |= __shfl_xor(m, 2);
+        if ((t & 3) == 0) J.rmask[tile * kRowMaskWords + slot * kBlock + i] = m;
+    }
 }
 
 // ---------------------------------------------------------------------------
-// triplet stage (pair-bin form)
+// triplet stage (bit-mask form)
 // ---------------------------------------------------------------------------
 
-// Pair bins are stored pre-scaled as 4*bin (a byte offset into a row of
-// 32-bit counters); 0xFF is the sentinel, and every valid value is < 128.
-//
-// LDS histogram of one term with n bins: unsorted cube word
-// z*sz + y*sy + x for x = bin(i,j), y = bin(i,k), z = bin(j,k), with odd
-// strides sy = n|1 and sz = (n*sy)|1 so that the lanes of one instruction
-// (same x, varying y and z) spread over all 32 banks.  A lane whose z is out
-// of range adds into a dump word n*sz + lane + y*sy + x instead of branching,
-// so the dump needs sz + 64 extra words.  COPIES privatized copies of the cube
-// (lane k writes copy k % COPIES) cut same-address serialization of LDS
-// atomics when few cells are hot; copies start 32/COPIES banks apart.
-__host__ __device__ inline int r123_sy(int n) { return n | 1; }
-__host__ __device__ inline int r123_sz(int n) { return (n * r123_sy(n)) | 1; }
-__host__ __device__ inline int r123_lds_words(int n) { return (n + 1) * r123_sz(n) + 64; }
-// Copy k of the privatized histogram starts k * stride words in; the stride
-// is a multiple of 32 words plus 16/copies, so the hot cells of the copies
-// fall in different banks (measured on the 50k+50k four-term workload:
-// skew 4 for 4 copies is 5 % faster than skew 8).
-__host__ __device__ inline int r123_copy_stride(int n, int copies) {
-    return ((r123_lds_words(n) + 31) & ~31) + (copies > 1 ? 16 / copies : 0);
-}
+// LDS histogram of one term with n bins: the unsorted cube, word
+// (z * n + y) * n + x for x = bin(i,j), y = bin(i,k), z = bin(j,k), folded
+// into the term's sorted uint64 cells on flush.
+__host__ __device__ inline int r123_lds_words(int n) { return n * n * n; }
 
-// Fold the privatized unsorted cubes into the term's sorted uint64 histogram
-// and clear them (dump words included).
-template <int COPIES>
+// Fold the unsorted cube into the term's sorted uint64 histogram and clear it.
 __device__ inline void r123_flush(uint32_t *hist, int n, unsigned long long *g) {
     __syncthreads();
-    const int sy = r123_sy(n), sz = r123_sz(n);
     const int words = r123_lds_words(n);
-    const int stride = r123_copy_stride(n, COPIES);
     for (int c = threadIdx.x; c < words; c += blockDim.x) {
-        uint32_t v = 0;
-#pragma unroll
-        for (int k = 0; k < COPIES; ++k) {
-            v += hist[k * stride + c];
-            hist[k * stride + c] = 0u;
-        }
-        int z = c / sz, y = (c % sz) / sy, x = (c % sz) % sy, t;
-        if (z < n && y < n && x < n && v) {
-            if (x > y) { t = x; x = y; y = t; }
-            if (y > z) { t = y; y = z; z = t; }
-            if (x > y) { t = x; x = y; y = t; }
-            atomicAdd(g + (x * n + y) * n + z, (unsigned long long)v);
-        }
+        const uint32_t v = hist[c];
+        if (!v) continue;
+        hist[c] = 0u;
+        int x = c % n, y = (c / n) % n, z = c / (n * n), t;
+        if (x > y) { t = x; x = y; y = t; }
+        if (y > z) { t = y; y = z; z = t; }
+        if (x > y) { t = x; x = y; y = t; }
+        atomicAdd(g + (x * n + y) * n + z, (unsigned long long)v);
     }
     __syncthreads();
 }
 
-typedef const uint32_t __attribute__((address_space(4))) *scalar_u32_ptr;
 // Tile bytes read per lane: global (not flat) loads, so waiting for them does
-// not also wait for the wave's outstanding LDS atomics (flat loads count
-// against the LDS counter too).
+// not also wait for LDS operations (flat loads count against both counters).
 typedef const uint8_t __attribute__((address_space(1))) *global_u8_ptr;
+// Tile summaries read through the constant address space (scalar loads).
+typedef const TileMeta __attribute__((address_space(4))) CTileMeta;
+struct MetaV {
+    uint32_t bins;
+    unsigned long long rows;
+};
+__device__ inline MetaV load_meta(const TileMeta *meta, int64_t t) {
+    CTileMeta *p = (CTileMeta *)(meta + t);
+    return MetaV{p->bins, p->rows};
+}
 
 __device__ inline void lds_add(uint32_t *hist, uint32_t byte_addr, uint32_t v) {
     __hip_atomic_fetch_add(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(hist) + byte_addr), v,
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Column j of a fully valid row: lanes whose run of equal cells ends at j
-// (the carry out of doubling w, which holds the run ends of the remaining
-// columns from the top bit down) add the run's length to its cell.  Only
-// those lanes reach the LDS (an add from every lane, the others into their
-// dump slots, measured 10 % slower on the 50k + 50k workload).
-__device__ __forceinline__ void r123_column(uint32_t *hist, uint32_t &w, int j, int &last, uint32_t zs,
-                                            uint32_t ys, uint32_t x4) {
-    uint32_t w2;
-    const bool end = __builtin_add_overflow(w, w, &w2);
-    w = w2;
-    if (end) {
-        const uint32_t run = (uint32_t)(j - last);
-        last = j;
-        lds_add(hist, zs + ys + x4, run);
+// Sum of x over the 64 lanes (uniform): inclusive sums within each row of 16
+// lanes by DPP shifts, then the four row totals.
+__device__ inline uint32_t wave_sum(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 15) + (uint32_t)__builtin_amdgcn_readlane((int)x, 31) +
+           (uint32_t)__builtin_amdgcn_readlane((int)x, 47) + (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// Row mask of `bin` in a tile for lane r = row r (bit k set where pair
+// (r, k) has the bin), from the tile bytes: one ballot per row.  For bins
+// past the kRowMaskSlots the pair stage stored.
+__device__ inline unsigned long long build_row_mask(global_u8_ptr g, uint32_t bin, int lane) {
+    uint32_t lo = 0u, hi = 0u;
+    for (int r = 0; r < kBlock; ++r) {
+        const unsigned long long m = __ballot((uint32_t)g[r * kBlock + lane] == bin);
+        if (lane == r) {
+            lo = (uint32_t)m;
+            hi = (uint32_t)(m >> 32);
+        }
     }
+    return (unsigned long long)hi << 32 | lo;
+}
+
+// One pass over the rows i of block A (set bits of `rows`) for up to NX x
+// bins, NY y bins and NZ z bins of a block triple (A, B, C).  Lane j is
+// point j of block B.  Per row, ym[a] is the ballot over the points k of
+// block C with bin(i,k) = ys[a]; zm[b] (per lane, from the pair stage) has
+// bit k set where bin(j,k) = zs[b]; so popcount(ym[a] & zm[b]) counts the
+// triplets (i, j, k) with those two bins, and it goes to the lane's
+// accumulator of x = bin(i,j) (a sentinel matches no bin).  The x bins share
+// accumulator words as 16-bit fields: bins xs[0] and xs[1] are the low and
+// high fields of acc0, xs[2] the high field of acc1, added with one
+// multiply-add by the lane's weight 1 or 65536 (0 for a lane whose x is not
+// in the pass).  A lane adds at most 64 rows x 64 points = 4096 per field.
+// At the end each field is summed over the wave and added to its LDS cell by
+// one lane.  A y bin with no point in a row is skipped for that row.
+// Returns the pass's VALU lane-ops by the kernel's cost model: per row, NY
+// ballot compares, 2 NX for the weights, and for each (y, z) pair of a y bin
+// present in the row 2 ANDs, 2 popcounts and one multiply-add per
+// accumulator word; per accumulator field 9 for its wave sum.
+template <int NX, int NY, int NZ>
+__device__ __forceinline__ unsigned long long r123_pass(uint32_t *hist, global_u8_ptr gAB, global_u8_ptr gAC,
+                                          unsigned long long rows, const uint32_t (&xs)[3],
+                                          const uint32_t (&ys)[3], const uint32_t (&zs)[3],
+                                          const unsigned long long (&zm)[3], int lane, uint32_t n) {
+    uint32_t acc0[NY][NZ], acc1[NY][NZ];
+#pragma unroll
+    for (int a = 0; a < NY; ++a)
+#pragma unroll
+        for (int b = 0; b < NZ; ++b) acc0[a][b] = acc1[a][b] = 0u;
+    constexpr int NA = NX > 2 ? 2 : 1;
+    const uint32_t nrows = (uint32_t)__builtin_popcountll(rows);
+    uint32_t ylive = 0u;  // (row, y bin) pairs with some point
+    int i = __builtin_ctzll(rows);
+    uint32_t yb = gAC[i * kBlock + lane], xb = gAB[i * kBlock + lane];
+    for (;;) {
+        rows &= rows - 1ull;
+        const int inext = rows ? __builtin_ctzll(rows) : i;
+        // the next row's bytes load during this row
+        const uint32_t ybn = gAC[inext * kBlock + lane], xbn = gAB[inext * kBlock + lane];
+        unsigned long long ym[NY];
+#pragma unroll
+        for (int a = 0; a < NY; ++a) ym[a] = __ballot(yb == ys[a]);
+        const uint32_t w0 = xb == xs[0] ? 1u : (NX > 1 && xb == xs[1] ? 65536u : 0u);
+        const uint32_t w1 = NX > 2 && xb == xs[2] ? 65536u : 0u;
+#pragma unroll
+        for (int a = 0; a < NY; ++a) {
+            if (!ym[a]) continue;
+            ++ylive;
+#pragma unroll
+            for (int b = 0; b < NZ; ++b) {
+                const uint32_t c = (uint32_t)__builtin_popcount((uint32_t)ym[a] & (uint32_t)zm[b]) +
+                                   (uint32_t)__builtin_popcount((uint32_t)(ym[a] >> 32) & (uint32_t)(zm[b] >> 32));
+                acc0[a][b] += __umul24(c, w0);
+                if (NX > 2) acc1[a][b] += __umul24(c, w1);
+            }
+        }
+        if (!rows) break;
+        i = inext;
+        yb = ybn;
+        xb = xbn;
+    }
+#pragma unroll
+    for (int a = 0; a < NY; ++a)
+#pragma unroll
+        for (int b = 0; b < NZ; ++b)
+#pragma unroll
+            for (int s = 0; s < NX; ++s) {
+                const uint32_t f = s == 0 ? acc0[a][b] & 0xFFFFu : (s == 1 ? acc0[a][b] >> 16 : acc1[a][b] >> 16);
+                const uint32_t tot = wave_sum(f);
+                if (tot && lane == 0) lds_add(hist, ((zs[b] * n + ys[a]) * n + xs[s]) * 4u, tot);
+            }
+    return 64ull * ((unsigned long long)nrows * (NY + 2 * NX) + (unsigned long long)ylive * NZ * (4 + NA) +
+                    9ull * NX * NY * NZ);
+}
+
+// Up to three bins from the set bits of *rest (lowest first), taken off it.
+__device__ __forceinline__ int take3(uint32_t *rest, uint32_t (&v)[3]) {
+    int k = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        v[q] = 0u;
+        if (*rest) {
+            v[q] = (uint32_t)__builtin_ctz(*rest);
+            *rest &= *rest - 1u;
+            k = q + 1;
+        }
+    }
+    return k;
 }
 
 // Persistent: workgroups pull items (term, A, B, C-range) from a global
 // counter until the queue is drained; the waves of a workgroup split the C
 // blocks of an item (wave w takes C = c0+w, c0+w+WAVES, ...) and share the
-// LDS histogram copies.  Lane k owns point k of block C: z = bin(j,k) is
-// preloaded for the 64 columns j of block B, y = bin(i,k) is per lane and
-// fixed for a row i of block A, and x = bin(i,j) is uniform, read with scalar
-// loads.  The blocks hold spatially ordered (Hilbert) points, so along a row a
-// lane's cell (x, y, z) often stays the same for several consecutive
-// columns: each lane adds one run of equal cells at a time (the run's length
-// at its last column).  A run ends where x changes along the row (one ballot
-// per row) or z changes along the lane's column (one mask per C block), so
-// the add is issued only by the lanes whose run ends there: fewer LDS atomics
-// and fewer lanes per atomic, hence fewer bank and same-address conflicts.
-// Every triplet is still counted one by one in its run.  A row with no valid
-// x is skipped; a row with some invalid x (sparse rows of culled or wide
-// geometries, where the run bookkeeping per column costs more than it saves)
-// adds one triplet per valid column, the invalid ones skipped by a scalar
-// test.
-template <int WAVES, int COPIES, bool ELASTIC>
+// LDS histogram.  A block triple is counted bin by bin: the bins present in
+// its three tiles (TileMeta, from the pair stage) are taken up to three per
+// side at a time, and each pass (r123_pass) counts every triplet whose three
+// bins are in the pass's sets, so the passes together count each triplet of
+// the block triple once.  Only rows of block A with some pair in range in
+// both tiles (A,B) and (A,C) are visited; block pairs and C blocks whose
+// tiles have no pair in range are skipped.
+template <int WAVES, bool ELASTIC>
 __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticCtl ctl,
                                                                   const TripletItem *__restrict__ items,
                                                                   int nitems, int *__restrict__ counter,
                                                                   const TripletTerm *__restrict__ terms,
-                                                                  int lds_words) {
+                                                                  int lds_words, unsigned long long *ops) {
     extern __shared__ uint32_t hist[];
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
@@ -263,7 +385,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticC
     if constexpr (ELASTIC)
         if (threadIdx.x == 0) atomicAdd(ctl.active, 1);
     int cur = -1, cur_n = 0;
-    unsigned long long since_flush = 0;
+    unsigned long long since_flush = 0, wave_ops = 0;
     unsigned long long *cur_hist = nullptr;
     // One C block adds at most 64^3 counts to a cell; flush the 32-bit LDS
     // counters well before they could wrap.
@@ -287,7 +409,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticC
         const int c1 = __builtin_amdgcn_readfirstlane(I.c1);
         const unsigned long long add = (unsigned long long)(c1 - c0) * (kBlock * kBlock * kBlock);
         if (term != cur || since_flush + add > flush_limit) {
-            if (cur >= 0) r123_flush<COPIES>(hist, cur_n, cur_hist);
+            if (cur >= 0) r123_flush(hist, cur_n, cur_hist);
             cur = term;
             cur_n = n;
             cur_hist = T.hist;
@@ -295,106 +417,66 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticC
         }
         since_flush += add;
         const int64_t nblk = T.nblk;
+        const MetaV mAB = load_meta(T.meta, tile_index(A, B, nblk));
         // No triplet of the item counts when its (A, B) tile has no pair
         // with a bin.
-        const uint8_t *tany = T.tile_any;
-        if (tany && !tany[tile_index(A, B, nblk)]) continue;
-        const uint32_t sy = (uint32_t)r123_sy(n), sz = (uint32_t)r123_sz(n);
-        const uint32_t copy_base = (uint32_t)((lane % COPIES) * r123_copy_stride(n, COPIES)) * 4u;
-        const uint32_t dump = copy_base + ((uint32_t)n * sz + (uint32_t)lane) * 4u;
-        const uint8_t *tAB = T.tiles + tile_index(A, B, nblk) * kTileBytes;
-        const global_u8_ptr gAB = (global_u8_ptr)tAB;
+        if (!mAB.bins) continue;
+        const global_u8_ptr gAB = (global_u8_ptr)(T.tiles + tile_index(A, B, nblk) * kTileBytes);
         for (int C = c0 + wave; C < c1; C += WAVES) {
-            // Likewise a C block whose (A, C) or (B, C) tile has none.
-            if (tany && !(tany[tile_index(A, C, nblk)] & tany[tile_index(B, C, nblk)])) continue;
-            const global_u8_ptr gAC = (global_u8_ptr)(T.tiles + tile_index(A, C, nblk) * kTileBytes);
-            const global_u8_ptr gBC = (global_u8_ptr)(T.tiles + tile_index(B, C, nblk) * kTileBytes);
-            // z of the 64 columns as byte offsets of the z slab in the lane's
-            // copy, or the lane's dump slot for the sentinel.
-            uint32_t Zs[kBlock];
+            const int64_t tAC = tile_index(A, C, nblk), tBC = tile_index(B, C, nblk);
+            const MetaV mAC = load_meta(T.meta, tAC), mBC = load_meta(T.meta, tBC);
+            // Likewise a C block whose (A, C) or (B, C) tile has none, and
+            // the rows of A with no pair in range in (A, B) or (A, C).
+            if (!mAC.bins || !mBC.bins) continue;
+            const unsigned long long rows = mAB.rows & mAC.rows;
+            if (!rows) continue;
+            const global_u8_ptr gAC = (global_u8_ptr)(T.tiles + tAC * kTileBytes);
+            const global_u8_ptr gBC = (global_u8_ptr)(T.tiles + tBC * kTileBytes);
+            uint32_t zrest = mBC.bins;
+            while (zrest) {
+                uint32_t zs[3];
+                unsigned long long zm[3];
+                const int nz = take3(&zrest, zs);
 #pragma unroll
-            for (int j = 0; j < kBlock; ++j) Zs[j] = gBC[j * kBlock + lane];
-            // Row i's y and x bytes are loaded during row i-1.
-            uint32_t y4n = gAC[lane], xvn = gAB[lane];
-#pragma unroll
-            for (int j = 0; j < kBlock; ++j) Zs[j] = Zs[j] < 128u ? __umul24(Zs[j], sz) + copy_base : dump;
-            // Run ends along the lane's column: bit 63-j set where the z slot
-            // of column j differs from that of column j+1 (and at j = 63);
-            // computed by the first row of the C block that merges runs.
-            unsigned long long zend = 0ull;
-            bool have_zend = false;
-            for (int i = 0; i < kBlock; ++i) {
-                const uint32_t y4 = y4n;
-                const uint32_t xv = xvn;
-                const int inext = i + 1 < kBlock ? i + 1 : i;
-                y4n = gAC[inext * kBlock + lane];
-                xvn = gAB[inext * kBlock + lane];
-                const unsigned long long rowmask = __ballot(xv < 128u);
-                if (rowmask == 0ull) continue;
-                // Fully valid rows merge runs.  Their run ends are found
-                // here, in uniform control flow (every lane active), before
-                // the lanes with invalid y drop out.
-                const bool runs = rowmask == ~0ull;
-                unsigned long long xend = 0ull;
-                if (runs) {
-                    // Along the row: x of column j differs from x of column
-                    // j+1 (lane j+1's value by a wave shift left; lane 63
-                    // keeps its own), bit j of the ballot reversed to 63-j.
-                    const uint32_t xv1 = (uint32_t)__builtin_amdgcn_update_dpp((int)xv, (int)xv, 0x130, 0xf, 0xf,
-                                                                                false);  // wave_shl:1
-                    xend = __builtin_bitreverse64(__ballot(xv != xv1));
-                    if (!have_zend) {
-                        zend = 1ull;
-#pragma unroll
-                        for (int j = 0; j + 1 < kBlock; ++j)
-                            zend |= (unsigned long long)(Zs[j] != Zs[j + 1]) << (kBlock - 1 - j);
-                        have_zend = true;
+                for (int b = 0; b < 3; ++b) {
+                    zm[b] = 0ull;
+                    if (b < nz) {
+                        const int slot = __builtin_popcount(mBC.bins & ((1u << zs[b]) - 1u));
+                        zm[b] = slot < kRowMaskSlots ? T.rmask[tBC * kRowMaskWords + slot * kBlock + lane]
+                                                     : build_row_mask(gBC, zs[b], lane);
                     }
                 }
-                if (y4 >= 128u) continue;
-                const uint32_t Ys = __umul24(y4, sy);
-                // x(i, j) for the 64 columns: one scalar load of the row.
-                // (Reading lane j of xv instead would be wrong here: inside
-                // the y-divergent region the compiler may rebuild xv in a
-                // fresh register for the active lanes only.)
-                const scalar_u32_ptr xrow = (scalar_u32_ptr)(tAB + i * kBlock);
-                uint32_t xw[16];
-#pragma unroll
-                for (int q = 0; q < 16; ++q) xw[q] = xrow[q];
-                // The row's words are waited for here, once: the columns
-                // below may each be skipped, and a wait inside them would
-                // also wait for the wave's outstanding LDS adds (scalar loads
-                // and LDS share a counter).  Emits no instruction.
-                asm volatile("" ::"s"(xw[0]), "s"(xw[1]), "s"(xw[2]), "s"(xw[3]), "s"(xw[4]), "s"(xw[5]),
-                             "s"(xw[6]), "s"(xw[7]), "s"(xw[8]), "s"(xw[9]), "s"(xw[10]), "s"(xw[11]),
-                             "s"(xw[12]), "s"(xw[13]), "s"(xw[14]), "s"(xw[15]));
-                const unsigned long long ends = xend | zend;
-                uint32_t whi = (uint32_t)(ends >> 32), wlo = (uint32_t)ends;
-                int last = -1;
-                if (runs) {
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int j = q * 4 + r;
-                            const uint32_t x4 = (xw[q] >> (8 * r)) & 0xFFu;
-                            r123_column(hist, j < 32 ? whi : wlo, j, last, Zs[j], Ys, x4);
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const uint32_t x4 = (xw[q] >> (8 * r)) & 0xFFu;
-                            if (x4 < 128u) lds_add(hist, Zs[q * 4 + r] + Ys + x4, 1u);
+                uint32_t yrest = mAC.bins;
+                while (yrest) {
+                    uint32_t ys[3];
+                    const int ny = take3(&yrest, ys);
+                    uint32_t xrest = mAB.bins;
+                    while (xrest) {
+                        uint32_t xs[3];
+                        const int nx = take3(&xrest, xs);
+                        switch ((nx - 1) * 9 + (ny - 1) * 3 + (nz - 1)) {
+#define TPCF_PASS(X, Y, Z)                                                                     \
+    case (X - 1) * 9 + (Y - 1) * 3 + (Z - 1):                                                  \
+        wave_ops += r123_pass<X, Y, Z>(hist, gAB, gAC, rows, xs, ys, zs, zm, lane, (uint32_t)n); \
+        break;
+                            TPCF_PASS(1, 1, 1) TPCF_PASS(1, 1, 2) TPCF_PASS(1, 1, 3)
+                            TPCF_PASS(1, 2, 1) TPCF_PASS(1, 2, 2) TPCF_PASS(1, 2, 3)
+                            TPCF_PASS(1, 3, 1) TPCF_PASS(1, 3, 2) TPCF_PASS(1, 3, 3)
+                            TPCF_PASS(2, 1, 1) TPCF_PASS(2, 1, 2) TPCF_PASS(2, 1, 3)
+                            TPCF_PASS(2, 2, 1) TPCF_PASS(2, 2, 2) TPCF_PASS(2, 2, 3)
+                            TPCF_PASS(2, 3, 1) TPCF_PASS(2, 3, 2) TPCF_PASS(2, 3, 3)
+                            TPCF_PASS(3, 1, 1) TPCF_PASS(3, 1, 2) TPCF_PASS(3, 1, 3)
+                            TPCF_PASS(3, 2, 1) TPCF_PASS(3, 2, 2) TPCF_PASS(3, 2, 3)
+                            TPCF_PASS(3, 3, 1) TPCF_PASS(3, 3, 2) TPCF_PASS(3, 3, 3)
+#undef TPCF_PASS
                         }
                     }
                 }
             }
         }
     }
-    if (cur >= 0) r123_flush<COPIES>(hist, cur_n, cur_hist);
+    if (cur >= 0) r123_flush(hist, cur_n, cur_hist);
+    if (ops && lane == 0 && wave_ops) atomicAdd(ops, wave_ops);
     if constexpr (ELASTIC)
         if (threadIdx.x == 0 && !left) atomicSub(ctl.active, 1);
 }
@@ -540,36 +622,39 @@ hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st) {
     return hipGetLastError();
 }
 
-#define TPCF_R123_SHAPES(X)                                                                                    \
-    X(2, 1) X(4, 1) X(8, 1) X(16, 1) X(2, 2) X(4, 2) X(8, 2) X(16, 2) X(2, 4) X(4, 4) X(8, 4) X(16, 4) X(2, 8)   \
-        X(4, 8) X(8, 8) X(16, 8)
+#define TPCF_R123_SHAPES(X) X(4) X(8) X(16)
 
+// One histogram copy per workgroup: `copies` must be 1.
 const void *r123_triplet_kernel_ptr(int waves, int copies, bool elastic) {
-#define TPCF_PTR(W, K)                                                                                   \
-    if (waves == W && copies == K)                                                                       \
-        return elastic ? reinterpret_cast<const void *>(&r123_triplet_kernel<W, K, true>)                 \
-                       : reinterpret_cast<const void *>(&r123_triplet_kernel<W, K, false>);
+    if (copies != 1) return nullptr;
+#define TPCF_PTR(W)                                                                                      \
+    if (waves == W)                                                                                      \
+        return elastic ? reinterpret_cast<const void *>(&r123_triplet_kernel<W, true>)                    \
+                       : reinterpret_cast<const void *>(&r123_triplet_kernel<W, false>);
     TPCF_R123_SHAPES(TPCF_PTR)
 #undef TPCF_PTR
     return nullptr;
 }
 
-int r123_lds_bytes(int n, int copies) { return copies * r123_copy_stride(n, copies) * 4; }
+int r123_lds_bytes(int n, int copies) { return copies * r123_lds_words(n) * 4; }
 
 hipError_t launch_r123_triplets(const ElasticCtl &ctl, const TripletItem *items, int nitems, int *counter,
                                 const TripletTerm *terms,
                                 int max_n, int waves, int copies, int grid, hipStream_t st) {
-    if (max_n < 1 || max_n > kMaxLdsR123Bins) return hipErrorInvalidValue;
-    const int words = copies * r123_copy_stride(max_n, copies);
+    // The kernel's modelled VALU lane-ops go to the 64-bit word after the
+    // item counter (counter[2..3]).
+    unsigned long long *ops = reinterpret_cast<unsigned long long *>(counter + 2);
+    if (max_n < 1 || max_n > kMaxLdsR123Bins || copies != 1) return hipErrorInvalidValue;
+    const int words = r123_lds_words(max_n);
     const size_t lds = (size_t)words * 4;
-#define TPCF_LAUNCH(W, K)                                                                                    \
-    if (waves == W && copies == K) {                                                                         \
+#define TPCF_LAUNCH(W)                                                                                       \
+    if (waves == W) {                                                                                        \
         if (ctl.active)                                                                                      \
-            hipLaunchKernelGGL((r123_triplet_kernel<W, K, true>), dim3(grid), dim3(W * 64), lds, st, ctl, items, \
-                               nitems, counter, terms, words);                                               \
+            hipLaunchKernelGGL((r123_triplet_kernel<W, true>), dim3(grid), dim3(W * 64), lds, st, ctl, items,  \
+                               nitems, counter, terms, words, ops);                                          \
         else                                                                                                 \
-            hipLaunchKernelGGL((r123_triplet_kernel<W, K, false>), dim3(grid), dim3(W * 64), lds, st, ctl, items, \
-                               nitems, counter, terms, words);                                               \
+            hipLaunchKernelGGL((r123_triplet_kernel<W, false>), dim3(grid), dim3(W * 64), lds, st, ctl, items, \
+                               nitems, counter, terms, words, ops);                                          \
         return hipGetLastError();                                                                            \
     }
     TPCF_R123_SHAPES(TPCF_LAUNCH)

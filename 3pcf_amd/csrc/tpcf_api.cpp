@@ -373,9 +373,10 @@ struct JobPlan {
     uint64_t triplets = 0;
     // byte offsets inside the job's device region
     size_t o_u64 = 0, o_u32 = 0, o_blk = 0, o_T64 = 0, o_T32 = 0, o_Eq = 0, o_Et = 0, o_cnt = 0, o_hist = 0,
-           o_tiles = 0, o_any = 0, o_dists = 0, o_masks = 0;
+           o_tiles = 0, o_any = 0, o_dists = 0, o_masks = 0, o_meta = 0, o_rmask = 0;
     size_t bytes = 0;
     bool need_tiles = true;
+    bool bitmask = false;  // tile summaries and row masks for the bit-mask triplet kernel
     double far_s = 0;  // SQT FAST: squared sides above this cannot be in any entry
     double mid_s = 0;  // SQT FAST: squared q sides above this cannot be in any entry
     tpcf_axis axq{}, axt{};  // SQT: q and theta/pi axes
@@ -461,6 +462,13 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     P.o_hist = o; o = align256(o + sizeof(uint64_t) * P.cells * P.nterms);
     P.o_tiles = o; o = align256(o + (P.need_tiles ? (size_t)tile_count(P.L.nblk) * kTileBytes : 0));
     P.o_any = o; o = align256(o + (mode != TPCF_MODE_PLAIN ? (size_t)tile_count(P.L.nblk) : 0));
+    // (r1,r2,r3) FAST paths with bins for the bit-mask kernel: per tile its
+    // summary and row masks.
+    const bool bitmask = spec.param == TPCF_R123 && mode != TPCF_MODE_PLAIN && P.n <= kMaxLdsR123Bins;
+    P.o_meta = o; o = align256(o + (bitmask ? (size_t)tile_count(P.L.nblk) * sizeof(TileMeta) : 0));
+    P.o_rmask = o;
+    o = align256(o + (bitmask ? (size_t)tile_count(P.L.nblk) * kRowMaskWords * sizeof(unsigned long long) : 0));
+    P.bitmask = bitmask;
     const bool sqt_fast = spec.param != TPCF_R123 && mode != TPCF_MODE_PLAIN;
     P.o_dists = o; o = align256(o + (sqt_fast ? (size_t)tile_count(P.L.nblk) * kTileBytes * sizeof(float) : 0));
     P.o_masks = o;
@@ -584,13 +592,15 @@ static int stage_jobs(const tpcf_job *const *jobs, const JobPlan *const *plans, 
     return TPCF_OK;
 }
 
-static PairJob pair_job(const JobPlan &P, char *base, int shift) {
+static PairJob pair_job(const JobPlan &P, char *base) {
     PairJob J{};
     J.u64 = reinterpret_cast<const double *>(base + P.o_u64);
     J.u32 = reinterpret_cast<const float4 *>(base + P.o_u32);
     J.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
     J.tiles = reinterpret_cast<uint8_t *>(base + P.o_tiles);
     J.tile_any = reinterpret_cast<uint8_t *>(base + P.o_any);
+    J.meta = P.bitmask ? reinterpret_cast<TileMeta *>(base + P.o_meta) : nullptr;
+    J.rmask = P.bitmask ? reinterpret_cast<unsigned long long *>(base + P.o_rmask) : nullptr;
     J.T64 = reinterpret_cast<const double *>(base + P.o_T64);
     J.T32 = reinterpret_cast<const float *>(base + P.o_T32);
     J.rechecked = reinterpret_cast<unsigned long long *>(base + P.o_cnt);
@@ -599,7 +609,6 @@ static PairJob pair_job(const JobPlan &P, char *base, int shift) {
     J.n = P.n;
     J.rmax = P.rmax;
     J.rel = P.rel;
-    J.shift = shift;
     J.need = P.cull ? reinterpret_cast<const unsigned long long *>(base + P.o_need) : nullptr;
     J.need_words = (P.L.nblk + 63) / 64;
     return J;
@@ -614,6 +623,9 @@ struct Workspace {
     DevBuf queue;      // items + terms + counter of the (r1,r2,r3) kernel
     DevBuf sqt_queue;  // items + terms + counter of the (s,q,theta) kernel
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // The (r1,r2,r3) bit-mask kernel's modelled VALU lane-ops of the last
+    // launch on this workspace (device word), or nullptr.
+    const unsigned long long *ops = nullptr;
     ~Workspace() {
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -629,9 +641,9 @@ void workspace_order(Workspace *ws, hipStream_t st) {
     ws->region.ordered = ws->queue.ordered = ws->sqt_queue.ordered = st;
 }
 
-// Launch shape of the LDS triplet kernel for bins up to max_n: as many
-// privatized histogram copies (4, 2 or 1) as fit the LDS budget, and enough
-// waves per workgroup for four waves per SIMD at the resulting occupancy.
+// Launch shape of the LDS triplet kernel for bins up to max_n: one
+// histogram copy per workgroup, 8 waves per workgroup and as many
+// workgroups per CU as the occupancy allows.
 // TPCF_R123_SHAPE="waves,copies[,wg_per_cu]" overrides (tuning runs).
 struct TripletShape {
     int waves, copies, grid;
@@ -640,13 +652,6 @@ struct TripletShape {
 static TripletShape triplet_shape(tpcf_ctx *ctx, int max_n) {
     constexpr int kLdsBudget = 155 * 1024;
     TripletShape s{8, 1, 0};
-    for (int k : {8, 4, 2, 1})
-        if (r123_lds_bytes(max_n, k) <= kLdsBudget) {
-            s.copies = k;
-            break;
-        }
-    const int lds = r123_lds_bytes(max_n, s.copies);
-    s.waves = lds > kLdsBudget / 2 ? 16 : 8;
     int per_cu_cap = 0;
     if (ctx->shape.waves > 0 && r123_triplet_kernel_ptr(ctx->shape.waves, ctx->shape.copies) &&
         r123_lds_bytes(max_n, ctx->shape.copies) <= kLdsBudget) {
@@ -681,6 +686,8 @@ static void build_queue(const std::vector<const JobPlan *> &plans, const std::ve
             TripletTerm T{};
             T.tiles = reinterpret_cast<const uint8_t *>(bases[j] + P.o_tiles);
             T.tile_any = reinterpret_cast<const uint8_t *>(bases[j] + P.o_any);
+            T.meta = P.bitmask ? reinterpret_cast<const TileMeta *>(bases[j] + P.o_meta) : nullptr;
+            T.rmask = P.bitmask ? reinterpret_cast<const unsigned long long *>(bases[j] + P.o_rmask) : nullptr;
             T.nblk = P.L.nblk;
             T.hist = reinterpret_cast<unsigned long long *>(bases[j] + P.o_hist) + (size_t)t * P.cells;
             T.n = P.n;
@@ -723,7 +730,8 @@ static int upload_queue(DevBuf &buf, const Queue &Q, hipStream_t st, QueueDev &D
 static int launch_queue(tpcf_ctx *ctx, const QueueDev &D, const TripletShape &sh, hipStream_t st,
                         ElasticHook *hook = nullptr) {
     if (!D.nitems) return TPCF_OK;
-    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(D.counter), 1, 0u, st));
+    // item counter and the kernel's 64-bit op count after it
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(D.counter), 4, 0u, st));
     if (D.max_n > kMaxLdsR123Bins) {
         // Wide specs: pair values are unscaled; global-atomic kernel.
         HIP_TRY(launch_r123_triplets_wide(D.items, D.nitems, D.terms, ctx->num_cu * 8, st));
@@ -755,6 +763,8 @@ static int launch_triplets(tpcf_ctx *ctx, Workspace &ws, const std::vector<const
     build_queue(plans, jobs, bases, Q);
     QueueDev D;
     if (int e = upload_queue(ws.queue, Q, st, D)) return e;
+    ws.ops = D.nitems && D.max_n <= kMaxLdsR123Bins ? reinterpret_cast<const unsigned long long *>(D.counter + 2)
+                                                     : nullptr;
     HIP_TRY(hipEventRecord(ws.ev[3], st));
     return launch_queue(ctx, D, triplet_shape(ctx, Q.max_n), st, hook);
 }
@@ -1010,6 +1020,7 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
     if (int e = plan_job(job, spec, mode, P)) return e;
     if (mode == TPCF_MODE_CELLS) mode = TPCF_MODE_FAST;  // the plan holds the culling
     if (int e = ws.init()) return e;
+    ws.ops = nullptr;
     if (int e = ws.region.reserve(P.bytes)) return e;
     char *base = reinterpret_cast<char *>(ws.region.p);
     {
@@ -1030,7 +1041,7 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
         std::vector<char *> bases{base};
         if (int e = launch_sqt_fast_jobs(ctx, ws, plans, jobs, bases, mode, st)) return e;
     } else if (P.L.nblk) {
-        HIP_TRY(launch_pair_tiles(pair_job(P, base, P.n <= kMaxLdsR123Bins ? 2 : 0), mode == TPCF_MODE_FAST, st));
+        HIP_TRY(launch_pair_tiles(pair_job(P, base), mode == TPCF_MODE_FAST, st));
         HIP_TRY(hipEventRecord(ws.ev[1], st));
         std::vector<const JobPlan *> plans{&P};
         std::vector<const tpcf_job *> jobs{&job};
@@ -1053,6 +1064,8 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
         stats->device_ms = ms;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (int e = stage_times(ws, stats)) return e;
+        stats->mask_ops = 0;
+        if (ws.ops) HIP_TRY(hipMemcpy(&stats->mask_ops, ws.ops, sizeof(stats->mask_ops), hipMemcpyDeviceToHost));
     }
     return TPCF_OK;
 }
@@ -1121,6 +1134,7 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     } share{hook, st};
     if (mode == TPCF_MODE_CELLS) mode = TPCF_MODE_FAST;  // the plans hold the culling
     if (int e = ws.init()) return e;
+    ws.ops = nullptr;
     // An elastic batch reserves its workspace, work-queue buffers included,
     // before it claims its share: nothing after the claim allocates.
     if (int e = ws.region.reserve(std::max<size_t>(bytes, 256))) return e;
@@ -1159,16 +1173,12 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     if (!sp.empty())
         if (int e = launch_sqt_fast_jobs(ctx, ws, sp, sj, sb, mode, st, hook)) return e;
     // (r1,r2,r3) jobs: pair stages, then one persistent launch.
-    int max_n = 1;
-    for (int j = 0; j < njobs; ++j)
-        if (plans[j].param == TPCF_R123) max_n = std::max(max_n, plans[j].n);
-    const int shift = max_n <= kMaxLdsR123Bins ? 2 : 0;
     std::vector<const JobPlan *> pp;
     std::vector<const tpcf_job *> jj;
     std::vector<char *> bb;
     for (int j = 0; j < njobs; ++j) {
         if (plans[j].param != TPCF_R123 || !plans[j].L.nblk) continue;
-        HIP_TRY(launch_pair_tiles(pair_job(plans[j], base + off[j], shift), mode == TPCF_MODE_FAST, st));
+        HIP_TRY(launch_pair_tiles(pair_job(plans[j], base + off[j]), mode == TPCF_MODE_FAST, st));
         pp.push_back(&plans[j]);
         jj.push_back(&jobs[j]);
         bb.push_back(base + off[j]);
@@ -1209,9 +1219,11 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     tot.device_ms = ms;
     tot.wall_ms = wall;
     if (int e = stage_times(ws, &tot)) return e;
+    if (ws.ops) HIP_TRY(hipMemcpy(&tot.mask_ops, ws.ops, sizeof(tot.mask_ops), hipMemcpyDeviceToHost));
     for (int j = 0; j < njobs && per; ++j) {
         per[j].pair_ms = tot.pair_ms;
         per[j].triplet_ms = tot.triplet_ms;
+        per[j].mask_ops = tot.mask_ops;
     }
     if (total) *total = tot;
     return TPCF_OK;
@@ -1539,7 +1551,6 @@ struct tpcf_batch {
     hipStream_t st = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     QueueDev qd;
-    int shift = 2;
     TripletShape shape{8, 1, 0};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1567,7 +1578,7 @@ static int batch_compute(tpcf_batch *b, const TripletShape &sh, hipStream_t st) 
     if (int e = sqt_enqueue(b->ctx, b->sqt, b->sqt_queue, st, nullptr, nullptr)) return e;
     for (int j = 0; j < b->njobs; ++j)
         if (b->plans[j].param == TPCF_R123 && b->plans[j].L.nblk)
-            HIP_TRY(launch_pair_tiles(pair_job(b->plans[j], base + b->off[j], b->shift), b->mode == TPCF_MODE_FAST,
+            HIP_TRY(launch_pair_tiles(pair_job(b->plans[j], base + b->off[j]), b->mode == TPCF_MODE_FAST,
                                       st));
     if (!b->qd.nitems) return TPCF_OK;
     return launch_queue(b->ctx, b->qd, sh, st);
@@ -1590,15 +1601,9 @@ static int batch_autotune(tpcf_batch *b) {
     const int n = b->qd.max_n;
     if (n > kMaxLdsR123Bins || !b->qd.nitems) return TPCF_OK;
     std::vector<TripletShape> cands;
-    int kmax = 1;
-    for (int k : {8, 4, 2, 1})
-        if (r123_lds_bytes(n, k) <= kLdsBudget) {
-            kmax = k;
-            break;
-        }
-    for (int k : {kmax, kmax / 2})
-        for (int w : {2, 4, 8, 16}) {
-            if (k < 1 || !r123_triplet_kernel_ptr(w, k)) continue;
+    for (int k : {1})
+        for (int w : {4, 8, 16}) {
+            if (!r123_triplet_kernel_ptr(w, k) || r123_lds_bytes(n, k) > kLdsBudget) continue;
             int occ = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, r123_triplet_kernel_ptr(w, k), w * 64,
                                                              r123_lds_bytes(n, k)) != hipSuccess || occ < 1)
@@ -1640,17 +1645,14 @@ int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *s
     b->plans.resize(njobs);
     b->off.resize(njobs);
     size_t bytes = 0;
-    int max_n = 1;
     for (int j = 0; j < njobs; ++j) {
         if (int e = plan_job(jobs[j], specs[j], mode, b->plans[j])) return e;
         b->off[j] = bytes;
         bytes = align256(bytes + b->plans[j].bytes);
-        if (specs[j].param == TPCF_R123) max_n = std::max(max_n, b->plans[j].n);
     }
     // CELLS: the plans hold the culling (Hilbert staging, reach, culled
     // items, all done once below); the replayed kernels are FAST's.
     if (mode == TPCF_MODE_CELLS) b->mode = TPCF_MODE_FAST;
-    b->shift = max_n <= kMaxLdsR123Bins ? 2 : 0;
     HIP_TRY(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
     for (auto &e : b->ev) HIP_TRY(hipEventCreate(&e));
     if (int e = b->region.reserve(std::max<size_t>(bytes, 256))) return e;
@@ -1719,6 +1721,9 @@ int tpcf_batch_run(tpcf_batch *b, uint64_t *const *hist, tpcf_stats *stats) {
                                hipMemcpyDeviceToHost, b->st));
         HIP_TRY(hipMemcpyAsync(&rech[j], base + b->off[j] + P.o_cnt, 8, hipMemcpyDeviceToHost, b->st));
     }
+    uint64_t ops = 0;
+    if (b->qd.nitems && b->qd.max_n <= kMaxLdsR123Bins)
+        HIP_TRY(hipMemcpyAsync(&ops, b->qd.counter + 2, sizeof ops, hipMemcpyDeviceToHost, b->st));
     HIP_TRY(hipStreamSynchronize(b->st));
     if (stats) {
         float ms = 0.f;
@@ -1733,6 +1738,7 @@ int tpcf_batch_run(tpcf_batch *b, uint64_t *const *hist, tpcf_stats *stats) {
         }
         s.device_ms = ms;
         s.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        s.mask_ops = ops;
         *stats = s;
     }
     return TPCF_OK;

@@ -20,10 +20,25 @@ __host__ __device__ inline int64_t tile_index(int64_t r, int64_t c, int64_t nblk
 }
 __host__ __device__ inline int64_t tile_count(int64_t nblk) { return nblk * (nblk + 1) / 2; }
 
+// Per-tile summary the (r1,r2,r3) pair stage writes for the bit-mask
+// triplet kernel (bins up to kMaxLdsR123Bins).
+struct TileMeta {
+    uint32_t bins;            // bit b set: some pair of the tile has bin b
+    uint32_t pad;
+    unsigned long long rows;  // bit i set: row i of the tile has some pair with a bin
+};
+// Row masks kept per tile, for its kRowMaskSlots lowest bins present: word
+// [slot * kBlock + i] has bit j set where pair (i, j) of the tile has that
+// bin.  Bins past the stored slots are built by the triplet kernel.
+constexpr int kRowMaskSlots = 4;
+constexpr int kRowMaskWords = kRowMaskSlots * kBlock;
+
 // One (job, term) of the triplet stage.
 struct TripletTerm {
     const uint8_t *tiles;  // pair-bin tiles of the job
     const uint8_t *tile_any;  // per tile: 1 if some pair of it has a bin (nullptr: not kept)
+    const TileMeta *meta;  // per tile (bit-mask kernel; nullptr for wide specs)
+    const unsigned long long *rmask;  // kRowMaskWords per tile (bit-mask kernel)
     int64_t nblk;          // blocks in the job's union layout
     unsigned long long *hist;  // nterm histogram, n^3 uint64 (sorted-side cells)
     int n;                 // bins per side
@@ -71,6 +86,10 @@ struct PairJob {
     const uint8_t *blk_count;  // real points in each block (0..64)
     uint8_t *tiles;
     uint8_t *tile_any;     // per tile: 1 if some pair of it has a bin, else 0 (nullptr: none kept)
+    // Bit-mask triplet kernel (n <= kMaxLdsR123Bins, shift 0): per tile its
+    // TileMeta and kRowMaskWords row masks (nullptr: not written).
+    TileMeta *meta;
+    unsigned long long *rmask;
     const double *T64;     // squared thresholds [0..n]
     const float *T32;      // T64 rounded to float
     unsigned long long *rechecked;  // FP64 recomputations
@@ -79,7 +98,7 @@ struct PairJob {
     int n;
     float rmax;            // max |coordinate - origin| over the union (FP32, rounded up)
     float rel;             // extra relative window around each threshold (squared units)
-    int shift;             // stored pair value = bin << shift (2 for the LDS kernel, 0 otherwise)
+    int shift;             // stored pair value = bin << shift (0: the bin itself)
     // Culled form: bit C of word [R * need_words + C / 64] set for the tiles
     // (R, C) the triplet stage reads; the others are skipped.  nullptr: all.
     const unsigned long long *need;

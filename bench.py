@@ -40,10 +40,13 @@ tp = importlib.import_module("3pcf_amd")
 synth = importlib.import_module("3pcf_amd.synth")
 
 METRIC = "triplets/s at 50k+50k DDD..RRR; batched voxel jobs/s + p99 vs plain, exact"
-# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz vector lane-ops, and the
-# nominal work of one triplet in the pair-bin form (6 VALU lane-ops).
+# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz vector lane-ops.  The
+# (r1,r2,r3) triplet kernel counts the VALU lane-ops of its own cost model
+# (stats.mask_ops: per visited row the y ballots, the x weights, and per
+# (y, z) bin pair of a y bin present in the row two ANDs, two popcounts and
+# the multiply-adds; per accumulator field its wave sum), so `achieved` is
+# the work the kernel visits, not a per-triplet constant.
 VALU_PEAK = 256 * 64 * 2.4e9
-OPS_PER_TRIPLET = 6.0
 
 
 def headline_catalogs(nd: int, nr: int):
@@ -882,7 +885,8 @@ def bench_headline(args) -> int:
     counted = int(out.sum())
     value = triplets * steps / dt
     tri_s = sum(s.triplet_ms for s in stats) / steps / 1e3
-    achieved = OPS_PER_TRIPLET * triplets / tri_s if tri_s > 0 else 0.0
+    ops = sum(s.mask_ops for s in stats) / steps
+    achieved = ops / tri_s if tri_s > 0 else 0.0
     line = {
         "metric": METRIC,
         "value": value,
@@ -913,16 +917,15 @@ def bench_headline(args) -> int:
             "pairs_rechecked_fraction": (stats[0].pairs_rechecked / stats[0].pairs) if stats[0].pairs else 0.0,
             "pair_stage_ms": stats[0].pair_ms,
             "triplet_kernel_ms": tri_s * 1e3,
-            # nominal LDS traffic of the pair-bin form: 8 B per triplet against
-            # 128 B/clk/CU x 256 CUs x 2.4 GHz
-            "lds_roofline_frac": (8.0 * triplets / tri_s) / (128 * 256 * 2.4e9) if tri_s > 0 else 0.0,
+            "kernel_valu_lane_ops": ops,
+            "valu_lane_ops_per_triplet": ops / triplets if triplets else 0.0,
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "valu",
             "achieved": achieved / 1e12,
             "peak": VALU_PEAK / 1e12,
-            "unit": "TOP/s (VALU lane-ops, 6 per triplet; every triplet of the step is in range and visited)",
+            "unit": "TOP/s (VALU lane-ops of the bit-mask triplet kernel's cost model, counted by the kernel)",
             "frac": achieved / VALU_PEAK,
             "traffic": measured_traffic(args),
         },

@@ -122,6 +122,8 @@ typedef struct {
     double pair_ms;            /* pair stage (FAST modes) */
     double triplet_ms;         /* triplet kernel alone */
     double queue_ms;           /* serving: submit to admission */
+    uint64_t mask_ops;         /* (r1,r2,r3) bit-mask triplet kernel: VALU lane-ops of its
+                                  cost model (whole launch; 0 for other kernels) */
 } tpcf_stats;
 
 /* ---- library / context ------------------------------------------------- */

@@ -107,7 +107,7 @@ def test_short_jobs_overtake_a_running_long_job():
         spec = tp.r123_spec(20, 1.0, 150.0, "log")
         sqt = tp.binspec(tp.SQT, [tp.axis(8, 2.0, 10.0, "log"), tp.axis(8, 1.0, 3.0, "lin"),
                                   tp.axis(16, 0.0, 1.0, "lin")])
-        (long_job,), keep = jobs_of(ctx, 1, 20000, seed=900)
+        (long_job,), keep = jobs_of(ctx, 1, 40000, seed=900)
         shorts, keep2 = jobs_of(ctx, 2, 3000, seed=910)
         work = [(long_job, spec), (shorts[0], spec), (shorts[1], sqt)]
         sub = submit_all(ctx, [long_job], [spec])
