@@ -83,6 +83,10 @@ class Stats(C.Structure):
         ("triplet_ms", C.c_double),
         ("queue_ms", C.c_double),
         ("mask_ops", C.c_uint64),
+        ("tile_bins_mean", C.c_double),
+        ("tile_fill", C.c_double),
+        ("triplet_form", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -48,9 +48,9 @@ __device__ inline int search_le(F s, const F *T, int n) {
     return lo;
 }
 
-// Stored pair value: the bin, or the sentinel outside [0, n).
-__device__ inline uint32_t bin_of(int m, int n) {
-    return (m >= 0 && m < n) ? (uint32_t)m : kSentinel;
+// Stored pair value: bin << shift, or the sentinel outside [0, n).
+__device__ inline uint32_t bin_of(int m, int n, int shift) {
+    return (m >= 0 && m < n) ? ((uint32_t)m << shift) : kSentinel;
 }
 
 __device__ inline double sq64(const double *u, int64_t npad, int64_t a, int64_t b) {
@@ -81,12 +81,13 @@ __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
     __shared__ float4 sR[kBlock], sC[kBlock];
     __shared__ double sT64[kMaxBins + 1];
     __shared__ float sT32[kMaxBins + 1];
-    __shared__ uint32_t s_bins;
+    __shared__ uint32_t s_bins, s_pairs;
     __shared__ unsigned long long s_rows;
     const int t = threadIdx.x;
     const int n = J.n;
     if (t == 0) {
         s_bins = 0u;
+        s_pairs = 0u;
         s_rows = 0ull;
     }
     for (int m = t; m <= n; m += 256) {
@@ -141,12 +142,12 @@ __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
                 }
                 if (nearb) {
                     ++nrecheck;
-                    b = bin_of(search_le(sq64(J.u64, J.npad, gi, gj), sT64, n), n);
+                    b = bin_of(search_le(sq64(J.u64, J.npad, gi, gj), sT64, n), n, J.shift);
                 } else {
-                    b = bin_of(m, n);
+                    b = bin_of(m, n, J.shift);
                 }
             } else {
-                b = bin_of(search_le(sq64(J.u64, J.npad, gi, gj), sT64, n), n);
+                b = bin_of(search_le(sq64(J.u64, J.npad, gi, gj), sT64, n), n, J.shift);
             }
         }
         packed[q >> 2] |= (b & 0xFFu) << (8 * (q & 3));
@@ -164,24 +165,34 @@ __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
         if (t == 0 && J.tile_any) J.tile_any[tile] = tile_has ? 1 : 0;
         return;
     }
-    // Bit-mask kernel (pair values are the bins themselves, n <= 32): the
-    // tile's bins and non-empty rows, then the row masks of its lowest
-    // kRowMaskSlots bins.
-    uint32_t bins = 0u;
+    // Bit-mask kernel (pair values 4 * bin, n <= 32): the tile's bins and
+    // non-empty rows, then the row masks of its lowest kRowMaskSlots bins,
+    // and the job's count of bins over its tiles (64 slots, by tile index).
+    uint32_t bins = 0u, npairs = 0u;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const uint32_t b = (packed[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-        if (b != kSentinel) bins |= 1u << b;
+        if (b != kSentinel) {
+            bins |= 1u << (b >> 2);
+            ++npairs;
+        }
     }
     if (bins) {
         atomicOr(&s_bins, bins);
         atomicOr(&s_rows, 1ull << i);
+        atomicAdd(&s_pairs, npairs);
     }
     __syncthreads();
     const uint32_t tb = s_bins;
     if (t == 0) {
         if (J.tile_any) J.tile_any[tile] = tb ? 1 : 0;
         J.meta[tile] = TileMeta{tb, 0u, s_rows};
+        if (tb) {
+            unsigned long long *st = J.tstat + 3 * (tile & 63);
+            atomicAdd(st, (unsigned long long)__builtin_popcount(tb));
+            atomicAdd(st + 1, 1ull);
+            atomicAdd(st + 2, (unsigned long long)s_pairs);
+        }
     }
     uint32_t rest = tb;
     for (int slot = 0; slot < kRowMaskSlots && rest; ++slot) {
@@ -190,7 +201,7 @@ __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
         uint32_t part = 0u;
 #pragma unroll
         for (int q = 0; q < 16; ++q)
-            part |= (uint32_t)(((packed[q >> 2] >> (8 * (q & 3))) & 0xFFu) == bin) << q;
+            part |= (uint32_t)(((packed[q >> 2] >> (8 * (q & 3))) & 0xFFu) == bin << 2) << q;
         // The four threads of row i hold its columns 0-15, 16-31, 32-47, 48-63.
         unsigned long long m = (unsigned long long)part << jb;
         m |= __shfl_xor(m, 1);
@@ -206,12 +217,12 @@ __global__ __launch_bounds__(256) void pair_tiles_kernel(PairJob J) {
 // LDS histogram of one term with n bins: the unsorted cube, word
 // (z * n + y) * n + x for x = bin(i,j), y = bin(i,k), z = bin(j,k), folded
 // into the term's sorted uint64 cells on flush.
-__host__ __device__ inline int r123_lds_words(int n) { return n * n * n; }
+__host__ __device__ inline int bits_lds_words(int n) { return n * n * n; }
 
 // Fold the unsorted cube into the term's sorted uint64 histogram and clear it.
-__device__ inline void r123_flush(uint32_t *hist, int n, unsigned long long *g) {
+__device__ inline void bits_flush(uint32_t *hist, int n, unsigned long long *g) {
     __syncthreads();
-    const int words = r123_lds_words(n);
+    const int words = bits_lds_words(n);
     for (int c = threadIdx.x; c < words; c += blockDim.x) {
         const uint32_t v = hist[c];
         if (!v) continue;
@@ -261,7 +272,7 @@ __device__ inline uint32_t wave_sum(uint32_t x) {
 __device__ inline unsigned long long build_row_mask(global_u8_ptr g, uint32_t bin, int lane) {
     uint32_t lo = 0u, hi = 0u;
     for (int r = 0; r < kBlock; ++r) {
-        const unsigned long long m = __ballot((uint32_t)g[r * kBlock + lane] == bin);
+        const unsigned long long m = __ballot((uint32_t)g[r * kBlock + lane] == bin << 2);
         if (lane == r) {
             lo = (uint32_t)m;
             hi = (uint32_t)(m >> 32);
@@ -288,7 +299,7 @@ __device__ inline unsigned long long build_row_mask(global_u8_ptr g, uint32_t bi
 // present in the row 2 ANDs, 2 popcounts and one multiply-add per
 // accumulator word; per accumulator field 9 for its wave sum.
 template <int NX, int NY, int NZ>
-__device__ __forceinline__ unsigned long long r123_pass(uint32_t *hist, global_u8_ptr gAB, global_u8_ptr gAC,
+__device__ __forceinline__ unsigned long long bits_pass(uint32_t *hist, global_u8_ptr gAB, global_u8_ptr gAC,
                                           unsigned long long rows, const uint32_t (&xs)[3],
                                           const uint32_t (&ys)[3], const uint32_t (&zs)[3],
                                           const unsigned long long (&zm)[3], int lane, uint32_t n) {
@@ -309,9 +320,9 @@ __device__ __forceinline__ unsigned long long r123_pass(uint32_t *hist, global_u
         const uint32_t ybn = gAC[inext * kBlock + lane], xbn = gAB[inext * kBlock + lane];
         unsigned long long ym[NY];
 #pragma unroll
-        for (int a = 0; a < NY; ++a) ym[a] = __ballot(yb == ys[a]);
-        const uint32_t w0 = xb == xs[0] ? 1u : (NX > 1 && xb == xs[1] ? 65536u : 0u);
-        const uint32_t w1 = NX > 2 && xb == xs[2] ? 65536u : 0u;
+        for (int a = 0; a < NY; ++a) ym[a] = __ballot(yb == ys[a] << 2);
+        const uint32_t w0 = xb == xs[0] << 2 ? 1u : (NX > 1 && xb == xs[1] << 2 ? 65536u : 0u);
+        const uint32_t w1 = NX > 2 && xb == xs[2] << 2 ? 65536u : 0u;
 #pragma unroll
         for (int a = 0; a < NY; ++a) {
             if (!ym[a]) continue;
@@ -363,17 +374,20 @@ __device__ __forceinline__ int take3(uint32_t *rest, uint32_t (&v)[3]) {
 // blocks of an item (wave w takes C = c0+w, c0+w+WAVES, ...) and share the
 // LDS histogram.  A block triple is counted bin by bin: the bins present in
 // its three tiles (TileMeta, from the pair stage) are taken up to three per
-// side at a time, and each pass (r123_pass) counts every triplet whose three
+// side at a time, and each pass (bits_pass) counts every triplet whose three
 // bins are in the pass's sets, so the passes together count each triplet of
 // the block triple once.  Only rows of block A with some pair in range in
 // both tiles (A,B) and (A,C) are visited; block pairs and C blocks whose
 // tiles have no pair in range are skipped.
 template <int WAVES, bool ELASTIC>
-__global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticCtl ctl,
+__global__ __launch_bounds__(WAVES * 64) void r123_bits_kernel(const ElasticCtl ctl,
                                                                   const TripletItem *__restrict__ items,
                                                                   int nitems, int *__restrict__ counter,
                                                                   const TripletTerm *__restrict__ terms,
-                                                                  int lds_words, unsigned long long *ops) {
+                                                                  int lds_words, int *ctlw) {
+    // Launched beside the run-merging kernel; runs only when chosen.
+    if (__builtin_amdgcn_readfirstlane(ctlw[kCtlForm]) != kFormBits) return;
+    unsigned long long *ops = reinterpret_cast<unsigned long long *>(ctlw + kCtlOps);
     extern __shared__ uint32_t hist[];
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
@@ -409,7 +423,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticC
         const int c1 = __builtin_amdgcn_readfirstlane(I.c1);
         const unsigned long long add = (unsigned long long)(c1 - c0) * (kBlock * kBlock * kBlock);
         if (term != cur || since_flush + add > flush_limit) {
-            if (cur >= 0) r123_flush(hist, cur_n, cur_hist);
+            if (cur >= 0) bits_flush(hist, cur_n, cur_hist);
             cur = term;
             cur_n = n;
             cur_hist = T.hist;
@@ -457,7 +471,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticC
                         switch ((nx - 1) * 9 + (ny - 1) * 3 + (nz - 1)) {
 #define TPCF_PASS(X, Y, Z)                                                                     \
     case (X - 1) * 9 + (Y - 1) * 3 + (Z - 1):                                                  \
-        wave_ops += r123_pass<X, Y, Z>(hist, gAB, gAC, rows, xs, ys, zs, zm, lane, (uint32_t)n); \
+        wave_ops += bits_pass<X, Y, Z>(hist, gAB, gAC, rows, xs, ys, zs, zm, lane, (uint32_t)n); \
         break;
                             TPCF_PASS(1, 1, 1) TPCF_PASS(1, 1, 2) TPCF_PASS(1, 1, 3)
                             TPCF_PASS(1, 2, 1) TPCF_PASS(1, 2, 2) TPCF_PASS(1, 2, 3)
@@ -475,8 +489,252 @@ __global__ __launch_bounds__(WAVES * 64) void r123_triplet_kernel(const ElasticC
             }
         }
     }
-    if (cur >= 0) r123_flush(hist, cur_n, cur_hist);
-    if (ops && lane == 0 && wave_ops) atomicAdd(ops, wave_ops);
+    if (cur >= 0) bits_flush(hist, cur_n, cur_hist);
+    if (lane == 0 && wave_ops) atomicAdd(ops, wave_ops);
+    if constexpr (ELASTIC)
+        if (threadIdx.x == 0 && !left) atomicSub(ctl.active, 1);
+}
+
+// ---------------------------------------------------------------------------
+// triplet stage, run-merging form (block triples with many bins per tile)
+// ---------------------------------------------------------------------------
+
+// Pair bins are stored pre-scaled as 4*bin (a byte offset into a row of
+// 32-bit counters); 0xFF is the sentinel, and every valid value is < 128.
+//
+// LDS histogram of one term with n bins: unsorted cube word
+// z*sz + y*sy + x for x = bin(i,j), y = bin(i,k), z = bin(j,k), with odd
+// strides sy = n|1 and sz = (n*sy)|1 so that the lanes of one instruction
+// (same x, varying y and z) spread over all 32 banks.  A lane whose z is out
+// of range adds into a dump word n*sz + lane + y*sy + x instead of branching,
+// so the dump needs sz + 64 extra words.  COPIES privatized copies of the cube
+// (lane k writes copy k % COPIES) cut same-address serialization of LDS
+// atomics when few cells are hot; copies start 32/COPIES banks apart.
+__host__ __device__ inline int runs_sy(int n) { return n | 1; }
+__host__ __device__ inline int runs_sz(int n) { return (n * runs_sy(n)) | 1; }
+__host__ __device__ inline int runs_lds_words(int n) { return (n + 1) * runs_sz(n) + 64; }
+// Copy k of the privatized histogram starts k * stride words in; the stride
+// is a multiple of 32 words plus 16/copies, so the hot cells of the copies
+// fall in different banks (measured on the 50k+50k four-term workload:
+// skew 4 for 4 copies is 5 % faster than skew 8).
+__host__ __device__ inline int runs_copy_stride(int n, int copies) {
+    return ((runs_lds_words(n) + 31) & ~31) + (copies > 1 ? 16 / copies : 0);
+}
+
+// Fold the privatized unsorted cubes into the term's sorted uint64 histogram
+// and clear them (dump words included).
+template <int COPIES>
+__device__ inline void runs_flush(uint32_t *hist, int n, unsigned long long *g) {
+    __syncthreads();
+    const int sy = runs_sy(n), sz = runs_sz(n);
+    const int words = runs_lds_words(n);
+    const int stride = runs_copy_stride(n, COPIES);
+    for (int c = threadIdx.x; c < words; c += blockDim.x) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < COPIES; ++k) {
+            v += hist[k * stride + c];
+            hist[k * stride + c] = 0u;
+        }
+        int z = c / sz, y = (c % sz) / sy, x = (c % sz) % sy, t;
+        if (z < n && y < n && x < n && v) {
+            if (x > y) { t = x; x = y; y = t; }
+            if (y > z) { t = y; y = z; z = t; }
+            if (x > y) { t = x; x = y; y = t; }
+            atomicAdd(g + (x * n + y) * n + z, (unsigned long long)v);
+        }
+    }
+    __syncthreads();
+}
+
+typedef const uint32_t __attribute__((address_space(4))) *scalar_u32_ptr;
+
+// Column j of a fully valid row: lanes whose run of equal cells ends at j
+// (the carry out of doubling w, which holds the run ends of the remaining
+// columns from the top bit down) add the run's length to its cell.  Only
+// those lanes reach the LDS (an add from every lane, the others into their
+// dump slots, measured 10 % slower on the 50k + 50k workload).
+__device__ __forceinline__ void runs_column(uint32_t *hist, uint32_t &w, int j, int &last, uint32_t zs,
+                                            uint32_t ys, uint32_t x4) {
+    uint32_t w2;
+    const bool end = __builtin_add_overflow(w, w, &w2);
+    w = w2;
+    if (end) {
+        const uint32_t run = (uint32_t)(j - last);
+        last = j;
+        lds_add(hist, zs + ys + x4, run);
+    }
+}
+
+// Persistent: workgroups pull items (term, A, B, C-range) from a global
+// counter until the queue is drained; the waves of a workgroup split the C
+// blocks of an item (wave w takes C = c0+w, c0+w+WAVES, ...) and share the
+// LDS histogram copies.  Lane k owns point k of block C: z = bin(j,k) is
+// preloaded for the 64 columns j of block B, y = bin(i,k) is per lane and
+// fixed for a row i of block A, and x = bin(i,j) is uniform, read with scalar
+// loads.  The blocks hold spatially ordered (Hilbert) points, so along a row a
+// lane's cell (x, y, z) often stays the same for several consecutive
+// columns: each lane adds one run of equal cells at a time (the run's length
+// at its last column).  A run ends where x changes along the row (one ballot
+// per row) or z changes along the lane's column (one mask per C block), so
+// the add is issued only by the lanes whose run ends there: fewer LDS atomics
+// and fewer lanes per atomic, hence fewer bank and same-address conflicts.
+// Every triplet is still counted one by one in its run.  A row with no valid
+// x is skipped; a row with some invalid x (sparse rows of culled or wide
+// geometries, where the run bookkeeping per column costs more than it saves)
+// adds one triplet per valid column, the invalid ones skipped by a scalar
+// test.
+template <int WAVES, int COPIES, bool ELASTIC>
+__global__ __launch_bounds__(WAVES * 64) void r123_runs_kernel(const ElasticCtl ctl,
+                                                                  const TripletItem *__restrict__ items,
+                                                                  int nitems, int *__restrict__ counter,
+                                                                  const TripletTerm *__restrict__ terms,
+                                                                  int lds_words, const int *ctlw) {
+    // Launched beside the bit-mask kernel; runs only when chosen.
+    if (__builtin_amdgcn_readfirstlane(ctlw[kCtlForm]) != kFormRuns) return;
+    extern __shared__ uint32_t hist[];
+    __shared__ int s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // Clear the whole allocation once; every flush clears what its term used,
+    // so terms of any bin count start from zero.
+    for (int c = threadIdx.x; c < lds_words; c += blockDim.x) hist[c] = 0u;
+    bool left = false;
+    if constexpr (ELASTIC)
+        if (threadIdx.x == 0) atomicAdd(ctl.active, 1);
+    int cur = -1, cur_n = 0;
+    unsigned long long since_flush = 0;
+    unsigned long long *cur_hist = nullptr;
+    // One C block adds at most 64^3 counts to a cell; flush the 32-bit LDS
+    // counters well before they could wrap.
+    const unsigned long long flush_limit = 0xC0000000ull;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if constexpr (ELASTIC) left = elastic_leave(ctl);
+            s_item = left ? nitems : atomicAdd(counter, 1);
+        }
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= nitems) break;
+        const TripletItem I = items[it];
+        const int term = __builtin_amdgcn_readfirstlane(I.term);
+        const TripletTerm T = terms[term];
+        const int n = __builtin_amdgcn_readfirstlane(T.n);
+        const int A = __builtin_amdgcn_readfirstlane(I.A);
+        const int B = __builtin_amdgcn_readfirstlane(I.B);
+        const int c0 = __builtin_amdgcn_readfirstlane(I.c0);
+        const int c1 = __builtin_amdgcn_readfirstlane(I.c1);
+        const unsigned long long add = (unsigned long long)(c1 - c0) * (kBlock * kBlock * kBlock);
+        if (term != cur || since_flush + add > flush_limit) {
+            if (cur >= 0) runs_flush<COPIES>(hist, cur_n, cur_hist);
+            cur = term;
+            cur_n = n;
+            cur_hist = T.hist;
+            since_flush = 0;
+        }
+        since_flush += add;
+        const int64_t nblk = T.nblk;
+        // No triplet of the item counts when its (A, B) tile has no pair
+        // with a bin.
+        const uint8_t *tany = T.tile_any;
+        if (tany && !tany[tile_index(A, B, nblk)]) continue;
+        const uint32_t sy = (uint32_t)runs_sy(n), sz = (uint32_t)runs_sz(n);
+        const uint32_t copy_base = (uint32_t)((lane % COPIES) * runs_copy_stride(n, COPIES)) * 4u;
+        const uint32_t dump = copy_base + ((uint32_t)n * sz + (uint32_t)lane) * 4u;
+        const uint8_t *tAB = T.tiles + tile_index(A, B, nblk) * kTileBytes;
+        const global_u8_ptr gAB = (global_u8_ptr)tAB;
+        for (int C = c0 + wave; C < c1; C += WAVES) {
+            // Likewise a C block whose (A, C) or (B, C) tile has none.
+            if (tany && !(tany[tile_index(A, C, nblk)] & tany[tile_index(B, C, nblk)])) continue;
+            const global_u8_ptr gAC = (global_u8_ptr)(T.tiles + tile_index(A, C, nblk) * kTileBytes);
+            const global_u8_ptr gBC = (global_u8_ptr)(T.tiles + tile_index(B, C, nblk) * kTileBytes);
+            // z of the 64 columns as byte offsets of the z slab in the lane's
+            // copy, or the lane's dump slot for the sentinel.
+            uint32_t Zs[kBlock];
+#pragma unroll
+            for (int j = 0; j < kBlock; ++j) Zs[j] = gBC[j * kBlock + lane];
+            // Row i's y and x bytes are loaded during row i-1.
+            uint32_t y4n = gAC[lane], xvn = gAB[lane];
+#pragma unroll
+            for (int j = 0; j < kBlock; ++j) Zs[j] = Zs[j] < 128u ? __umul24(Zs[j], sz) + copy_base : dump;
+            // Run ends along the lane's column: bit 63-j set where the z slot
+            // of column j differs from that of column j+1 (and at j = 63);
+            // computed by the first row of the C block that merges runs.
+            unsigned long long zend = 0ull;
+            bool have_zend = false;
+            for (int i = 0; i < kBlock; ++i) {
+                const uint32_t y4 = y4n;
+                const uint32_t xv = xvn;
+                const int inext = i + 1 < kBlock ? i + 1 : i;
+                y4n = gAC[inext * kBlock + lane];
+                xvn = gAB[inext * kBlock + lane];
+                const unsigned long long rowmask = __ballot(xv < 128u);
+                if (rowmask == 0ull) continue;
+                // Fully valid rows merge runs.  Their run ends are found
+                // here, in uniform control flow (every lane active), before
+                // the lanes with invalid y drop out.
+                const bool runs = rowmask == ~0ull;
+                unsigned long long xend = 0ull;
+                if (runs) {
+                    // Along the row: x of column j differs from x of column
+                    // j+1 (lane j+1's value by a wave shift left; lane 63
+                    // keeps its own), bit j of the ballot reversed to 63-j.
+                    const uint32_t xv1 = (uint32_t)__builtin_amdgcn_update_dpp((int)xv, (int)xv, 0x130, 0xf, 0xf,
+                                                                                false);  // wave_shl:1
+                    xend = __builtin_bitreverse64(__ballot(xv != xv1));
+                    if (!have_zend) {
+                        zend = 1ull;
+#pragma unroll
+                        for (int j = 0; j + 1 < kBlock; ++j)
+                            zend |= (unsigned long long)(Zs[j] != Zs[j + 1]) << (kBlock - 1 - j);
+                        have_zend = true;
+                    }
+                }
+                if (y4 >= 128u) continue;
+                const uint32_t Ys = __umul24(y4, sy);
+                // x(i, j) for the 64 columns: one scalar load of the row.
+                // (Reading lane j of xv instead would be wrong here: inside
+                // the y-divergent region the compiler may rebuild xv in a
+                // fresh register for the active lanes only.)
+                const scalar_u32_ptr xrow = (scalar_u32_ptr)(tAB + i * kBlock);
+                uint32_t xw[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) xw[q] = xrow[q];
+                // The row's words are waited for here, once: the columns
+                // below may each be skipped, and a wait inside them would
+                // also wait for the wave's outstanding LDS adds (scalar loads
+                // and LDS share a counter).  Emits no instruction.
+                asm volatile("" ::"s"(xw[0]), "s"(xw[1]), "s"(xw[2]), "s"(xw[3]), "s"(xw[4]), "s"(xw[5]),
+                             "s"(xw[6]), "s"(xw[7]), "s"(xw[8]), "s"(xw[9]), "s"(xw[10]), "s"(xw[11]),
+                             "s"(xw[12]), "s"(xw[13]), "s"(xw[14]), "s"(xw[15]));
+                const unsigned long long ends = xend | zend;
+                uint32_t whi = (uint32_t)(ends >> 32), wlo = (uint32_t)ends;
+                int last = -1;
+                if (runs) {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int j = q * 4 + r;
+                            const uint32_t x4 = (xw[q] >> (8 * r)) & 0xFFu;
+                            runs_column(hist, j < 32 ? whi : wlo, j, last, Zs[j], Ys, x4);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const uint32_t x4 = (xw[q] >> (8 * r)) & 0xFFu;
+                            if (x4 < 128u) lds_add(hist, Zs[q * 4 + r] + Ys + x4, 1u);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (cur >= 0) runs_flush<COPIES>(hist, cur_n, cur_hist);
     if constexpr (ELASTIC)
         if (threadIdx.x == 0 && !left) atomicSub(ctl.active, 1);
 }
@@ -622,42 +880,110 @@ hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st) {
     return hipGetLastError();
 }
 
-#define TPCF_R123_SHAPES(X) X(4) X(8) X(16)
+// Picks the triplet kernel of a launch from the pair stages' tile counts
+// (every term of the launch, 64 slots each): over the tiles with a bin, the
+// mean number of bins m and the fraction f of their pairs with one.  The
+// bit-mask kernel's work grows with the bin pairs of a block triple (about
+// m^2), the run-merging kernel's with its triplets in range (about f), so the
+// bit-mask kernel is chosen when m <= max_bins * sqrt(f).  force >= 0 chooses
+// that form.  One 64-thread workgroup; writes ctlw[kCtlForm], and m and f
+// (float bits) to ctlw[kCtlBinsMean], ctlw[kCtlFill].
+__global__ __launch_bounds__(64) void r123_choose_kernel(const TripletTerm *__restrict__ terms, int nterms,
+                                                         int *ctlw, float max_bins, int force) {
+    const int t = threadIdx.x;
+    unsigned long long nb = 0, nt = 0, np = 0;
+    for (int q = 0; q < nterms; ++q) {
+        const unsigned long long *st = terms[q].tstat;
+        if (!st) continue;
+        nb += st[3 * t];
+        nt += st[3 * t + 1];
+        np += st[3 * t + 2];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        nb += __shfl_xor(nb, o);
+        nt += __shfl_xor(nt, o);
+        np += __shfl_xor(np, o);
+    }
+    if (t == 0) {
+        const float mean = nt ? (float)((double)nb / (double)nt) : 0.f;
+        const float fill = nt ? (float)((double)np / ((double)nt * kTileBytes)) : 0.f;
+        ctlw[kCtlForm] = force >= 0 ? force : (mean <= max_bins * sqrtf(fill) ? kFormBits : kFormRuns);
+        ctlw[kCtlBinsMean] = __float_as_int(mean);
+        ctlw[kCtlFill] = __float_as_int(fill);
+    }
+}
 
-// One histogram copy per workgroup: `copies` must be 1.
+#define TPCF_R123_SHAPES(X)                                                                                    \
+    X(2, 1) X(4, 1) X(8, 1) X(16, 1) X(2, 2) X(4, 2) X(8, 2) X(16, 2) X(2, 4) X(4, 4) X(8, 4) X(16, 4) X(2, 8)   \
+        X(4, 8) X(8, 8) X(16, 8)
+#define TPCF_BITS_SHAPES(X) X(4) X(8) X(16)
+
 const void *r123_triplet_kernel_ptr(int waves, int copies, bool elastic) {
-    if (copies != 1) return nullptr;
-#define TPCF_PTR(W)                                                                                      \
-    if (waves == W)                                                                                      \
-        return elastic ? reinterpret_cast<const void *>(&r123_triplet_kernel<W, true>)                    \
-                       : reinterpret_cast<const void *>(&r123_triplet_kernel<W, false>);
+#define TPCF_PTR(W, K)                                                                                   \
+    if (waves == W && copies == K)                                                                       \
+        return elastic ? reinterpret_cast<const void *>(&r123_runs_kernel<W, K, true>)                    \
+                       : reinterpret_cast<const void *>(&r123_runs_kernel<W, K, false>);
     TPCF_R123_SHAPES(TPCF_PTR)
 #undef TPCF_PTR
     return nullptr;
 }
 
-int r123_lds_bytes(int n, int copies) { return copies * r123_lds_words(n) * 4; }
+const void *r123_bits_kernel_ptr(int waves, bool elastic) {
+#define TPCF_PTR(W)                                                                                      \
+    if (waves == W)                                                                                      \
+        return elastic ? reinterpret_cast<const void *>(&r123_bits_kernel<W, true>)                       \
+                       : reinterpret_cast<const void *>(&r123_bits_kernel<W, false>);
+    TPCF_BITS_SHAPES(TPCF_PTR)
+#undef TPCF_PTR
+    return nullptr;
+}
+
+int r123_lds_bytes(int n, int copies) { return copies * runs_copy_stride(n, copies) * 4; }
+int r123_bits_lds_bytes(int n) { return bits_lds_words(n) * 4; }
+
+hipError_t launch_r123_choose(const TripletTerm *terms, int nterms, int *counter, float max_bins, int force,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(r123_choose_kernel, dim3(1), dim3(64), 0, st, terms, nterms, counter, max_bins, force);
+    return hipGetLastError();
+}
 
 hipError_t launch_r123_triplets(const ElasticCtl &ctl, const TripletItem *items, int nitems, int *counter,
                                 const TripletTerm *terms,
                                 int max_n, int waves, int copies, int grid, hipStream_t st) {
-    // The kernel's modelled VALU lane-ops go to the 64-bit word after the
-    // item counter (counter[2..3]).
-    unsigned long long *ops = reinterpret_cast<unsigned long long *>(counter + 2);
-    if (max_n < 1 || max_n > kMaxLdsR123Bins || copies != 1) return hipErrorInvalidValue;
-    const int words = r123_lds_words(max_n);
+    if (max_n < 1 || max_n > kMaxLdsR123Bins) return hipErrorInvalidValue;
+    const int words = copies * runs_copy_stride(max_n, copies);
+    const size_t lds = (size_t)words * 4;
+#define TPCF_LAUNCH(W, K)                                                                                    \
+    if (waves == W && copies == K) {                                                                         \
+        if (ctl.active)                                                                                      \
+            hipLaunchKernelGGL((r123_runs_kernel<W, K, true>), dim3(grid), dim3(W * 64), lds, st, ctl, items,   \
+                               nitems, counter, terms, words, counter);                                      \
+        else                                                                                                 \
+            hipLaunchKernelGGL((r123_runs_kernel<W, K, false>), dim3(grid), dim3(W * 64), lds, st, ctl, items,  \
+                               nitems, counter, terms, words, counter);                                      \
+        return hipGetLastError();                                                                            \
+    }
+    TPCF_R123_SHAPES(TPCF_LAUNCH)
+#undef TPCF_LAUNCH
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_r123_bits(const ElasticCtl &ctl, const TripletItem *items, int nitems, int *counter,
+                            const TripletTerm *terms, int max_n, int waves, int grid, hipStream_t st) {
+    if (max_n < 1 || max_n > kMaxLdsR123Bins) return hipErrorInvalidValue;
+    const int words = bits_lds_words(max_n);
     const size_t lds = (size_t)words * 4;
 #define TPCF_LAUNCH(W)                                                                                       \
     if (waves == W) {                                                                                        \
         if (ctl.active)                                                                                      \
-            hipLaunchKernelGGL((r123_triplet_kernel<W, true>), dim3(grid), dim3(W * 64), lds, st, ctl, items,  \
-                               nitems, counter, terms, words, ops);                                          \
+            hipLaunchKernelGGL((r123_bits_kernel<W, true>), dim3(grid), dim3(W * 64), lds, st, ctl, items,      \
+                               nitems, counter, terms, words, counter);                                      \
         else                                                                                                 \
-            hipLaunchKernelGGL((r123_triplet_kernel<W, false>), dim3(grid), dim3(W * 64), lds, st, ctl, items, \
-                               nitems, counter, terms, words, ops);                                          \
+            hipLaunchKernelGGL((r123_bits_kernel<W, false>), dim3(grid), dim3(W * 64), lds, st, ctl, items,     \
+                               nitems, counter, terms, words, counter);                                      \
         return hipGetLastError();                                                                            \
     }
-    TPCF_R123_SHAPES(TPCF_LAUNCH)
+    TPCF_BITS_SHAPES(TPCF_LAUNCH)
 #undef TPCF_LAUNCH
     return hipErrorInvalidValue;
 }

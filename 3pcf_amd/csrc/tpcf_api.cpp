@@ -373,7 +373,7 @@ struct JobPlan {
     uint64_t triplets = 0;
     // byte offsets inside the job's device region
     size_t o_u64 = 0, o_u32 = 0, o_blk = 0, o_T64 = 0, o_T32 = 0, o_Eq = 0, o_Et = 0, o_cnt = 0, o_hist = 0,
-           o_tiles = 0, o_any = 0, o_dists = 0, o_masks = 0, o_meta = 0, o_rmask = 0;
+           o_tiles = 0, o_any = 0, o_dists = 0, o_masks = 0, o_meta = 0, o_rmask = 0, o_tstat = 0;
     size_t bytes = 0;
     bool need_tiles = true;
     bool bitmask = false;  // tile summaries and row masks for the bit-mask triplet kernel
@@ -468,6 +468,7 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
     P.o_meta = o; o = align256(o + (bitmask ? (size_t)tile_count(P.L.nblk) * sizeof(TileMeta) : 0));
     P.o_rmask = o;
     o = align256(o + (bitmask ? (size_t)tile_count(P.L.nblk) * kRowMaskWords * sizeof(unsigned long long) : 0));
+    P.o_tstat = o; o = align256(o + (bitmask ? 192 * sizeof(unsigned long long) : 0));
     P.bitmask = bitmask;
     const bool sqt_fast = spec.param != TPCF_R123 && mode != TPCF_MODE_PLAIN;
     P.o_dists = o; o = align256(o + (sqt_fast ? (size_t)tile_count(P.L.nblk) * kTileBytes * sizeof(float) : 0));
@@ -561,6 +562,7 @@ static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStrea
         HIP_TRY(hipMemcpyAsync(base + P.o_Et, P.Et.data(), sizeof(double) * P.Et.size(), hipMemcpyHostToDevice, st));
     }
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(base + P.o_cnt), 16, 0u, st));
+    if (P.bitmask) HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(base + P.o_tstat), 384, 0u, st));
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(base + P.o_hist), 2 * P.cells * P.nterms, 0u, st));
     if (P.need_tiles && P.L.npad) HIP_TRY(launch_to_f32(u64, P.L.npad, P.origin, reinterpret_cast<float4 *>(base + P.o_u32), st));
     return TPCF_OK;
@@ -592,15 +594,20 @@ static int stage_jobs(const tpcf_job *const *jobs, const JobPlan *const *plans, 
     return TPCF_OK;
 }
 
-static PairJob pair_job(const JobPlan &P, char *base) {
+// shift: 2 when the launch's triplet kernels are the LDS ones (every
+// (r1,r2,r3) job of it has at most kMaxLdsR123Bins bins), 0 for the wide one.
+static PairJob pair_job(const JobPlan &P, char *base, int shift) {
     PairJob J{};
     J.u64 = reinterpret_cast<const double *>(base + P.o_u64);
     J.u32 = reinterpret_cast<const float4 *>(base + P.o_u32);
     J.blk_count = reinterpret_cast<const uint8_t *>(base + P.o_blk);
     J.tiles = reinterpret_cast<uint8_t *>(base + P.o_tiles);
     J.tile_any = reinterpret_cast<uint8_t *>(base + P.o_any);
-    J.meta = P.bitmask ? reinterpret_cast<TileMeta *>(base + P.o_meta) : nullptr;
-    J.rmask = P.bitmask ? reinterpret_cast<unsigned long long *>(base + P.o_rmask) : nullptr;
+    const bool bits = P.bitmask && shift == 2;
+    J.meta = bits ? reinterpret_cast<TileMeta *>(base + P.o_meta) : nullptr;
+    J.rmask = bits ? reinterpret_cast<unsigned long long *>(base + P.o_rmask) : nullptr;
+    J.tstat = bits ? reinterpret_cast<unsigned long long *>(base + P.o_tstat) : nullptr;
+    J.shift = shift;
     J.T64 = reinterpret_cast<const double *>(base + P.o_T64);
     J.T32 = reinterpret_cast<const float *>(base + P.o_T32);
     J.rechecked = reinterpret_cast<unsigned long long *>(base + P.o_cnt);
@@ -623,9 +630,9 @@ struct Workspace {
     DevBuf queue;      // items + terms + counter of the (r1,r2,r3) kernel
     DevBuf sqt_queue;  // items + terms + counter of the (s,q,theta) kernel
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    // The (r1,r2,r3) bit-mask kernel's modelled VALU lane-ops of the last
-    // launch on this workspace (device word), or nullptr.
-    const unsigned long long *ops = nullptr;
+    // Control block of the last (r1,r2,r3) LDS-kernel launch on this
+    // workspace (device words, kCtl*), or nullptr.
+    const int *ctl = nullptr;
     ~Workspace() {
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -641,17 +648,32 @@ void workspace_order(Workspace *ws, hipStream_t st) {
     ws->region.ordered = ws->queue.ordered = ws->sqt_queue.ordered = st;
 }
 
-// Launch shape of the LDS triplet kernel for bins up to max_n: one
-// histogram copy per workgroup, 8 waves per workgroup and as many
-// workgroups per CU as the occupancy allows.
-// TPCF_R123_SHAPE="waves,copies[,wg_per_cu]" overrides (tuning runs).
+// Launch shapes of the LDS triplet kernels for bins up to max_n.  The
+// run-merging kernel: as many privatized histogram copies (4, 2 or 1) as fit
+// the LDS budget, and enough waves per workgroup for four waves per SIMD at
+// the resulting occupancy.  The bit-mask kernel: one histogram copy, 8 waves
+// per workgroup, as many workgroups per CU as the occupancy allows.  form:
+// which of the two counts the launch (-1: chosen on the device from the
+// tiles' bins).  TPCF_R123_SHAPE="waves,copies[,wg_per_cu]" overrides the
+// run-merging shape and TPCF_R123_FORM / TPCF_R123_BITS_MAX the choice
+// (tuning runs).
 struct TripletShape {
-    int waves, copies, grid;
+    int waves, copies, grid;  // run-merging kernel
+    int bwaves, bgrid;        // bit-mask kernel
+    int form;
+    float bits_max;
 };
 
 static TripletShape triplet_shape(tpcf_ctx *ctx, int max_n) {
     constexpr int kLdsBudget = 155 * 1024;
-    TripletShape s{8, 1, 0};
+    TripletShape s{8, 1, 0, 8, 0, ctx->shape.form, ctx->shape.bits_max};
+    for (int k : {8, 4, 2, 1})
+        if (r123_lds_bytes(max_n, k) <= kLdsBudget) {
+            s.copies = k;
+            break;
+        }
+    const int lds = r123_lds_bytes(max_n, s.copies);
+    s.waves = lds > kLdsBudget / 2 ? 16 : 8;
     int per_cu_cap = 0;
     if (ctx->shape.waves > 0 && r123_triplet_kernel_ptr(ctx->shape.waves, ctx->shape.copies) &&
         r123_lds_bytes(max_n, ctx->shape.copies) <= kLdsBudget) {
@@ -666,6 +688,12 @@ static TripletShape triplet_shape(tpcf_ctx *ctx, int max_n) {
         per_cu = 1;
     if (per_cu_cap > 0) per_cu = std::min(per_cu, per_cu_cap);
     s.grid = ctx->num_cu * per_cu;
+    int bper_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bper_cu, r123_bits_kernel_ptr(s.bwaves), s.bwaves * 64,
+                                                     r123_bits_lds_bytes(max_n)) != hipSuccess ||
+        bper_cu < 1)
+        bper_cu = 1;
+    s.bgrid = ctx->num_cu * bper_cu;
     return s;
 }
 
@@ -688,6 +716,7 @@ static void build_queue(const std::vector<const JobPlan *> &plans, const std::ve
             T.tile_any = reinterpret_cast<const uint8_t *>(bases[j] + P.o_any);
             T.meta = P.bitmask ? reinterpret_cast<const TileMeta *>(bases[j] + P.o_meta) : nullptr;
             T.rmask = P.bitmask ? reinterpret_cast<const unsigned long long *>(bases[j] + P.o_rmask) : nullptr;
+            T.tstat = P.bitmask ? reinterpret_cast<const unsigned long long *>(bases[j] + P.o_tstat) : nullptr;
             T.nblk = P.L.nblk;
             T.hist = reinterpret_cast<unsigned long long *>(bases[j] + P.o_hist) + (size_t)t * P.cells;
             T.n = P.n;
@@ -705,11 +734,13 @@ struct QueueDev {
     const TripletTerm *terms = nullptr;
     int *counter = nullptr;
     int nitems = 0;
+    int nterms = 0;
     int max_n = 1;
 };
 
 static int upload_queue(DevBuf &buf, const Queue &Q, hipStream_t st, QueueDev &D) {
     D.nitems = (int)Q.items.size();
+    D.nterms = (int)Q.terms.size();
     D.max_n = Q.max_n;
     if (Q.items.empty()) return TPCF_OK;
     const size_t o_terms = align256(sizeof(TripletItem) * Q.items.size());
@@ -730,28 +761,38 @@ static int upload_queue(DevBuf &buf, const Queue &Q, hipStream_t st, QueueDev &D
 static int launch_queue(tpcf_ctx *ctx, const QueueDev &D, const TripletShape &sh, hipStream_t st,
                         ElasticHook *hook = nullptr) {
     if (!D.nitems) return TPCF_OK;
-    // item counter and the kernel's 64-bit op count after it
-    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(D.counter), 4, 0u, st));
+    // the control block: item counter and the bit-mask kernel's 64-bit op
+    // count (the choose kernel writes the form)
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(D.counter), kCtlForm, 0u, st));
     if (D.max_n > kMaxLdsR123Bins) {
         // Wide specs: pair values are unscaled; global-atomic kernel.
         HIP_TRY(launch_r123_triplets_wide(D.items, D.nitems, D.terms, ctx->num_cu * 8, st));
         return TPCF_OK;
     }
+    // Both LDS kernels are launched; the one not chosen by the choose kernel
+    // returns at once (either one gives the same histograms).
+    HIP_TRY(launch_r123_choose(D.terms, D.nterms, D.counter, sh.bits_max, sh.form, st));
     ElasticCtl ctl{};
-    int grid = sh.grid;
+    int grid = sh.grid, bgrid = sh.bgrid;
     if (hook) {
+        // One share for the launch, sized by the bit-mask kernel's occupancy;
+        // the run-merging kernel's workgroups beyond its own occupancy wait
+        // to be scheduled, and leave at once when over the share.
         const QueueDev d = D;
         const TripletShape s = sh;
         auto relaunch = [d, s](int wgs, hipStream_t other, const ElasticCtl &c) -> int {
             HIP_TRY(launch_r123_triplets(c, d.items, d.nitems, d.counter, d.terms, d.max_n, s.waves, s.copies, wgs,
                                          other));
+            HIP_TRY(launch_r123_bits(c, d.items, d.nitems, d.counter, d.terms, d.max_n, s.bwaves, wgs, other));
             return TPCF_OK;
         };
-        if (int e = hook->control(std::max(1, sh.grid / ctx->num_cu), D.nitems, relaunch, st, &ctl, &grid))
+        if (int e = hook->control(std::max(1, sh.bgrid / ctx->num_cu), D.nitems, relaunch, st, &ctl, &bgrid))
             return e;
+        if (ctl.active) grid = bgrid;
     }
     HIP_TRY(launch_r123_triplets(ctl, D.items, D.nitems, D.counter, D.terms, D.max_n, sh.waves, sh.copies, grid,
                                  st));
+    HIP_TRY(launch_r123_bits(ctl, D.items, D.nitems, D.counter, D.terms, D.max_n, sh.bwaves, bgrid, st));
     return TPCF_OK;
 }
 
@@ -763,8 +804,7 @@ static int launch_triplets(tpcf_ctx *ctx, Workspace &ws, const std::vector<const
     build_queue(plans, jobs, bases, Q);
     QueueDev D;
     if (int e = upload_queue(ws.queue, Q, st, D)) return e;
-    ws.ops = D.nitems && D.max_n <= kMaxLdsR123Bins ? reinterpret_cast<const unsigned long long *>(D.counter + 2)
-                                                     : nullptr;
+    ws.ctl = D.nitems && D.max_n <= kMaxLdsR123Bins ? D.counter : nullptr;
     HIP_TRY(hipEventRecord(ws.ev[3], st));
     return launch_queue(ctx, D, triplet_shape(ctx, Q.max_n), st, hook);
 }
@@ -1002,6 +1042,29 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
     return sqt_enqueue(ctx, L, ws.sqt_queue, st, nullptr, ws.ev[3], false, hook);
 }
 
+// The (r1,r2,r3) LDS launch's control block (after the launch has
+// finished): the form that counted, the mean bins per tile it was chosen on
+// and the bit-mask kernel's modelled VALU lane-ops.  No launch: form -1.
+static int read_ctl(const int *ctl, tpcf_stats *s) {
+    s->mask_ops = 0;
+    s->triplet_form = -1;
+    s->tile_bins_mean = 0.0;
+    s->tile_fill = 0.0;
+    if (!ctl) return TPCF_OK;
+    int w[8];
+    HIP_TRY(hipMemcpy(w, ctl, sizeof w, hipMemcpyDeviceToHost));
+    unsigned long long ops;
+    std::memcpy(&ops, w + kCtlOps, sizeof ops);
+    float mean, fill;
+    std::memcpy(&mean, w + kCtlBinsMean, sizeof mean);
+    std::memcpy(&fill, w + kCtlFill, sizeof fill);
+    s->mask_ops = w[kCtlForm] == kFormBits ? ops : 0;
+    s->triplet_form = w[kCtlForm];
+    s->tile_bins_mean = mean;
+    s->tile_fill = fill;
+    return TPCF_OK;
+}
+
 // Pair-stage and triplet-kernel times from the workspace events.
 static int stage_times(Workspace &ws, tpcf_stats *s) {
     float a = 0.f, b = 0.f;
@@ -1020,7 +1083,7 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
     if (int e = plan_job(job, spec, mode, P)) return e;
     if (mode == TPCF_MODE_CELLS) mode = TPCF_MODE_FAST;  // the plan holds the culling
     if (int e = ws.init()) return e;
-    ws.ops = nullptr;
+    ws.ctl = nullptr;
     if (int e = ws.region.reserve(P.bytes)) return e;
     char *base = reinterpret_cast<char *>(ws.region.p);
     {
@@ -1041,7 +1104,7 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
         std::vector<char *> bases{base};
         if (int e = launch_sqt_fast_jobs(ctx, ws, plans, jobs, bases, mode, st)) return e;
     } else if (P.L.nblk) {
-        HIP_TRY(launch_pair_tiles(pair_job(P, base), mode == TPCF_MODE_FAST, st));
+        HIP_TRY(launch_pair_tiles(pair_job(P, base, P.n <= kMaxLdsR123Bins ? 2 : 0), mode == TPCF_MODE_FAST, st));
         HIP_TRY(hipEventRecord(ws.ev[1], st));
         std::vector<const JobPlan *> plans{&P};
         std::vector<const tpcf_job *> jobs{&job};
@@ -1064,8 +1127,7 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
         stats->device_ms = ms;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (int e = stage_times(ws, stats)) return e;
-        stats->mask_ops = 0;
-        if (ws.ops) HIP_TRY(hipMemcpy(&stats->mask_ops, ws.ops, sizeof(stats->mask_ops), hipMemcpyDeviceToHost));
+        if (int e = read_ctl(ws.ctl, stats)) return e;
     }
     return TPCF_OK;
 }
@@ -1134,7 +1196,7 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     } share{hook, st};
     if (mode == TPCF_MODE_CELLS) mode = TPCF_MODE_FAST;  // the plans hold the culling
     if (int e = ws.init()) return e;
-    ws.ops = nullptr;
+    ws.ctl = nullptr;
     // An elastic batch reserves its workspace, work-queue buffers included,
     // before it claims its share: nothing after the claim allocates.
     if (int e = ws.region.reserve(std::max<size_t>(bytes, 256))) return e;
@@ -1173,12 +1235,16 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     if (!sp.empty())
         if (int e = launch_sqt_fast_jobs(ctx, ws, sp, sj, sb, mode, st, hook)) return e;
     // (r1,r2,r3) jobs: pair stages, then one persistent launch.
+    int max_n = 1;
+    for (int j = 0; j < njobs; ++j)
+        if (plans[j].param == TPCF_R123) max_n = std::max(max_n, plans[j].n);
+    const int shift = max_n <= kMaxLdsR123Bins ? 2 : 0;
     std::vector<const JobPlan *> pp;
     std::vector<const tpcf_job *> jj;
     std::vector<char *> bb;
     for (int j = 0; j < njobs; ++j) {
         if (plans[j].param != TPCF_R123 || !plans[j].L.nblk) continue;
-        HIP_TRY(launch_pair_tiles(pair_job(plans[j], base + off[j]), mode == TPCF_MODE_FAST, st));
+        HIP_TRY(launch_pair_tiles(pair_job(plans[j], base + off[j], shift), mode == TPCF_MODE_FAST, st));
         pp.push_back(&plans[j]);
         jj.push_back(&jobs[j]);
         bb.push_back(base + off[j]);
@@ -1219,11 +1285,14 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     tot.device_ms = ms;
     tot.wall_ms = wall;
     if (int e = stage_times(ws, &tot)) return e;
-    if (ws.ops) HIP_TRY(hipMemcpy(&tot.mask_ops, ws.ops, sizeof(tot.mask_ops), hipMemcpyDeviceToHost));
+    if (int e = read_ctl(ws.ctl, &tot)) return e;
     for (int j = 0; j < njobs && per; ++j) {
         per[j].pair_ms = tot.pair_ms;
         per[j].triplet_ms = tot.triplet_ms;
         per[j].mask_ops = tot.mask_ops;
+        per[j].triplet_form = tot.triplet_form;
+        per[j].tile_bins_mean = tot.tile_bins_mean;
+        per[j].tile_fill = tot.tile_fill;
     }
     if (total) *total = tot;
     return TPCF_OK;
@@ -1355,6 +1424,8 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
         for (int w : {2, 4, 8, 16})
             for (int k : {1, 2, 4, 8})
                 if (const void *fn = r123_triplet_kernel_ptr(w, k, el)) lds_kernels.push_back(fn);
+        for (int w : {4, 8, 16})
+            if (const void *fn = r123_bits_kernel_ptr(w, el)) lds_kernels.push_back(fn);
     }
     for (const void *fn : lds_kernels) {
         hipFuncAttributes fa{};
@@ -1368,8 +1439,17 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     }
     if (const char *env = std::getenv("TPCF_R123_SHAPE")) {
         int w = 0, k = 0, p = 0;
-        if (std::sscanf(env, "%d,%d,%d", &w, &k, &p) >= 2) c->shape = LaunchShape{w, k, p};
+        if (std::sscanf(env, "%d,%d,%d", &w, &k, &p) >= 2) {
+            c->shape.waves = w;
+            c->shape.copies = k;
+            c->shape.per_cu = p;
+        }
     }
+    if (const char *env = std::getenv("TPCF_R123_FORM")) {
+        if (!std::strcmp(env, "runs")) c->shape.form = kFormRuns;
+        if (!std::strcmp(env, "bits")) c->shape.form = kFormBits;
+    }
+    if (const char *env = std::getenv("TPCF_R123_BITS_MAX")) c->shape.bits_max = (float)std::atof(env);
     *out = c;
     return TPCF_OK;
 }
@@ -1551,7 +1631,8 @@ struct tpcf_batch {
     hipStream_t st = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     QueueDev qd;
-    TripletShape shape{8, 1, 0};
+    int shift = 2;
+    TripletShape shape{8, 1, 0, 8, 0, -1, 6.3f};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string tuning;  // autotuner log
@@ -1578,7 +1659,7 @@ static int batch_compute(tpcf_batch *b, const TripletShape &sh, hipStream_t st) 
     if (int e = sqt_enqueue(b->ctx, b->sqt, b->sqt_queue, st, nullptr, nullptr)) return e;
     for (int j = 0; j < b->njobs; ++j)
         if (b->plans[j].param == TPCF_R123 && b->plans[j].L.nblk)
-            HIP_TRY(launch_pair_tiles(pair_job(b->plans[j], base + b->off[j]), b->mode == TPCF_MODE_FAST,
+            HIP_TRY(launch_pair_tiles(pair_job(b->plans[j], base + b->off[j], b->shift), b->mode == TPCF_MODE_FAST,
                                       st));
     if (!b->qd.nitems) return TPCF_OK;
     return launch_queue(b->ctx, b->qd, sh, st);
@@ -1600,23 +1681,56 @@ static int batch_autotune(tpcf_batch *b) {
     constexpr int kLdsBudget = 155 * 1024;
     const int n = b->qd.max_n;
     if (n > kMaxLdsR123Bins || !b->qd.nitems) return TPCF_OK;
+    // The run-merging kernel's shapes, then the bit-mask kernel's, each
+    // timed with its form forced.
+    const TripletShape base = b->shape;
     std::vector<TripletShape> cands;
-    for (int k : {1})
-        for (int w : {4, 8, 16}) {
-            if (!r123_triplet_kernel_ptr(w, k) || r123_lds_bytes(n, k) > kLdsBudget) continue;
+    int kmax = 1;
+    for (int k : {8, 4, 2, 1})
+        if (r123_lds_bytes(n, k) <= kLdsBudget) {
+            kmax = k;
+            break;
+        }
+    for (int k : {kmax, kmax / 2})
+        for (int w : {2, 4, 8, 16}) {
+            if (k < 1 || !r123_triplet_kernel_ptr(w, k)) continue;
             int occ = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, r123_triplet_kernel_ptr(w, k), w * 64,
                                                              r123_lds_bytes(n, k)) != hipSuccess || occ < 1)
                 continue;
-            cands.push_back({w, k, b->ctx->num_cu * occ});
-            if (occ > 1) cands.push_back({w, k, b->ctx->num_cu * std::max(1, occ / 2)});
+            TripletShape c = base;
+            c.waves = w;
+            c.copies = k;
+            c.form = kFormRuns;
+            c.grid = b->ctx->num_cu * occ;
+            cands.push_back(c);
+            if (occ > 1) {
+                c.grid = b->ctx->num_cu * std::max(1, occ / 2);
+                cands.push_back(c);
+            }
         }
+    for (int w : {4, 8, 16}) {
+        int occ = 0;
+        if (!r123_bits_kernel_ptr(w) ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, r123_bits_kernel_ptr(w), w * 64,
+                                                         r123_bits_lds_bytes(n)) != hipSuccess || occ < 1)
+            continue;
+        TripletShape c = base;
+        c.bwaves = w;
+        c.bgrid = b->ctx->num_cu * occ;
+        c.form = kFormBits;
+        cands.push_back(c);
+    }
     float best = 1e30f;
-    char line[128];
+    char line[160];
     for (const auto &c : cands) {
         float ms = 0.f;
         if (int e = batch_time(b, c, ms)) return e;
-        std::snprintf(line, sizeof line, "waves=%d copies=%d wg=%d: %.3f ms\n", c.waves, c.copies, c.grid, ms);
+        if (c.form == kFormBits)
+            std::snprintf(line, sizeof line, "bit-mask waves=%d wg=%d: %.3f ms\n", c.bwaves, c.bgrid, ms);
+        else
+            std::snprintf(line, sizeof line, "run-merging waves=%d copies=%d wg=%d: %.3f ms\n", c.waves, c.copies,
+                          c.grid, ms);
         b->tuning += line;
         if (ms < best) {
             best = ms;
@@ -1645,11 +1759,14 @@ int tpcf_batch_create(tpcf_ctx *ctx, const tpcf_job *jobs, const tpcf_binspec *s
     b->plans.resize(njobs);
     b->off.resize(njobs);
     size_t bytes = 0;
+    int max_n = 1;
     for (int j = 0; j < njobs; ++j) {
         if (int e = plan_job(jobs[j], specs[j], mode, b->plans[j])) return e;
         b->off[j] = bytes;
         bytes = align256(bytes + b->plans[j].bytes);
+        if (specs[j].param == TPCF_R123) max_n = std::max(max_n, b->plans[j].n);
     }
+    b->shift = max_n <= kMaxLdsR123Bins ? 2 : 0;
     // CELLS: the plans hold the culling (Hilbert staging, reach, culled
     // items, all done once below); the replayed kernels are FAST's.
     if (mode == TPCF_MODE_CELLS) b->mode = TPCF_MODE_FAST;
@@ -1721,9 +1838,6 @@ int tpcf_batch_run(tpcf_batch *b, uint64_t *const *hist, tpcf_stats *stats) {
                                hipMemcpyDeviceToHost, b->st));
         HIP_TRY(hipMemcpyAsync(&rech[j], base + b->off[j] + P.o_cnt, 8, hipMemcpyDeviceToHost, b->st));
     }
-    uint64_t ops = 0;
-    if (b->qd.nitems && b->qd.max_n <= kMaxLdsR123Bins)
-        HIP_TRY(hipMemcpyAsync(&ops, b->qd.counter + 2, sizeof ops, hipMemcpyDeviceToHost, b->st));
     HIP_TRY(hipStreamSynchronize(b->st));
     if (stats) {
         float ms = 0.f;
@@ -1738,7 +1852,8 @@ int tpcf_batch_run(tpcf_batch *b, uint64_t *const *hist, tpcf_stats *stats) {
         }
         s.device_ms = ms;
         s.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        s.mask_ops = ops;
+        if (int e = read_ctl(b->qd.nitems && b->qd.max_n <= kMaxLdsR123Bins ? b->qd.counter : nullptr, &s))
+            return e;
         *stats = s;
     }
     return TPCF_OK;
@@ -1748,9 +1863,11 @@ const char *tpcf_batch_info(const tpcf_batch *b) {
     if (!b) return "";
     static thread_local std::string info;
     char head[160];
-    std::snprintf(head, sizeof head, "shape waves=%d copies=%d wg=%d graph=%d items=%d sqt_items=%d\n",
-                  b->shape.waves, b->shape.copies, b->shape.grid, b->exec ? 1 : 0, b->qd.nitems,
-                  (int)b->sqt.items.size());
+    std::snprintf(head, sizeof head,
+                  "shape form=%s waves=%d copies=%d wg=%d bits_waves=%d bits_wg=%d graph=%d items=%d sqt_items=%d\n",
+                  b->shape.form == kFormBits ? "bit-mask" : (b->shape.form == kFormRuns ? "run-merging" : "auto"),
+                  b->shape.waves, b->shape.copies, b->shape.grid, b->shape.bwaves, b->shape.bgrid, b->exec ? 1 : 0,
+                  b->qd.nitems, (int)b->sqt.items.size());
     info = head + b->tuning;
     return info.c_str();
 }

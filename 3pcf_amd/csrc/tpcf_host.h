@@ -85,6 +85,11 @@ struct LaunchShape {
     int waves = 0;     // waves per workgroup (0 = automatic)
     int copies = 1;    // privatized LDS histogram copies
     int per_cu = 0;    // workgroups per CU (0 = occupancy limit)
+    // (r1,r2,r3) triplet kernel: -1 chosen per launch from the tiles' bins,
+    // 0 run-merging, 1 bit-mask; and the scale of the choice (the bit-mask
+    // kernel when the tiles' mean bins <= bits_max * sqrt(their fill)).
+    int form = -1;
+    float bits_max = 6.3f;
 };
 
 }  // namespace tpcf

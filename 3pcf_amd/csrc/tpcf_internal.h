@@ -39,6 +39,7 @@ struct TripletTerm {
     const uint8_t *tile_any;  // per tile: 1 if some pair of it has a bin (nullptr: not kept)
     const TileMeta *meta;  // per tile (bit-mask kernel; nullptr for wide specs)
     const unsigned long long *rmask;  // kRowMaskWords per tile (bit-mask kernel)
+    const unsigned long long *tstat;  // the job's tile counts: 64 slots of (bins, tiles with a bin, their pairs with a bin)
     int64_t nblk;          // blocks in the job's union layout
     unsigned long long *hist;  // nterm histogram, n^3 uint64 (sorted-side cells)
     int n;                 // bins per side
@@ -72,6 +73,14 @@ __device__ inline bool elastic_leave(const ElasticCtl &c) {
     }
 }
 
+// Words of a triplet launch's control block (int *counter of the launchers,
+// 256-byte aligned, cleared before each launch up to kCtlForm): the item
+// counter, the bit-mask kernel's modelled VALU lane-ops (64 bits), the
+// chosen form, and the mean bins per tile and the fill of the tiles the
+// choice was made on.
+constexpr int kCtlItems = 0, kCtlOps = 2, kCtlForm = 4, kCtlBinsMean = 5, kCtlFill = 6;
+constexpr int kFormRuns = 0, kFormBits = 1;
+
 // A unit of triplet work: blocks A <= B <= C, C in [c0, c1).
 struct TripletItem {
     int term;  // index into the TripletTerm table
@@ -86,10 +95,13 @@ struct PairJob {
     const uint8_t *blk_count;  // real points in each block (0..64)
     uint8_t *tiles;
     uint8_t *tile_any;     // per tile: 1 if some pair of it has a bin, else 0 (nullptr: none kept)
-    // Bit-mask triplet kernel (n <= kMaxLdsR123Bins, shift 0): per tile its
+    // Bit-mask triplet kernel (n <= kMaxLdsR123Bins, shift 2): per tile its
     // TileMeta and kRowMaskWords row masks (nullptr: not written).
     TileMeta *meta;
     unsigned long long *rmask;
+    // 64 slots (by tile index) of 3 words: bins summed over the tiles with a
+    // bin, their count, and their pairs with a bin.
+    unsigned long long *tstat;
     const double *T64;     // squared thresholds [0..n]
     const float *T32;      // T64 rounded to float
     unsigned long long *rechecked;  // FP64 recomputations
@@ -98,7 +110,7 @@ struct PairJob {
     int n;
     float rmax;            // max |coordinate - origin| over the union (FP32, rounded up)
     float rel;             // extra relative window around each threshold (squared units)
-    int shift;             // stored pair value = bin << shift (0: the bin itself)
+    int shift;             // stored pair value = bin << shift (2 for the LDS kernels, 0 otherwise)
     // Culled form: bit C of word [R * need_words + C / 64] set for the tiles
     // (R, C) the triplet stage reads; the others are skipped.  nullptr: all.
     const unsigned long long *need;
@@ -267,5 +279,13 @@ hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_to_f32(const double *u64, int64_t npad, const double o[3], float4 *u32, hipStream_t st);
 int r123_lds_bytes(int n, int copies);
 const void *r123_triplet_kernel_ptr(int waves, int copies, bool elastic = false);
+// The bit-mask triplet kernel, launched beside the run-merging one; a choose
+// kernel run first picks which of the two counts the launch.
+hipError_t launch_r123_choose(const TripletTerm *terms, int nterms, int *counter, float max_bins, int force,
+                              hipStream_t st);
+hipError_t launch_r123_bits(const ElasticCtl &ctl, const TripletItem *items, int nitems, int *counter,
+                            const TripletTerm *terms, int max_n, int waves, int grid, hipStream_t st);
+int r123_bits_lds_bytes(int n);
+const void *r123_bits_kernel_ptr(int waves, bool elastic = false);
 
 }  // namespace tpcf

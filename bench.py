@@ -49,6 +49,24 @@ METRIC = "triplets/s at 50k+50k DDD..RRR; batched voxel jobs/s + p99 vs plain, e
 VALU_PEAK = 256 * 64 * 2.4e9
 
 
+# stats.triplet_form: which (r1,r2,r3) LDS triplet kernel counted
+FORMS = {0: "run-merging", 1: "bit-mask"}
+
+
+def valu_roofline(ops: float, seconds: float) -> dict:
+    """Roofline entry of the (r1,r2,r3) bit-mask triplet kernel: its counted
+    VALU lane-ops over its HIP-event time, against the VALU peak."""
+    achieved = ops / seconds if seconds > 0 else 0.0
+    return {
+        "bound": "valu",
+        "achieved": achieved / 1e12,
+        "peak": VALU_PEAK / 1e12,
+        "unit": "TOP/s (VALU lane-ops of the bit-mask triplet kernel's cost model, counted by the kernel)",
+        "frac": achieved / VALU_PEAK,
+        "traffic": None,
+    }
+
+
 def headline_catalogs(nd: int, nr: int):
     d = synth.neyman_scott(nd, nclust=200, sigma=2.0, size=200.0, seed=0)
     r = synth.uniform_cube(nr, size=200.0, seed=1)
@@ -261,6 +279,9 @@ def bench_batch(args) -> int:
             "speedup_vs_plain": fast_rate / plain_rate,
             "histograms_identical_to_plain": exact,
             "count_batch_fast_ms": uncull_s * 1e3,
+            "count_batch_fast_triplet_form": FORMS.get(ust.triplet_form, "none"),
+            "count_batch_fast_tile_bins_mean": ust.tile_bins_mean,
+            "count_batch_fast_tile_fill": ust.tile_fill,
             "count_batch_cells_ms": cells_s * 1e3,
             "count_batch_cells_triplet_kernel_ms": cst.triplet_ms,
             "cells_voxel_jobs_per_s": nv / cells_s,
@@ -657,6 +678,15 @@ def bench_survey(args) -> int:
         line["config"][f"{name}_pair_stage_ms"] = st.pair_ms
         line["config"][f"{name}_entries"] = entries
         line["config"][f"{name}_entries_per_s"] = entries / dt
+    st = res["r123"][1]
+    line["config"]["r123_kernel_valu_lane_ops"] = st.mask_ops
+    line["config"]["r123_triplet_form"] = FORMS.get(st.triplet_form, "none")
+    line["config"]["r123_tile_bins_mean"] = st.tile_bins_mean
+    line["config"]["r123_tile_fill"] = st.tile_fill
+    # the (r1,r2,r3) job's triplet kernel: the work it visits, counted by it
+    # (the bit-mask kernel only)
+    if st.triplet_form == 1:
+        line["roofline"] = valu_roofline(st.mask_ops, st.triplet_ms / 1e3)
     print(json.dumps(line))
     return 0
 
@@ -717,10 +747,18 @@ def bench_precision(args) -> int:
             "cells_ms_per_step": dtc * 1e3,
             "cells_triplet_kernel_ms": stc.triplet_ms,
             "cells_equals_fast": bool(np.array_equal(out, outc)),
+            "kernel_valu_lane_ops": stats[0].mask_ops,
+            "triplet_form": FORMS.get(stats[0].triplet_form, "none"),
+            "tile_bins_mean": stats[0].tile_bins_mean,
+            "tile_fill": stats[0].tile_fill,
         },
-        # No roofline: most block triples of the wedge have a side out of
-        # range and are skipped by their tiles' flags.
     }
+    # The bit-mask kernel counts the work it visits (rows and bin pairs of
+    # the block triples it does not skip), so its roofline holds here too;
+    # the run-merging kernel settles skipped tiles without counting, so a
+    # launch of it carries no roofline.
+    if stats[0].triplet_form == 1:
+        line["roofline"] = valu_roofline(sum(x.mask_ops for x in stats) / len(stats), tri_s)
     print(json.dumps(line))
     return 0
 
@@ -886,7 +924,6 @@ def bench_headline(args) -> int:
     value = triplets * steps / dt
     tri_s = sum(s.triplet_ms for s in stats) / steps / 1e3
     ops = sum(s.mask_ops for s in stats) / steps
-    achieved = ops / tri_s if tri_s > 0 else 0.0
     line = {
         "metric": METRIC,
         "value": value,
@@ -919,16 +956,12 @@ def bench_headline(args) -> int:
             "triplet_kernel_ms": tri_s * 1e3,
             "kernel_valu_lane_ops": ops,
             "valu_lane_ops_per_triplet": ops / triplets if triplets else 0.0,
+            "triplet_form": FORMS.get(stats[0].triplet_form, "none"),
+            "tile_bins_mean": stats[0].tile_bins_mean,
+            "tile_fill": stats[0].tile_fill,
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "valu",
-            "achieved": achieved / 1e12,
-            "peak": VALU_PEAK / 1e12,
-            "unit": "TOP/s (VALU lane-ops of the bit-mask triplet kernel's cost model, counted by the kernel)",
-            "frac": achieved / VALU_PEAK,
-            "traffic": measured_traffic(args),
-        },
+        "roofline": dict(valu_roofline(ops, tri_s), traffic=measured_traffic(args)),
     }
     if not args.no_batch:
         line["batch"] = voxel_batch_figures(reps=10)
