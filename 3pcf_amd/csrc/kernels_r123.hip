@@ -289,9 +289,10 @@ __device__ inline unsigned long long build_row_mask(global_u8_ptr g, uint32_t bi
 // triplets (i, j, k) with those two bins, and it goes to the lane's
 // accumulator of x = bin(i,j) (a sentinel matches no bin).  The x bins share
 // accumulator words as 16-bit fields: bins xs[0] and xs[1] are the low and
-// high fields of acc0, xs[2] the high field of acc1, added with one
-// multiply-add by the lane's weight 1 or 65536 (0 for a lane whose x is not
-// in the pass).  A lane adds at most 64 rows x 64 points = 4096 per field.
+// high fields of acc0, xs[2] the high and xs[3] the low field of acc1, added
+// with one multiply-add by the lane's weight 1 or 65536 (0 for a lane whose
+// x is not in the pass).  A lane adds at most 64 rows x 64 points = 4096 per
+// field.
 // At the end each field is summed over the wave and added to its LDS cell by
 // one lane.  A y bin with no point in a row is skipped for that row.
 // Returns the pass's VALU lane-ops by the kernel's cost model: per row, NY
@@ -300,9 +301,9 @@ __device__ inline unsigned long long build_row_mask(global_u8_ptr g, uint32_t bi
 // accumulator word; per accumulator field 9 for its wave sum.
 template <int NX, int NY, int NZ>
 __device__ __forceinline__ unsigned long long bits_pass(uint32_t *hist, global_u8_ptr gAB, global_u8_ptr gAC,
-                                          unsigned long long rows, const uint32_t (&xs)[3],
-                                          const uint32_t (&ys)[3], const uint32_t (&zs)[3],
-                                          const unsigned long long (&zm)[3], int lane, uint32_t n) {
+                                          unsigned long long rows, const uint32_t (&xs)[4],
+                                          const uint32_t (&ys)[4], const uint32_t (&zs)[4],
+                                          const unsigned long long (&zm)[4], int lane, uint32_t n) {
     uint32_t acc0[NY][NZ], acc1[NY][NZ];
 #pragma unroll
     for (int a = 0; a < NY; ++a)
@@ -322,7 +323,7 @@ __device__ __forceinline__ unsigned long long bits_pass(uint32_t *hist, global_u
 #pragma unroll
         for (int a = 0; a < NY; ++a) ym[a] = __ballot(yb == ys[a] << 2);
         const uint32_t w0 = xb == xs[0] << 2 ? 1u : (NX > 1 && xb == xs[1] << 2 ? 65536u : 0u);
-        const uint32_t w1 = NX > 2 && xb == xs[2] << 2 ? 65536u : 0u;
+        const uint32_t w1 = NX > 2 && xb == xs[2] << 2 ? 65536u : (NX > 3 && xb == xs[3] << 2 ? 1u : 0u);
 #pragma unroll
         for (int a = 0; a < NY; ++a) {
             if (!ym[a]) continue;
@@ -346,7 +347,8 @@ __device__ __forceinline__ unsigned long long bits_pass(uint32_t *hist, global_u
         for (int b = 0; b < NZ; ++b)
 #pragma unroll
             for (int s = 0; s < NX; ++s) {
-                const uint32_t f = s == 0 ? acc0[a][b] & 0xFFFFu : (s == 1 ? acc0[a][b] >> 16 : acc1[a][b] >> 16);
+                const uint32_t f = s == 0 ? acc0[a][b] & 0xFFFFu
+                                   : (s == 1 ? acc0[a][b] >> 16 : (s == 2 ? acc1[a][b] >> 16 : acc1[a][b] & 0xFFFFu));
                 const uint32_t tot = wave_sum(f);
                 if (tot && lane == 0) lds_add(hist, ((zs[b] * n + ys[a]) * n + xs[s]) * 4u, tot);
             }
@@ -354,11 +356,12 @@ __device__ __forceinline__ unsigned long long bits_pass(uint32_t *hist, global_u
                     9ull * NX * NY * NZ);
 }
 
-// Up to three bins from the set bits of *rest (lowest first), taken off it.
-__device__ __forceinline__ int take3(uint32_t *rest, uint32_t (&v)[3]) {
+// Up to N bins from the set bits of *rest (lowest first), taken off it.
+template <int N>
+__device__ __forceinline__ int take_bins(uint32_t *rest, uint32_t (&v)[N]) {
     int k = 0;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < N; ++q) {
         v[q] = 0u;
         if (*rest) {
             v[q] = (uint32_t)__builtin_ctz(*rest);
@@ -373,8 +376,10 @@ __device__ __forceinline__ int take3(uint32_t *rest, uint32_t (&v)[3]) {
 // counter until the queue is drained; the waves of a workgroup split the C
 // blocks of an item (wave w takes C = c0+w, c0+w+WAVES, ...) and share the
 // LDS histogram.  A block triple is counted bin by bin: the bins present in
-// its three tiles (TileMeta, from the pair stage) are taken up to three per
-// side at a time, and each pass (bits_pass) counts every triplet whose three
+// its three tiles (TileMeta, from the pair stage) are taken up to four per
+// side at a time (four x bins cost a pass no more than three: both need two
+// accumulator words), and each pass (bits_pass)
+// counts every triplet whose three
 // bins are in the pass's sets, so the passes together count each triplet of
 // the block triple once.  Only rows of block A with some pair in range in
 // both tiles (A,B) and (A,C) are visited; block pairs and C blocks whose
@@ -448,11 +453,11 @@ __global__ __launch_bounds__(WAVES * 64) void r123_bits_kernel(const ElasticCtl 
             const global_u8_ptr gBC = (global_u8_ptr)(T.tiles + tBC * kTileBytes);
             uint32_t zrest = mBC.bins;
             while (zrest) {
-                uint32_t zs[3];
-                unsigned long long zm[3];
-                const int nz = take3(&zrest, zs);
+                uint32_t zs[4];
+                unsigned long long zm[4];
+                const int nz = take_bins<4>(&zrest, zs);
 #pragma unroll
-                for (int b = 0; b < 3; ++b) {
+                for (int b = 0; b < 4; ++b) {
                     zm[b] = 0ull;
                     if (b < nz) {
                         const int slot = __builtin_popcount(mBC.bins & ((1u << zs[b]) - 1u));
@@ -462,26 +467,33 @@ __global__ __launch_bounds__(WAVES * 64) void r123_bits_kernel(const ElasticCtl 
                 }
                 uint32_t yrest = mAC.bins;
                 while (yrest) {
-                    uint32_t ys[3];
-                    const int ny = take3(&yrest, ys);
+                    uint32_t ys[4];
+                    const int ny = take_bins<4>(&yrest, ys);
                     uint32_t xrest = mAB.bins;
                     while (xrest) {
-                        uint32_t xs[3];
-                        const int nx = take3(&xrest, xs);
-                        switch ((nx - 1) * 9 + (ny - 1) * 3 + (nz - 1)) {
+                        uint32_t xs[4];
+                        const int nx = take_bins<4>(&xrest, xs);
+                        switch ((nx - 1) * 16 + (ny - 1) * 4 + (nz - 1)) {
 #define TPCF_PASS(X, Y, Z)                                                                     \
-    case (X - 1) * 9 + (Y - 1) * 3 + (Z - 1):                                                  \
+    case (X - 1) * 16 + (Y - 1) * 4 + (Z - 1):                                                 \
         wave_ops += bits_pass<X, Y, Z>(hist, gAB, gAC, rows, xs, ys, zs, zm, lane, (uint32_t)n); \
         break;
-                            TPCF_PASS(1, 1, 1) TPCF_PASS(1, 1, 2) TPCF_PASS(1, 1, 3)
-                            TPCF_PASS(1, 2, 1) TPCF_PASS(1, 2, 2) TPCF_PASS(1, 2, 3)
-                            TPCF_PASS(1, 3, 1) TPCF_PASS(1, 3, 2) TPCF_PASS(1, 3, 3)
-                            TPCF_PASS(2, 1, 1) TPCF_PASS(2, 1, 2) TPCF_PASS(2, 1, 3)
-                            TPCF_PASS(2, 2, 1) TPCF_PASS(2, 2, 2) TPCF_PASS(2, 2, 3)
-                            TPCF_PASS(2, 3, 1) TPCF_PASS(2, 3, 2) TPCF_PASS(2, 3, 3)
-                            TPCF_PASS(3, 1, 1) TPCF_PASS(3, 1, 2) TPCF_PASS(3, 1, 3)
-                            TPCF_PASS(3, 2, 1) TPCF_PASS(3, 2, 2) TPCF_PASS(3, 2, 3)
-                            TPCF_PASS(3, 3, 1) TPCF_PASS(3, 3, 2) TPCF_PASS(3, 3, 3)
+                            TPCF_PASS(1, 1, 1) TPCF_PASS(1, 1, 2) TPCF_PASS(1, 1, 3) TPCF_PASS(1, 1, 4)
+                            TPCF_PASS(1, 2, 1) TPCF_PASS(1, 2, 2) TPCF_PASS(1, 2, 3) TPCF_PASS(1, 2, 4)
+                            TPCF_PASS(1, 3, 1) TPCF_PASS(1, 3, 2) TPCF_PASS(1, 3, 3) TPCF_PASS(1, 3, 4)
+                            TPCF_PASS(1, 4, 1) TPCF_PASS(1, 4, 2) TPCF_PASS(1, 4, 3) TPCF_PASS(1, 4, 4)
+                            TPCF_PASS(2, 1, 1) TPCF_PASS(2, 1, 2) TPCF_PASS(2, 1, 3) TPCF_PASS(2, 1, 4)
+                            TPCF_PASS(2, 2, 1) TPCF_PASS(2, 2, 2) TPCF_PASS(2, 2, 3) TPCF_PASS(2, 2, 4)
+                            TPCF_PASS(2, 3, 1) TPCF_PASS(2, 3, 2) TPCF_PASS(2, 3, 3) TPCF_PASS(2, 3, 4)
+                            TPCF_PASS(2, 4, 1) TPCF_PASS(2, 4, 2) TPCF_PASS(2, 4, 3) TPCF_PASS(2, 4, 4)
+                            TPCF_PASS(3, 1, 1) TPCF_PASS(3, 1, 2) TPCF_PASS(3, 1, 3) TPCF_PASS(3, 1, 4)
+                            TPCF_PASS(3, 2, 1) TPCF_PASS(3, 2, 2) TPCF_PASS(3, 2, 3) TPCF_PASS(3, 2, 4)
+                            TPCF_PASS(3, 3, 1) TPCF_PASS(3, 3, 2) TPCF_PASS(3, 3, 3) TPCF_PASS(3, 3, 4)
+                            TPCF_PASS(3, 4, 1) TPCF_PASS(3, 4, 2) TPCF_PASS(3, 4, 3) TPCF_PASS(3, 4, 4)
+                            TPCF_PASS(4, 1, 1) TPCF_PASS(4, 1, 2) TPCF_PASS(4, 1, 3) TPCF_PASS(4, 1, 4)
+                            TPCF_PASS(4, 2, 1) TPCF_PASS(4, 2, 2) TPCF_PASS(4, 2, 3) TPCF_PASS(4, 2, 4)
+                            TPCF_PASS(4, 3, 1) TPCF_PASS(4, 3, 2) TPCF_PASS(4, 3, 3) TPCF_PASS(4, 3, 4)
+                            TPCF_PASS(4, 4, 1) TPCF_PASS(4, 4, 2) TPCF_PASS(4, 4, 3) TPCF_PASS(4, 4, 4)
 #undef TPCF_PASS
                         }
                     }
