@@ -385,7 +385,7 @@ __device__ __forceinline__ int take_bins(uint32_t *rest, uint32_t (&v)[N]) {
 // both tiles (A,B) and (A,C) are visited; block pairs and C blocks whose
 // tiles have no pair in range are skipped.
 template <int WAVES, bool ELASTIC>
-__global__ __launch_bounds__(WAVES * 64) void r123_bits_kernel(const ElasticCtl ctl,
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(8, 8))) void r123_bits_kernel(const ElasticCtl ctl,
                                                                   const TripletItem *__restrict__ items,
                                                                   int nitems, int *__restrict__ counter,
                                                                   const TripletTerm *__restrict__ terms,
