@@ -26,6 +26,8 @@
 //
 // Compile with -ffp-contract=off: the FP64 arithmetic must be the unfused
 // IEEE sequence ((dx*dx + dy*dy) + dz*dz) that the CPU definition uses.
+#include <cstdlib>
+
 #include "tpcf_internal.h"
 
 namespace tpcf {
@@ -508,6 +510,331 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(8, 8
 }
 
 // ---------------------------------------------------------------------------
+// triplet stage, matrix-core form (bit-mask passes as int8 products)
+// ---------------------------------------------------------------------------
+
+typedef int mfma_v4i __attribute__((ext_vector_type(4)));
+
+// Bins per side of one matrix-core pass, at most (the registers of the Y
+// operands grow with the y bins, the accumulators with all three).
+constexpr int kMfmaCX = 4, kMfmaCY = 2, kMfmaCZ = 2;
+// LDS after the histogram: the item's x one-hot table, per 16 x 16 block
+// (i-block, j-block) and lane one uint4 (four x bins of one pass, each a
+// dword of four one-hot bytes).
+constexpr int kMfmaXohBytes = 16 * kBlock * 16;
+__host__ __device__ inline int mfma_hist_words(int n) { return (n * n * n + 3) & ~3; }
+
+// Up to M bins from the set bits of *rest (lowest first), taken off it; the
+// other entries of v are 0.
+template <int M>
+__device__ __forceinline__ int take_upto(uint32_t *rest, uint32_t (&v)[4]) {
+    int k = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        v[q] = 0u;
+        if (q < M && *rest) {
+            v[q] = (uint32_t)__builtin_ctz(*rest);
+            *rest &= *rest - 1u;
+            k = q + 1;
+        }
+    }
+    return k;
+}
+
+// The four bytes of w that equal v (v < 0x80) as bytes 1, the others 0.
+__device__ inline uint32_t onehot4(uint32_t w, uint32_t v) {
+    const uint32_t t = w ^ (v * 0x01010101u);
+    const uint32_t u = ((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t;  // bit 7 of a byte set iff the byte is nonzero
+    return (~u >> 7) & 0x01010101u;
+}
+
+// The int8 operand of bin `bin` of a tile for v_mfma_i32_16x16x64_i8: lane l
+// holds row 16 * blk + (l & 15) of the tile and its 16 columns
+// k = 16 * (l >> 4) + 4 q + t (byte t of dword q) as one-hot bytes; the A and
+// B operands of the instruction both take this k order, so a product sums
+// over every column once.  From the tile's stored row mask when the bin has
+// a slot, else from the tile bytes.
+// The dword of the stored row mask an operand needs: bits 16 * (l >> 4) ..
+// + 15 of row 16 * blk + (l & 15) are in dword (l >> 5) of that row's mask.
+__device__ __forceinline__ uint32_t mfma_mask_dword(const unsigned long long *rmask, int slot, int blk, int lane) {
+    typedef const uint32_t __attribute__((address_space(1))) *global_u32_ptr;
+    if (slot >= kRowMaskSlots) return 0u;
+    return ((global_u32_ptr)(rmask + slot * kBlock + 16 * blk + (lane & 15)))[lane >> 5];
+}
+
+// The operand from its mask dword (mfma_mask_dword) when the bin has a
+// slot, else from the tile bytes.
+__device__ __forceinline__ mfma_v4i mfma_operand_from(uint32_t mw, global_u8_ptr tile, int slot, uint32_t bin,
+                                                      int blk, int lane) {
+    uint32_t d[4];
+    if (slot < kRowMaskSlots) {
+        const uint32_t bits = __builtin_amdgcn_ubfe(mw, 16 * ((lane >> 4) & 1), 16);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            d[q] = __umul24(__builtin_amdgcn_ubfe(bits, 4 * q, 4), 0x00204081u) & 0x01010101u;
+    } else {
+        typedef const uint32_t __attribute__((address_space(1))) *global_u32_ptr;
+        const global_u32_ptr w = (global_u32_ptr)(tile + (16 * blk + (lane & 15)) * kBlock + 16 * (lane >> 4));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d[q] = onehot4(w[q], bin << 2);
+    }
+    return mfma_v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
+}
+
+__device__ __forceinline__ mfma_v4i mfma_operand(global_u8_ptr tile, const unsigned long long *rmask, int slot,
+                                                 uint32_t bin, int blk, int lane) {
+    const int row = 16 * blk + (lane & 15), g = lane >> 4;
+    uint32_t d[4];
+    if (slot < kRowMaskSlots) {
+        typedef const uint32_t __attribute__((address_space(1))) *global_u32_ptr;
+        const global_u32_ptr w = (global_u32_ptr)(rmask + slot * kBlock + row);
+        const uint32_t bits = __builtin_amdgcn_ubfe(w[g >> 1], 16 * (g & 1), 16);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            d[q] = __umul24(__builtin_amdgcn_ubfe(bits, 4 * q, 4), 0x00204081u) & 0x01010101u;
+    } else {
+        typedef const uint32_t __attribute__((address_space(1))) *global_u32_ptr;
+        const global_u32_ptr w = (global_u32_ptr)(tile + row * kBlock + 16 * g);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d[q] = onehot4(w[q], bin << 2);
+    }
+    return mfma_v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
+}
+
+// One pass of a block triple (A, B, C) on the matrix cores for up to NX x
+// bins, NY y bins and NZ z bins.  For each 16 x 16 block (j of B, i of A)
+// and (y, z) pair, v_mfma_i32_16x16x64_i8 of the one-hot operands
+// Z[j][k] = [bin(j,k) = z] and Y[i][k] = [bin(i,k) = y] gives
+// D[j][i] = the points k of C with both bins; lane l holds
+// i = 16 ib + (l & 15) and the four j = 16 jb + 4 (l >> 4) + q, whose
+// x = bin(i, j) one-hot bytes the workgroup staged in LDS (xoh).  D (<= 64
+// each) is packed into bytes and v_dot4 with the one-hot bytes of each x bin
+// adds it to the lane's accumulator of that (x, y, z); a lane adds at most
+// 16 blocks x 4 x 64 = 4096 per accumulator.  The Y operands of the four i
+// blocks stay in registers; the Z operands are built once per j block.
+// Blocks of rows with no pair in range (i: in (A, B) and (A, C); j: in
+// (B, C)) are skipped.  At the end each accumulator is summed over the wave
+// and added to its LDS cell by one lane.  Returns the pass's VALU lane-ops by
+// the kernel's cost model: 14 per operand, per (i, j) block and (y, z) pair
+// 3 to pack D and NX dot products, per accumulator 9 for its wave sum.
+template <int NX, int NY, int NZ>
+__device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const uint4 *xoh, global_u8_ptr gAC,
+                                                        global_u8_ptr gBC, const unsigned long long *rmAC,
+                                                        const unsigned long long *rmBC, unsigned long long rows,
+                                                        unsigned long long rowsB, const uint32_t (&xs)[4],
+                                                        const uint32_t (&ys)[4], const uint32_t (&zs)[4],
+                                                        const int (&sy)[4], const int (&sz)[4], int lane, uint32_t n) {
+    uint32_t acc[NX][NY][NZ];
+#pragma unroll
+    for (int s = 0; s < NX; ++s)
+#pragma unroll
+        for (int a = 0; a < NY; ++a)
+#pragma unroll
+            for (int b = 0; b < NZ; ++b) acc[s][a][b] = 0u;
+    // The Y mask dwords of all four i blocks and the Z mask dwords of the
+    // first j block are loaded before any operand is built, and each j
+    // block's loads of the next one's, so the loads' latencies overlap.
+    uint32_t ymw[4][NY], zmw[NZ];
+#pragma unroll
+    for (int blk = 0; blk < 4; ++blk)
+#pragma unroll
+        for (int a = 0; a < NY; ++a) ymw[blk][a] = mfma_mask_dword(rmAC, sy[a], blk, lane);
+#pragma unroll
+    for (int b = 0; b < NZ; ++b) zmw[b] = mfma_mask_dword(rmBC, sz[b], 0, lane);
+    mfma_v4i yop[4][NY];
+    uint32_t iblk = 0u;
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) {
+        const bool live = ((rows >> (16 * ib)) & 0xFFFFull) != 0ull;
+        if (live) iblk |= 1u << ib;
+#pragma unroll
+        for (int a = 0; a < NY; ++a)
+            yop[ib][a] = live ? mfma_operand_from(ymw[ib][a], gAC, sy[a], ys[a], ib, lane) : mfma_v4i{0, 0, 0, 0};
+    }
+    uint32_t nops = 0u, nblocks = 0u;
+#pragma unroll 1
+    for (int jb = 0; jb < 4; ++jb) {
+        uint32_t zcur[NZ];
+#pragma unroll
+        for (int b = 0; b < NZ; ++b) {
+            zcur[b] = zmw[b];
+            zmw[b] = mfma_mask_dword(rmBC, sz[b], (jb + 1) & 3, lane);
+        }
+        if (!((rowsB >> (16 * jb)) & 0xFFFFull)) continue;
+        mfma_v4i zop[NZ];
+#pragma unroll
+        for (int b = 0; b < NZ; ++b) zop[b] = mfma_operand_from(zcur[b], gBC, sz[b], zs[b], jb, lane);
+        nops += NZ;
+#pragma unroll
+        for (int ib = 0; ib < 4; ++ib) {
+            if (!((iblk >> ib) & 1u)) continue;
+            ++nblocks;
+            const uint4 x4 = xoh[(ib * 4 + jb) * kBlock + lane];
+            const uint32_t xo[4] = {x4.x, x4.y, x4.z, x4.w};
+            constexpr int P = NY * NZ;
+            // Software pipeline: the product of pair p + 1 is issued before
+            // pair p is packed and routed, so the matrix core and the VALU
+            // overlap.
+            mfma_v4i dcur = __builtin_amdgcn_mfma_i32_16x16x64_i8(zop[0], yop[ib][0], mfma_v4i{0, 0, 0, 0}, 0, 0, 0);
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const int a = p / NZ, b = p % NZ;
+                mfma_v4i dnext = dcur;
+                if (p + 1 < P)
+                    dnext = __builtin_amdgcn_mfma_i32_16x16x64_i8(zop[(p + 1) % NZ], yop[ib][(p + 1) / NZ],
+                                                                  mfma_v4i{0, 0, 0, 0}, 0, 0, 0);
+                // the four counts (each <= 64) as the bytes of one dword
+                const uint32_t t01 = __builtin_amdgcn_perm((uint32_t)dcur[1], (uint32_t)dcur[0], 0x0c0c0400u);
+                const uint32_t t23 = __builtin_amdgcn_perm((uint32_t)dcur[3], (uint32_t)dcur[2], 0x0c0c0400u);
+                const uint32_t t4 = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+#pragma unroll
+                for (int s = 0; s < NX; ++s) acc[s][a][b] = __builtin_amdgcn_udot4(xo[s], t4, acc[s][a][b], false);
+                dcur = dnext;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NX; ++s)
+#pragma unroll
+        for (int a = 0; a < NY; ++a)
+#pragma unroll
+            for (int b = 0; b < NZ; ++b) {
+                const uint32_t tot = wave_sum(acc[s][a][b]);
+                if (tot && lane == 0) lds_add(hist, ((zs[b] * n + ys[a]) * n + xs[s]) * 4u, tot);
+            }
+    return 64ull * (14ull * ((unsigned long long)__builtin_popcount(iblk) * NY + nops) +
+                    (unsigned long long)nblocks * NY * NZ * (3 + NX) + 9ull * NX * NY * NZ);
+}
+
+// Persistent, as the bit-mask kernel: workgroups pull items (term, A, B,
+// C-range), the waves split the C blocks of an item and share the LDS
+// histogram.  Per item and chunk of up to four x bins the workgroup stages
+// the x one-hot bytes of the (A, B) tile in LDS, then every C block of the
+// item is counted in passes of up to kMfmaCY y and kMfmaCZ z bins
+// (mfma_pass).  Items and C blocks whose tiles have no pair in range are
+// skipped.
+template <int WAVES, bool ELASTIC>
+__global__ __launch_bounds__(WAVES * 64) void r123_mfma_kernel(const ElasticCtl ctl,
+                                                                  const TripletItem *__restrict__ items,
+                                                                  int nitems, int *__restrict__ counter,
+                                                                  const TripletTerm *__restrict__ terms,
+                                                                  int lds_words, int *ctlw) {
+    if (__builtin_amdgcn_readfirstlane(ctlw[kCtlForm]) != kFormBits) return;
+    unsigned long long *ops = reinterpret_cast<unsigned long long *>(ctlw + kCtlOps);
+    extern __shared__ uint4 mfma_lds[];
+    __shared__ int s_item;
+    uint32_t *hist = reinterpret_cast<uint32_t *>(mfma_lds);
+    uint4 *xoh = reinterpret_cast<uint4 *>(hist + lds_words);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int c = threadIdx.x; c < lds_words; c += blockDim.x) hist[c] = 0u;
+    bool left = false;
+    if constexpr (ELASTIC)
+        if (threadIdx.x == 0) atomicAdd(ctl.active, 1);
+    int cur = -1, cur_n = 0;
+    unsigned long long since_flush = 0, wave_ops = 0;
+    unsigned long long *cur_hist = nullptr;
+    const unsigned long long flush_limit = 0xC0000000ull;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if constexpr (ELASTIC) left = elastic_leave(ctl);
+            s_item = left ? nitems : atomicAdd(counter, 1);
+        }
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= nitems) break;
+        const TripletItem I = items[it];
+        const int term = __builtin_amdgcn_readfirstlane(I.term);
+        const TripletTerm T = terms[term];
+        const int n = __builtin_amdgcn_readfirstlane(T.n);
+        const int A = __builtin_amdgcn_readfirstlane(I.A);
+        const int B = __builtin_amdgcn_readfirstlane(I.B);
+        const int c0 = __builtin_amdgcn_readfirstlane(I.c0);
+        const int c1 = __builtin_amdgcn_readfirstlane(I.c1);
+        const unsigned long long add = (unsigned long long)(c1 - c0) * (kBlock * kBlock * kBlock);
+        if (term != cur || since_flush + add > flush_limit) {
+            if (cur >= 0) bits_flush(hist, cur_n, cur_hist);
+            cur = term;
+            cur_n = n;
+            cur_hist = T.hist;
+            since_flush = 0;
+        }
+        since_flush += add;
+        const int64_t nblk = T.nblk;
+        const MetaV mAB = load_meta(T.meta, tile_index(A, B, nblk));
+        if (!mAB.bins) continue;
+        const global_u8_ptr gAB = (global_u8_ptr)(T.tiles + tile_index(A, B, nblk) * kTileBytes);
+        uint32_t xrest = mAB.bins;
+        while (xrest) {
+            uint32_t xs[4];
+            const int nx = take_upto<kMfmaCX>(&xrest, xs);
+            // The x one-hot table of this chunk: entry (block, lane) holds the
+            // bytes of row i = 16 ib + (lane & 15), columns
+            // j = 16 jb + 4 (lane >> 4) + 0..3 of tile (A, B).
+            __syncthreads();
+            for (int e = threadIdx.x; e < 16 * kBlock; e += blockDim.x) {
+                const int blk = e >> 6, l = e & 63;
+                const int i = 16 * (blk >> 2) + (l & 15), j = 16 * (blk & 3) + 4 * (l >> 4);
+                typedef const uint32_t __attribute__((address_space(1))) *global_u32_ptr;
+                const uint32_t w = *(global_u32_ptr)(gAB + i * kBlock + j);
+                uint4 o;
+                o.x = onehot4(w, xs[0] << 2);
+                o.y = nx > 1 ? onehot4(w, xs[1] << 2) : 0u;
+                o.z = nx > 2 ? onehot4(w, xs[2] << 2) : 0u;
+                o.w = nx > 3 ? onehot4(w, xs[3] << 2) : 0u;
+                xoh[e] = o;
+            }
+            __syncthreads();
+            wave_ops += 64ull * (16 * kBlock / (64 * WAVES)) * 25;
+            for (int C = c0 + wave; C < c1; C += WAVES) {
+                const int64_t tAC = tile_index(A, C, nblk), tBC = tile_index(B, C, nblk);
+                const MetaV mAC = load_meta(T.meta, tAC), mBC = load_meta(T.meta, tBC);
+                if (!mAC.bins || !mBC.bins) continue;
+                const unsigned long long rows = mAB.rows & mAC.rows;
+                if (!rows) continue;
+                const global_u8_ptr gAC = (global_u8_ptr)(T.tiles + tAC * kTileBytes);
+                const global_u8_ptr gBC = (global_u8_ptr)(T.tiles + tBC * kTileBytes);
+                const unsigned long long *rmAC = T.rmask + tAC * kRowMaskWords;
+                const unsigned long long *rmBC = T.rmask + tBC * kRowMaskWords;
+                uint32_t zrest = mBC.bins;
+                while (zrest) {
+                    uint32_t zs[4];
+                    int sz[4];
+                    const int nz = take_upto<kMfmaCZ>(&zrest, zs);
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) sz[b] = __builtin_popcount(mBC.bins & ((1u << zs[b]) - 1u));
+                    uint32_t yrest = mAC.bins;
+                    while (yrest) {
+                        uint32_t ys[4];
+                        int sy[4];
+                        const int ny = take_upto<kMfmaCY>(&yrest, ys);
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) sy[a] = __builtin_popcount(mAC.bins & ((1u << ys[a]) - 1u));
+                        switch ((nx - 1) * 4 + (ny - 1) * 2 + (nz - 1)) {
+#define TPCF_MPASS(X, Y, Z)                                                                                 \
+    case (X - 1) * 4 + (Y - 1) * 2 + (Z - 1):                                                               \
+        wave_ops += mfma_pass<X, Y, Z>(hist, xoh, gAC, gBC, rmAC, rmBC, rows, mBC.rows, xs, ys, zs, sy, sz, lane, \
+                                       (uint32_t)n);                                                       \
+        break;
+                            TPCF_MPASS(1, 1, 1) TPCF_MPASS(1, 1, 2) TPCF_MPASS(1, 2, 1) TPCF_MPASS(1, 2, 2)
+                            TPCF_MPASS(2, 1, 1) TPCF_MPASS(2, 1, 2) TPCF_MPASS(2, 2, 1) TPCF_MPASS(2, 2, 2)
+                            TPCF_MPASS(3, 1, 1) TPCF_MPASS(3, 1, 2) TPCF_MPASS(3, 2, 1) TPCF_MPASS(3, 2, 2)
+                            TPCF_MPASS(4, 1, 1) TPCF_MPASS(4, 1, 2) TPCF_MPASS(4, 2, 1) TPCF_MPASS(4, 2, 2)
+#undef TPCF_MPASS
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (cur >= 0) bits_flush(hist, cur_n, cur_hist);
+    if (lane == 0 && wave_ops) atomicAdd(ops, wave_ops);
+    if constexpr (ELASTIC)
+        if (threadIdx.x == 0 && !left) atomicSub(ctl.active, 1);
+}
+
+// ---------------------------------------------------------------------------
 // triplet stage, run-merging form (block triples with many bins per tile)
 // ---------------------------------------------------------------------------
 
@@ -940,8 +1267,13 @@ const void *r123_triplet_kernel_ptr(int waves, int copies, bool elastic) {
     return nullptr;
 }
 
+static bool use_mfma();
+
 const void *r123_bits_kernel_ptr(int waves, bool elastic) {
 #define TPCF_PTR(W)                                                                                      \
+    if (waves == W && use_mfma())                                                                        \
+        return elastic ? reinterpret_cast<const void *>(&r123_mfma_kernel<W, true>)                       \
+                       : reinterpret_cast<const void *>(&r123_mfma_kernel<W, false>);                     \
     if (waves == W)                                                                                      \
         return elastic ? reinterpret_cast<const void *>(&r123_bits_kernel<W, true>)                       \
                        : reinterpret_cast<const void *>(&r123_bits_kernel<W, false>);
@@ -951,7 +1283,19 @@ const void *r123_bits_kernel_ptr(int waves, bool elastic) {
 }
 
 int r123_lds_bytes(int n, int copies) { return copies * runs_copy_stride(n, copies) * 4; }
-int r123_bits_lds_bytes(int n) { return bits_lds_words(n) * 4; }
+// The bit-mask form runs as matrix-core passes (r123_mfma_kernel);
+// TPCF_R123_MFMA=0 selects the VALU passes (r123_bits_kernel) instead.
+static bool use_mfma() {
+    static const bool on = [] {
+        const char *e = std::getenv("TPCF_R123_MFMA");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+int r123_bits_lds_bytes(int n) {
+    return use_mfma() ? mfma_hist_words(n) * 4 + kMfmaXohBytes : bits_lds_words(n) * 4;
+}
 
 hipError_t launch_r123_choose(const TripletTerm *terms, int nterms, int *counter, float max_bins, int force,
                               hipStream_t st) {
@@ -983,6 +1327,23 @@ hipError_t launch_r123_triplets(const ElasticCtl &ctl, const TripletItem *items,
 hipError_t launch_r123_bits(const ElasticCtl &ctl, const TripletItem *items, int nitems, int *counter,
                             const TripletTerm *terms, int max_n, int waves, int grid, hipStream_t st) {
     if (max_n < 1 || max_n > kMaxLdsR123Bins) return hipErrorInvalidValue;
+    if (use_mfma()) {
+        const int words = mfma_hist_words(max_n);
+        const size_t lds = (size_t)words * 4 + kMfmaXohBytes;
+#define TPCF_LAUNCH(W)                                                                                       \
+    if (waves == W) {                                                                                        \
+        if (ctl.active)                                                                                      \
+            hipLaunchKernelGGL((r123_mfma_kernel<W, true>), dim3(grid), dim3(W * 64), lds, st, ctl, items,      \
+                               nitems, counter, terms, words, counter);                                      \
+        else                                                                                                 \
+            hipLaunchKernelGGL((r123_mfma_kernel<W, false>), dim3(grid), dim3(W * 64), lds, st, ctl, items,     \
+                               nitems, counter, terms, words, counter);                                      \
+        return hipGetLastError();                                                                            \
+    }
+        TPCF_BITS_SHAPES(TPCF_LAUNCH)
+#undef TPCF_LAUNCH
+        return hipErrorInvalidValue;
+    }
     const int words = bits_lds_words(max_n);
     const size_t lds = (size_t)words * 4;
 #define TPCF_LAUNCH(W)                                                                                       \
