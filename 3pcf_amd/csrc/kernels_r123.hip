@@ -522,6 +522,9 @@ constexpr int kMfmaCX = 4, kMfmaCY = 2, kMfmaCZ = 2;
 // (i-block, j-block) and lane one uint4 (four x bins of one pass, each a
 // dword of four one-hot bytes).
 constexpr int kMfmaXohBytes = 16 * kBlock * 16;
+// Then a table of the one-hot bytes of every byte value: entry v holds the
+// bits of v as eight bytes 0/1 (byte t of the pair = bit t of v).
+constexpr int kMfmaLutBytes = 256 * 8;
 __host__ __device__ inline int mfma_hist_words(int n) { return (n * n * n + 3) & ~3; }
 
 // Up to M bins from the set bits of *rest (lowest first), taken off it; the
@@ -563,15 +566,19 @@ __device__ __forceinline__ uint32_t mfma_mask_dword(const unsigned long long *rm
 }
 
 // The operand from its mask dword (mfma_mask_dword) when the bin has a
-// slot, else from the tile bytes.
-__device__ __forceinline__ mfma_v4i mfma_operand_from(uint32_t mw, global_u8_ptr tile, int slot, uint32_t bin,
-                                                      int blk, int lane) {
+// slot (its two mask bytes expanded through the LDS table lut), else from
+// the tile bytes.
+__device__ __forceinline__ mfma_v4i mfma_operand_from(uint32_t mw, const uint2 *lut, global_u8_ptr tile, int slot,
+                                                      uint32_t bin, int blk, int lane) {
     uint32_t d[4];
     if (slot < kRowMaskSlots) {
-        const uint32_t bits = __builtin_amdgcn_ubfe(mw, 16 * ((lane >> 4) & 1), 16);
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            d[q] = __umul24(__builtin_amdgcn_ubfe(bits, 4 * q, 4), 0x00204081u) & 0x01010101u;
+        const uint32_t off = 16 * ((lane >> 4) & 1);
+        const uint2 lo = lut[__builtin_amdgcn_ubfe(mw, off, 8)];
+        const uint2 hi = lut[__builtin_amdgcn_ubfe(mw, off + 8, 8)];
+        d[0] = lo.x;
+        d[1] = lo.y;
+        d[2] = hi.x;
+        d[3] = hi.y;
     } else {
         typedef const uint32_t __attribute__((address_space(1))) *global_u32_ptr;
         const global_u32_ptr w = (global_u32_ptr)(tile + (16 * blk + (lane & 15)) * kBlock + 16 * (lane >> 4));
@@ -615,10 +622,12 @@ __device__ __forceinline__ mfma_v4i mfma_operand(global_u8_ptr tile, const unsig
 // Blocks of rows with no pair in range (i: in (A, B) and (A, C); j: in
 // (B, C)) are skipped.  At the end each accumulator is summed over the wave
 // and added to its LDS cell by one lane.  Returns the pass's VALU lane-ops by
-// the kernel's cost model: 14 per operand, per (i, j) block and (y, z) pair
-// 3 to pack D and NX dot products, per accumulator 9 for its wave sum.
+// the kernel's cost model: 4 per operand (two mask bytes and their table
+// addresses), per (i, j) block and (y, z) pair 3 to pack D and NX dot
+// products, per accumulator 9 for its wave sum.
 template <int NX, int NY, int NZ>
-__device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const uint4 *xoh, global_u8_ptr gAC,
+__device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const uint4 *xoh, const uint2 *lut,
+                                                        global_u8_ptr gAC,
                                                         global_u8_ptr gBC, const unsigned long long *rmAC,
                                                         const unsigned long long *rmBC, unsigned long long rows,
                                                         unsigned long long rowsB, const uint32_t (&xs)[4],
@@ -649,7 +658,7 @@ __device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const ui
         if (live) iblk |= 1u << ib;
 #pragma unroll
         for (int a = 0; a < NY; ++a)
-            yop[ib][a] = live ? mfma_operand_from(ymw[ib][a], gAC, sy[a], ys[a], ib, lane) : mfma_v4i{0, 0, 0, 0};
+            yop[ib][a] = live ? mfma_operand_from(ymw[ib][a], lut, gAC, sy[a], ys[a], ib, lane) : mfma_v4i{0, 0, 0, 0};
     }
     uint32_t nops = 0u, nblocks = 0u;
 #pragma unroll 1
@@ -663,7 +672,7 @@ __device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const ui
         if (!((rowsB >> (16 * jb)) & 0xFFFFull)) continue;
         mfma_v4i zop[NZ];
 #pragma unroll
-        for (int b = 0; b < NZ; ++b) zop[b] = mfma_operand_from(zcur[b], gBC, sz[b], zs[b], jb, lane);
+        for (int b = 0; b < NZ; ++b) zop[b] = mfma_operand_from(zcur[b], lut, gBC, sz[b], zs[b], jb, lane);
         nops += NZ;
 #pragma unroll
         for (int ib = 0; ib < 4; ++ib) {
@@ -702,7 +711,7 @@ __device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const ui
                 const uint32_t tot = wave_sum(acc[s][a][b]);
                 if (tot && lane == 0) lds_add(hist, ((zs[b] * n + ys[a]) * n + xs[s]) * 4u, tot);
             }
-    return 64ull * (14ull * ((unsigned long long)__builtin_popcount(iblk) * NY + nops) +
+    return 64ull * (4ull * ((unsigned long long)__builtin_popcount(iblk) * NY + nops) +
                     (unsigned long long)nblocks * NY * NZ * (3 + NX) + 9ull * NX * NY * NZ);
 }
 
@@ -725,9 +734,13 @@ __global__ __launch_bounds__(WAVES * 64) void r123_mfma_kernel(const ElasticCtl 
     __shared__ int s_item;
     uint32_t *hist = reinterpret_cast<uint32_t *>(mfma_lds);
     uint4 *xoh = reinterpret_cast<uint4 *>(hist + lds_words);
+    uint2 *lut = reinterpret_cast<uint2 *>(xoh + 16 * kBlock);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int c = threadIdx.x; c < lds_words; c += blockDim.x) hist[c] = 0u;
+    for (int v = threadIdx.x; v < 256; v += blockDim.x)
+        lut[v] = make_uint2(__umul24((uint32_t)v & 0xFu, 0x00204081u) & 0x01010101u,
+                            __umul24((uint32_t)v >> 4, 0x00204081u) & 0x01010101u);
     bool left = false;
     if constexpr (ELASTIC)
         if (threadIdx.x == 0) atomicAdd(ctl.active, 1);
@@ -814,7 +827,7 @@ __global__ __launch_bounds__(WAVES * 64) void r123_mfma_kernel(const ElasticCtl 
                         switch ((nx - 1) * 4 + (ny - 1) * 2 + (nz - 1)) {
 #define TPCF_MPASS(X, Y, Z)                                                                                 \
     case (X - 1) * 4 + (Y - 1) * 2 + (Z - 1):                                                               \
-        wave_ops += mfma_pass<X, Y, Z>(hist, xoh, gAC, gBC, rmAC, rmBC, rows, mBC.rows, xs, ys, zs, sy, sz, lane, \
+        wave_ops += mfma_pass<X, Y, Z>(hist, xoh, lut, gAC, gBC, rmAC, rmBC, rows, mBC.rows, xs, ys, zs, sy, sz, lane, \
                                        (uint32_t)n);                                                       \
         break;
                             TPCF_MPASS(1, 1, 1) TPCF_MPASS(1, 1, 2) TPCF_MPASS(1, 2, 1) TPCF_MPASS(1, 2, 2)
@@ -1294,7 +1307,7 @@ static bool use_mfma() {
 }
 
 int r123_bits_lds_bytes(int n) {
-    return use_mfma() ? mfma_hist_words(n) * 4 + kMfmaXohBytes : bits_lds_words(n) * 4;
+    return use_mfma() ? mfma_hist_words(n) * 4 + kMfmaXohBytes + kMfmaLutBytes : bits_lds_words(n) * 4;
 }
 
 hipError_t launch_r123_choose(const TripletTerm *terms, int nterms, int *counter, float max_bins, int force,
@@ -1329,7 +1342,7 @@ hipError_t launch_r123_bits(const ElasticCtl &ctl, const TripletItem *items, int
     if (max_n < 1 || max_n > kMaxLdsR123Bins) return hipErrorInvalidValue;
     if (use_mfma()) {
         const int words = mfma_hist_words(max_n);
-        const size_t lds = (size_t)words * 4 + kMfmaXohBytes;
+        const size_t lds = (size_t)words * 4 + kMfmaXohBytes + kMfmaLutBytes;
 #define TPCF_LAUNCH(W)                                                                                       \
     if (waves == W) {                                                                                        \
         if (ctl.active)                                                                                      \
