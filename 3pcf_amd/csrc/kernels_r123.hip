@@ -544,6 +544,36 @@ __device__ __forceinline__ int take_upto(uint32_t *rest, uint32_t (&v)[4]) {
     return k;
 }
 
+// Sums over the 64 lanes of K values per lane (K a multiple of 4), uniform:
+// v_permlane32_swap pairs values so one register holds the 32-lane halves'
+// sums of two values (the first in lanes 0-31), v_permlane16_swap pairs
+// those so one register holds four values' 16-lane sums (rows 0-3: values
+// 4m, 4m + 2, 4m + 1, 4m + 3), then DPP sums within the rows and the four row
+// ends are read.  3.5 VALU instructions per value, against 8 for wave_sum.
+template <int K>
+__device__ __forceinline__ void wave_sums(const uint32_t (&v)[K], uint32_t (&out)[K]) {
+    static_assert(K % 4 == 0, "wave_sums takes a multiple of 4 values");
+    uint32_t u[K / 2];
+#pragma unroll
+    for (int m = 0; m < K / 2; ++m) {
+        const auto r = __builtin_amdgcn_permlane32_swap(v[2 * m], v[2 * m + 1], false, false);
+        u[m] = r[0] + r[1];
+    }
+#pragma unroll
+    for (int m = 0; m < K / 4; ++m) {
+        const auto s = __builtin_amdgcn_permlane16_swap(u[2 * m], u[2 * m + 1], false, false);
+        uint32_t x = s[0] + s[1];
+        x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+        x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+        x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+        x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+        out[4 * m] = (uint32_t)__builtin_amdgcn_readlane((int)x, 15);
+        out[4 * m + 2] = (uint32_t)__builtin_amdgcn_readlane((int)x, 31);
+        out[4 * m + 1] = (uint32_t)__builtin_amdgcn_readlane((int)x, 47);
+        out[4 * m + 3] = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+    }
+}
+
 // The four bytes of w that equal v (v < 0x80) as bytes 1, the others 0.
 __device__ inline uint32_t onehot4(uint32_t w, uint32_t v) {
     const uint32_t t = w ^ (v * 0x01010101u);
@@ -620,11 +650,11 @@ __device__ __forceinline__ mfma_v4i mfma_operand(global_u8_ptr tile, const unsig
 // 16 blocks x 4 x 64 = 4096 per accumulator.  The Y operands of the four i
 // blocks stay in registers; the Z operands are built once per j block.
 // Blocks of rows with no pair in range (i: in (A, B) and (A, C); j: in
-// (B, C)) are skipped.  At the end each accumulator is summed over the wave
-// and added to its LDS cell by one lane.  Returns the pass's VALU lane-ops by
-// the kernel's cost model: 4 per operand (two mask bytes and their table
-// addresses), per (i, j) block and (y, z) pair 3 to pack D and NX dot
-// products, per accumulator 9 for its wave sum.
+// (B, C)) are skipped.  At the end the accumulators are summed over the wave
+// (wave_sums) and each added to its LDS cell by one lane.  Returns the
+// pass's VALU lane-ops by the kernel's cost model: 4 per operand (two mask
+// bytes and their table addresses), per (i, j) block and (y, z) pair 3 to
+// pack D and NX dot products, per four accumulators 14 for their wave sums.
 template <int NX, int NY, int NZ>
 __device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const uint4 *xoh, const uint2 *lut,
                                                         global_u8_ptr gAC,
@@ -702,17 +732,18 @@ __device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const ui
             }
         }
     }
+    constexpr int K = NX * NY * NZ, K4 = (K + 3) & ~3;
+    uint32_t v[K4], tot[K4];
 #pragma unroll
-    for (int s = 0; s < NX; ++s)
+    for (int q = 0; q < K4; ++q) v[q] = q < K ? acc[q / (NY * NZ)][(q / NZ) % NY][q % NZ] : 0u;
+    wave_sums<K4>(v, tot);
 #pragma unroll
-        for (int a = 0; a < NY; ++a)
-#pragma unroll
-            for (int b = 0; b < NZ; ++b) {
-                const uint32_t tot = wave_sum(acc[s][a][b]);
-                if (tot && lane == 0) lds_add(hist, ((zs[b] * n + ys[a]) * n + xs[s]) * 4u, tot);
-            }
+    for (int q = 0; q < K; ++q) {
+        const int s = q / (NY * NZ), a = (q / NZ) % NY, b = q % NZ;
+        if (tot[q] && lane == 0) lds_add(hist, ((zs[b] * n + ys[a]) * n + xs[s]) * 4u, tot[q]);
+    }
     return 64ull * (4ull * ((unsigned long long)__builtin_popcount(iblk) * NY + nops) +
-                    (unsigned long long)nblocks * NY * NZ * (3 + NX) + 9ull * NX * NY * NZ);
+                    (unsigned long long)nblocks * NY * NZ * (3 + NX) + 14ull * (K4 / 4));
 }
 
 // Persistent, as the bit-mask kernel: workgroups pull items (term, A, B,
