@@ -10,16 +10,24 @@
 //                         a rigorous error bound (or the requested relative
 //                         window) of an edge recomputed in FP64; or FP64
 //                         throughout.
-//   r123_triplet_kernel   persistent triplet stage over a work queue of
-//                         (term, A, B, C-range) block triples: for each row i
-//                         of A and point j of B (a lane), the points k of C
-//                         with bins y = bin(i,k) and z = bin(j,k) are counted
-//                         64 at a time as popcount(row mask of y & row mask of
-//                         z), bin pair by bin pair, into per-lane accumulators
-//                         of x = bin(i,j); wave sums go to an unsorted-cube LDS
-//                         histogram folded into sorted uint64 cells on flush.
-//                         Block pairs, C blocks and rows with no pair in range
-//                         are skipped.
+//   r123_mfma_kernel      persistent triplet stage over a work queue of
+//                         (term, A, B, C-range) block triples, bit-mask form
+//                         on the matrix cores: per 16 x 16 block (j of B,
+//                         i of A) and bin pair (y, z), an int8 product of the
+//                         one-hot matrices [bin(j,k) = z] and [bin(i,k) = y]
+//                         counts the points k of C with both bins; v_dot4
+//                         with the one-hot bytes of x = bin(i,j) routes the
+//                         counts to per-lane accumulators; wave sums go to an
+//                         unsorted-cube LDS histogram folded into sorted
+//                         uint64 cells on flush.  Block pairs, C blocks and
+//                         16-row blocks with no pair in range are skipped.
+//   r123_bits_kernel      the same passes on the VALU: for each row i of A
+//                         and point j of B (a lane), popcount(row mask of y &
+//                         row mask of z) counts 64 points k at a time.
+//   r123_runs_kernel      run-merging form for tiles with many bins: one LDS
+//                         atomic per run of equal cells along a row.
+//   r123_choose_kernel    picks the form per launch from the pair stage's
+//                         tile statistics.
 //   r123_plain_kernel     the FP64 baseline: all three distances of every
 //                         triplet recomputed in FP64 and binned, like the
 //                         reference's distance() (src/3pcf_C_version.cc:54-142).

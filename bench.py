@@ -61,7 +61,8 @@ def valu_roofline(ops: float, seconds: float) -> dict:
         "bound": "valu",
         "achieved": achieved / 1e12,
         "peak": VALU_PEAK / 1e12,
-        "unit": "TOP/s (VALU lane-ops of the bit-mask triplet kernel's cost model, counted by the kernel)",
+        "unit": "TOP/s (VALU lane-ops of the bit-mask triplet kernel's cost model, counted by the kernel; "
+                "matrix-core passes: 4 per operand, per 16x16 block and bin pair 1 + 3 + NX, 3.5 per accumulator)",
         "frac": achieved / VALU_PEAK,
         "traffic": None,
     }

@@ -1,4 +1,4 @@
-"""GPU: the two (r1,r2,r3) LDS triplet kernels -- bit-mask (work per bin pair
+"""GPU: the two (r1,r2,r3) LDS triplet forms -- bit-mask (work per bin pair
 of a block triple) and run-merging (work per triplet) -- each forced through
 TPCF_R123_FORM, on geometries where either would be chosen: both equal the
 CPU oracle bit for bit, the stats name the form that counted, and the
@@ -74,3 +74,40 @@ def test_both_forms_equal_the_oracle(orc, name, nd, nr, size, seed, spec, mode):
     print(f"  auto: form {st.triplet_form}, mean bins {st.tile_bins_mean:.3f}, fill {st.tile_fill:.3f}", flush=True)
     assert st.triplet_form == (1 if st.tile_bins_mean <= 6.3 * math.sqrt(st.tile_fill) else 0)
     assert np.array_equal(got, want)
+
+
+VALU_CHILD = r"""
+import importlib, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+tp = importlib.import_module("3pcf_amd")
+synth = importlib.import_module("3pcf_amd.synth")
+d = synth.neyman_scott(1000, nclust=20, sigma=2.0, size=100.0, seed=3)
+r = synth.uniform_cube(800, 100.0, seed=4)
+with tp.Context(0) as ctx:
+    cd, cr = ctx.catalog(d), ctx.catalog(r)
+    out, st = ctx.count(tp.four_term_job(cd, cr), tp.r123_spec(16, 1.0, 180.0, "log"), tp.MODE_FAST)
+np.save(sys.argv[2], out)
+print("form", st.triplet_form, "ops", st.mask_ops, flush=True)
+"""
+
+
+def test_valu_bit_mask_passes_equal_the_oracle(orc, tmp_path):
+    """The bit-mask passes on the VALU (TPCF_R123_MFMA=0, read once per
+    process, so in a child process) equal the oracle bit for bit."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    dest = str(tmp_path / "valu.npy")
+    env = dict(os.environ, TPCF_R123_MFMA="0", TPCF_R123_FORM="bits")
+    res = subprocess.run([sys.executable, "-c", VALU_CHILD, root, dest], env=env, capture_output=True, text=True,
+                         timeout=240)
+    print(res.stdout, res.stderr[-2000:], flush=True)
+    assert res.returncode == 0
+    form, ops = res.stdout.split()[1], int(res.stdout.split()[3])
+    assert form == "1" and ops > 0
+    d = synth.neyman_scott(1000, nclust=20, sigma=2.0, size=100.0, seed=3)
+    r = synth.uniform_cube(800, 100.0, seed=4)
+    want = orc.clean_count([d, r], tp.FOUR_TERMS, orc.R123, [orc.axis(16, 1.0, 180.0, "log")])
+    assert np.array_equal(np.load(dest), want)
