@@ -595,12 +595,21 @@ __device__ inline uint32_t onehot4(uint32_t w, uint32_t v) {
 // B operands of the instruction both take this k order, so a product sums
 // over every column once.  From the tile's stored row mask when the bin has
 // a slot, else from the tile bytes.
+// A buffer resource over a tile's stored row masks (kRowMaskWords 64-bit
+// words): the mask loads of the passes take their base from it and their
+// offsets as operands, with no per-lane address arithmetic.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mfma_mask_rsrc(const unsigned long long *rmask) {
+    // dword 3: 32-bit data format, as for untyped dword loads on gfx9
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<unsigned long long *>(rmask), (short)0,
+                                             kRowMaskWords * 8, 0x00020000);
+}
 // The dword of the stored row mask an operand needs: bits 16 * (l >> 4) ..
-// + 15 of row 16 * blk + (l & 15) are in dword (l >> 5) of that row's mask.
-__device__ __forceinline__ uint32_t mfma_mask_dword(const unsigned long long *rmask, int slot, int blk, int lane) {
-    typedef const uint32_t __attribute__((address_space(1))) *global_u32_ptr;
+// + 15 of row 16 * blk + (l & 15) are in dword (l >> 5) of that row's mask;
+// lane_off = 8 (l & 15) + 4 (l >> 5) is its byte offset from the block's
+// first row mask.
+__device__ __forceinline__ uint32_t mfma_mask_dword(__amdgpu_buffer_rsrc_t rm, int slot, int blk, uint32_t lane_off) {
     if (slot >= kRowMaskSlots) return 0u;
-    return ((global_u32_ptr)(rmask + slot * kBlock + 16 * blk + (lane & 15)))[lane >> 5];
+    return __builtin_amdgcn_raw_buffer_load_b32(rm, lane_off, (slot * kBlock + 16 * blk) * 8, 0);
 }
 
 // The operand from its mask dword (mfma_mask_dword) when the bin has a
@@ -655,9 +664,10 @@ __device__ __forceinline__ mfma_v4i mfma_operand(global_u8_ptr tile, const unsig
 // x = bin(i, j) one-hot bytes the workgroup staged in LDS (xoh).  D (<= 64
 // each) is packed into bytes and v_dot4 with the one-hot bytes of each x bin
 // adds it to the lane's accumulator of that (x, y, z); a lane adds at most
-// 16 blocks x 4 x 64 = 4096 per accumulator.  The Y operands of the four i
-// blocks stay in registers; the Z operands are built once per j block.
-// Blocks of rows with no pair in range (i: in (A, B) and (A, C); j: in
+// 16 blocks x 4 x 64 = 4096 per accumulator.  The i blocks are taken in two
+// halves: the Y operands of a half stay in registers while every j block's
+// Z operands are built and counted against them (79 VGPRs: 6 waves per
+// SIMD, three workgroups of 8 waves per CU).  Blocks of rows with no pair in range (i: in (A, B) and (A, C); j: in
 // (B, C)) are skipped.  At the end the accumulators are summed over the wave
 // (wave_sums) and each added to its LDS cell by one lane.  Returns the
 // pass's VALU lane-ops by the kernel's cost model: 4 per operand (two mask
@@ -666,8 +676,8 @@ __device__ __forceinline__ mfma_v4i mfma_operand(global_u8_ptr tile, const unsig
 template <int NX, int NY, int NZ>
 __device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const uint4 *xoh, const uint2 *lut,
                                                         global_u8_ptr gAC,
-                                                        global_u8_ptr gBC, const unsigned long long *rmAC,
-                                                        const unsigned long long *rmBC, unsigned long long rows,
+                                                        global_u8_ptr gBC, __amdgpu_buffer_rsrc_t rmAC,
+                                                        __amdgpu_buffer_rsrc_t rmBC, unsigned long long rows,
                                                         unsigned long long rowsB, const uint32_t (&xs)[4],
                                                         const uint32_t (&ys)[4], const uint32_t (&zs)[4],
                                                         const int (&sy)[4], const int (&sz)[4], int lane, uint32_t n) {
@@ -678,65 +688,70 @@ __device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const ui
         for (int a = 0; a < NY; ++a)
 #pragma unroll
             for (int b = 0; b < NZ; ++b) acc[s][a][b] = 0u;
-    // The Y mask dwords of all four i blocks and the Z mask dwords of the
-    // first j block are loaded before any operand is built, and each j
-    // block's loads of the next one's, so the loads' latencies overlap.
-    uint32_t ymw[4][NY], zmw[NZ];
-#pragma unroll
-    for (int blk = 0; blk < 4; ++blk)
-#pragma unroll
-        for (int a = 0; a < NY; ++a) ymw[blk][a] = mfma_mask_dword(rmAC, sy[a], blk, lane);
-#pragma unroll
-    for (int b = 0; b < NZ; ++b) zmw[b] = mfma_mask_dword(rmBC, sz[b], 0, lane);
-    mfma_v4i yop[4][NY];
+    const uint32_t lane_off = 8u * (uint32_t)(lane & 15) + 4u * (uint32_t)(lane >> 5);
     uint32_t iblk = 0u;
 #pragma unroll
-    for (int ib = 0; ib < 4; ++ib) {
-        const bool live = ((rows >> (16 * ib)) & 0xFFFFull) != 0ull;
-        if (live) iblk |= 1u << ib;
-#pragma unroll
-        for (int a = 0; a < NY; ++a)
-            yop[ib][a] = live ? mfma_operand_from(ymw[ib][a], lut, gAC, sy[a], ys[a], ib, lane) : mfma_v4i{0, 0, 0, 0};
-    }
+    for (int ib = 0; ib < 4; ++ib)
+        if ((rows >> (16 * ib)) & 0xFFFFull) iblk |= 1u << ib;
     uint32_t nops = 0u, nblocks = 0u;
+    // The i blocks in two halves of two: the Y operands of a half stay in
+    // registers while every j block is counted against them (the Z operands
+    // are built once per half and j block).
 #pragma unroll 1
-    for (int jb = 0; jb < 4; ++jb) {
-        uint32_t zcur[NZ];
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t hblk = (iblk >> (2 * h)) & 3u;
+        if (!hblk) continue;
+        // The Y mask dwords of the half and the Z mask dwords of the first
+        // j block are loaded before any operand is built, and each j block's
+        // loads of the next one's, so the loads' latencies overlap.
+        uint32_t ymw[2][NY], zmw[NZ];
 #pragma unroll
-        for (int b = 0; b < NZ; ++b) {
-            zcur[b] = zmw[b];
-            zmw[b] = mfma_mask_dword(rmBC, sz[b], (jb + 1) & 3, lane);
-        }
-        if (!((rowsB >> (16 * jb)) & 0xFFFFull)) continue;
-        mfma_v4i zop[NZ];
+        for (int u = 0; u < 2; ++u)
 #pragma unroll
-        for (int b = 0; b < NZ; ++b) zop[b] = mfma_operand_from(zcur[b], lut, gBC, sz[b], zs[b], jb, lane);
-        nops += NZ;
+            for (int a = 0; a < NY; ++a) ymw[u][a] = mfma_mask_dword(rmAC, sy[a], 2 * h + u, lane_off);
 #pragma unroll
-        for (int ib = 0; ib < 4; ++ib) {
-            if (!((iblk >> ib) & 1u)) continue;
-            ++nblocks;
-            const uint4 x4 = xoh[(ib * 4 + jb) * kBlock + lane];
-            const uint32_t xo[4] = {x4.x, x4.y, x4.z, x4.w};
-            constexpr int P = NY * NZ;
-            // Software pipeline: the product of pair p + 1 is issued before
-            // pair p is packed and routed, so the matrix core and the VALU
-            // overlap.
-            mfma_v4i dcur = __builtin_amdgcn_mfma_i32_16x16x64_i8(zop[0], yop[ib][0], mfma_v4i{0, 0, 0, 0}, 0, 0, 0);
+        for (int b = 0; b < NZ; ++b) zmw[b] = mfma_mask_dword(rmBC, sz[b], 0, lane_off);
+        mfma_v4i yop[2][NY];
 #pragma unroll
-            for (int p = 0; p < P; ++p) {
-                const int a = p / NZ, b = p % NZ;
-                mfma_v4i dnext = dcur;
-                if (p + 1 < P)
-                    dnext = __builtin_amdgcn_mfma_i32_16x16x64_i8(zop[(p + 1) % NZ], yop[ib][(p + 1) / NZ],
-                                                                  mfma_v4i{0, 0, 0, 0}, 0, 0, 0);
-                // the four counts (each <= 64) as the bytes of one dword
-                const uint32_t t01 = __builtin_amdgcn_perm((uint32_t)dcur[1], (uint32_t)dcur[0], 0x0c0c0400u);
-                const uint32_t t23 = __builtin_amdgcn_perm((uint32_t)dcur[3], (uint32_t)dcur[2], 0x0c0c0400u);
-                const uint32_t t4 = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+        for (int u = 0; u < 2; ++u)
 #pragma unroll
-                for (int s = 0; s < NX; ++s) acc[s][a][b] = __builtin_amdgcn_udot4(xo[s], t4, acc[s][a][b], false);
-                dcur = dnext;
+            for (int a = 0; a < NY; ++a)
+                yop[u][a] = ((hblk >> u) & 1u) ? mfma_operand_from(ymw[u][a], lut, gAC, sy[a], ys[a], 2 * h + u, lane)
+                                               : mfma_v4i{0, 0, 0, 0};
+#pragma unroll 1
+        for (int jb = 0; jb < 4; ++jb) {
+            uint32_t zcur[NZ];
+#pragma unroll
+            for (int b = 0; b < NZ; ++b) {
+                zcur[b] = zmw[b];
+                zmw[b] = mfma_mask_dword(rmBC, sz[b], (jb + 1) & 3, lane_off);
+            }
+            if (!((rowsB >> (16 * jb)) & 0xFFFFull)) continue;
+            mfma_v4i zop[NZ];
+#pragma unroll
+            for (int b = 0; b < NZ; ++b) zop[b] = mfma_operand_from(zcur[b], lut, gBC, sz[b], zs[b], jb, lane);
+            nops += NZ;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (!((hblk >> u) & 1u)) continue;
+                const int ib = 2 * h + u;
+                ++nblocks;
+                const uint4 x4 = xoh[(ib * 4 + jb) * kBlock + lane];
+                const uint32_t xo[4] = {x4.x, x4.y, x4.z, x4.w};
+                constexpr int P = NY * NZ;
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    const int a = p / NZ, b = p % NZ;
+                    const mfma_v4i dcur =
+                        __builtin_amdgcn_mfma_i32_16x16x64_i8(zop[b], yop[u][a], mfma_v4i{0, 0, 0, 0}, 0, 0, 0);
+                    // the four counts (each <= 64) as the bytes of one dword
+                    const uint32_t t01 = __builtin_amdgcn_perm((uint32_t)dcur[1], (uint32_t)dcur[0], 0x0c0c0400u);
+                    const uint32_t t23 = __builtin_amdgcn_perm((uint32_t)dcur[3], (uint32_t)dcur[2], 0x0c0c0400u);
+                    const uint32_t t4 = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+#pragma unroll
+                    for (int s = 0; s < NX; ++s)
+                        acc[s][a][b] = __builtin_amdgcn_udot4(xo[s], t4, acc[s][a][b], false);
+                }
             }
         }
     }
@@ -762,11 +777,9 @@ __device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const ui
 // (mfma_pass).  Items and C blocks whose tiles have no pair in range are
 // skipped.
 template <int WAVES, bool ELASTIC>
-__global__ __launch_bounds__(WAVES * 64) void r123_mfma_kernel(const ElasticCtl ctl,
-                                                                  const TripletItem *__restrict__ items,
-                                                                  int nitems, int *__restrict__ counter,
-                                                                  const TripletTerm *__restrict__ terms,
-                                                                  int lds_words, int *ctlw) {
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(6, 6))) void r123_mfma_kernel(
+    const ElasticCtl ctl, const TripletItem *__restrict__ items, int nitems, int *__restrict__ counter,
+    const TripletTerm *__restrict__ terms, int lds_words, int *ctlw) {
     if (__builtin_amdgcn_readfirstlane(ctlw[kCtlForm]) != kFormBits) return;
     unsigned long long *ops = reinterpret_cast<unsigned long long *>(ctlw + kCtlOps);
     extern __shared__ uint4 mfma_lds[];
@@ -847,8 +860,8 @@ __global__ __launch_bounds__(WAVES * 64) void r123_mfma_kernel(const ElasticCtl 
                 if (!rows) continue;
                 const global_u8_ptr gAC = (global_u8_ptr)(T.tiles + tAC * kTileBytes);
                 const global_u8_ptr gBC = (global_u8_ptr)(T.tiles + tBC * kTileBytes);
-                const unsigned long long *rmAC = T.rmask + tAC * kRowMaskWords;
-                const unsigned long long *rmBC = T.rmask + tBC * kRowMaskWords;
+                const __amdgpu_buffer_rsrc_t rmAC = mfma_mask_rsrc(T.rmask + tAC * kRowMaskWords);
+                const __amdgpu_buffer_rsrc_t rmBC = mfma_mask_rsrc(T.rmask + tBC * kRowMaskWords);
                 uint32_t zrest = mBC.bins;
                 while (zrest) {
                     uint32_t zs[4];
