@@ -614,11 +614,13 @@ __device__ __forceinline__ uint32_t mfma_mask_dword(__amdgpu_buffer_rsrc_t rm, i
 
 // The operand from its mask dword (mfma_mask_dword) when the bin has a
 // slot (its two mask bytes expanded through the LDS table lut), else from
-// the tile bytes.
+// the tile bytes; *live (uniform) is whether the bin occurs in the block's
+// 16 rows at all.
 __device__ __forceinline__ mfma_v4i mfma_operand_from(uint32_t mw, const uint2 *lut, global_u8_ptr tile, int slot,
-                                                      uint32_t bin, int blk, int lane) {
+                                                      uint32_t bin, int blk, int lane, bool *live) {
     uint32_t d[4];
     if (slot < kRowMaskSlots) {
+        *live = __builtin_amdgcn_ballot_w64(mw != 0u) != 0ull;
         const uint32_t off = 16 * ((lane >> 4) & 1);
         const uint2 lo = lut[__builtin_amdgcn_ubfe(mw, off, 8)];
         const uint2 hi = lut[__builtin_amdgcn_ubfe(mw, off + 8, 8)];
@@ -631,6 +633,7 @@ __device__ __forceinline__ mfma_v4i mfma_operand_from(uint32_t mw, const uint2 *
         const global_u32_ptr w = (global_u32_ptr)(tile + (16 * blk + (lane & 15)) * kBlock + 16 * (lane >> 4));
 #pragma unroll
         for (int q = 0; q < 4; ++q) d[q] = onehot4(w[q], bin << 2);
+        *live = __builtin_amdgcn_ballot_w64((d[0] | d[1] | d[2] | d[3]) != 0u) != 0ull;
     }
     return mfma_v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
 }
@@ -667,8 +670,12 @@ __device__ __forceinline__ mfma_v4i mfma_operand(global_u8_ptr tile, const unsig
 // 16 blocks x 4 x 64 = 4096 per accumulator.  The i blocks are taken in two
 // halves: the Y operands of a half stay in registers while every j block's
 // Z operands are built and counted against them (79 VGPRs: 6 waves per
-// SIMD, three workgroups of 8 waves per CU).  Blocks of rows with no pair in range (i: in (A, B) and (A, C); j: in
-// (B, C)) are skipped.  At the end the accumulators are summed over the wave
+// SIMD, three workgroups of 8 waves per CU).  Blocks of rows with no pair in
+// range (i: in (A, B) and (A, C); j: in (B, C)) are skipped, and so is a
+// 16 x 16 block when none of the pass's y bins occurs in its i rows or none
+// of its z bins in its j rows (one ballot per operand gives a uniform flag;
+// within a block every (y, z) pair is counted, which measured faster than a
+// test per pair).  At the end the accumulators are summed over the wave
 // (wave_sums) and each added to its LDS cell by one lane.  Returns the
 // pass's VALU lane-ops by the kernel's cost model: 4 per operand (two mask
 // bytes and their table addresses), per (i, j) block and (y, z) pair 3 to
@@ -712,12 +719,18 @@ __device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const ui
 #pragma unroll
         for (int b = 0; b < NZ; ++b) zmw[b] = mfma_mask_dword(rmBC, sz[b], 0, lane_off);
         mfma_v4i yop[2][NY];
+        uint32_t ylive = 0u;  // bit u: some y bin of the pass occurs in i block 2 h + u
 #pragma unroll
         for (int u = 0; u < 2; ++u)
 #pragma unroll
-            for (int a = 0; a < NY; ++a)
-                yop[u][a] = ((hblk >> u) & 1u) ? mfma_operand_from(ymw[u][a], lut, gAC, sy[a], ys[a], 2 * h + u, lane)
-                                               : mfma_v4i{0, 0, 0, 0};
+            for (int a = 0; a < NY; ++a) {
+                bool l = false;
+                yop[u][a] = ((hblk >> u) & 1u)
+                                ? mfma_operand_from(ymw[u][a], lut, gAC, sy[a], ys[a], 2 * h + u, lane, &l)
+                                : mfma_v4i{0, 0, 0, 0};
+                if (l) ylive |= 1u << u;
+            }
+        if (!ylive) continue;
 #pragma unroll 1
         for (int jb = 0; jb < 4; ++jb) {
             uint32_t zcur[NZ];
@@ -728,12 +741,18 @@ __device__ __forceinline__ unsigned long long mfma_pass(uint32_t *hist, const ui
             }
             if (!((rowsB >> (16 * jb)) & 0xFFFFull)) continue;
             mfma_v4i zop[NZ];
+            bool zlive = false;  // some z bin of the pass occurs in j block jb
 #pragma unroll
-            for (int b = 0; b < NZ; ++b) zop[b] = mfma_operand_from(zcur[b], lut, gBC, sz[b], zs[b], jb, lane);
+            for (int b = 0; b < NZ; ++b) {
+                bool l = false;
+                zop[b] = mfma_operand_from(zcur[b], lut, gBC, sz[b], zs[b], jb, lane, &l);
+                zlive = zlive || l;
+            }
             nops += NZ;
+            if (!zlive) continue;
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
-                if (!((hblk >> u) & 1u)) continue;
+                if (!((ylive >> u) & 1u)) continue;
                 const int ib = 2 * h + u;
                 ++nblocks;
                 const uint4 x4 = xoh[(ib * 4 + jb) * kBlock + lane];
