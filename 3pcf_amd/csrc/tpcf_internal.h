@@ -30,7 +30,7 @@ struct TileMeta {
 // Row masks kept per tile, for its kRowMaskSlots lowest bins present: word
 // [slot * kBlock + i] has bit j set where pair (i, j) of the tile has that
 // bin.  Bins past the stored slots are built by the triplet kernel.
-constexpr int kRowMaskSlots = 8;
+constexpr int kRowMaskSlots = 12;
 constexpr int kRowMaskWords = kRowMaskSlots * kBlock;
 
 // One (job, term) of the triplet stage.
