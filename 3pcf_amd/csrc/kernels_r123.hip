@@ -638,26 +638,6 @@ __device__ __forceinline__ mfma_v4i mfma_operand_from(uint32_t mw, const uint2 *
     return mfma_v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
 }
 
-__device__ __forceinline__ mfma_v4i mfma_operand(global_u8_ptr tile, const unsigned long long *rmask, int slot,
-                                                 uint32_t bin, int blk, int lane) {
-    const int row = 16 * blk + (lane & 15), g = lane >> 4;
-    uint32_t d[4];
-    if (slot < kRowMaskSlots) {
-        typedef const uint32_t __attribute__((address_space(1))) *global_u32_ptr;
-        const global_u32_ptr w = (global_u32_ptr)(rmask + slot * kBlock + row);
-        const uint32_t bits = __builtin_amdgcn_ubfe(w[g >> 1], 16 * (g & 1), 16);
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            d[q] = __umul24(__builtin_amdgcn_ubfe(bits, 4 * q, 4), 0x00204081u) & 0x01010101u;
-    } else {
-        typedef const uint32_t __attribute__((address_space(1))) *global_u32_ptr;
-        const global_u32_ptr w = (global_u32_ptr)(tile + row * kBlock + 16 * g);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) d[q] = onehot4(w[q], bin << 2);
-    }
-    return mfma_v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
-}
-
 // One pass of a block triple (A, B, C) on the matrix cores for up to NX x
 // bins, NY y bins and NZ z bins.  For each 16 x 16 block (j of B, i of A)
 // and (y, z) pair, v_mfma_i32_16x16x64_i8 of the one-hot operands
