@@ -1632,7 +1632,7 @@ struct tpcf_batch {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     QueueDev qd;
     int shift = 2;
-    TripletShape shape{8, 1, 0, 8, 0, -1, 6.3f};
+    TripletShape shape{8, 1, 0, 8, 0, -1, 7.0f};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string tuning;  // autotuner log

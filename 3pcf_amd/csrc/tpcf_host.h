@@ -89,7 +89,7 @@ struct LaunchShape {
     // 0 run-merging, 1 bit-mask; and the scale of the choice (the bit-mask
     // kernel when the tiles' mean bins <= bits_max * sqrt(their fill)).
     int form = -1;
-    float bits_max = 6.3f;
+    float bits_max = 7.0f;
 };
 
 }  // namespace tpcf

@@ -3,7 +3,7 @@ of a block triple) and run-merging (work per triplet) -- each forced through
 TPCF_R123_FORM, on geometries where either would be chosen: both equal the
 CPU oracle bit for bit, the stats name the form that counted, and the
 automatic choice follows the tiles' mean bins m and fill f (bit-mask when
-m <= 6.3 sqrt(f))."""
+m <= 7.0 sqrt(f))."""
 import importlib
 import math
 import os
@@ -72,7 +72,7 @@ def test_both_forms_equal_the_oracle(orc, name, nd, nr, size, seed, spec, mode):
     got, st = count_with_form(None, make, spec, mode)
     assert st.triplet_form in (0, 1) and st.tile_bins_mean > 0 and 0 < st.tile_fill <= 1
     print(f"  auto: form {st.triplet_form}, mean bins {st.tile_bins_mean:.3f}, fill {st.tile_fill:.3f}", flush=True)
-    assert st.triplet_form == (1 if st.tile_bins_mean <= 6.3 * math.sqrt(st.tile_fill) else 0)
+    assert st.triplet_form == (1 if st.tile_bins_mean <= 7.0 * math.sqrt(st.tile_fill) else 0)
     assert np.array_equal(got, want)
 
 
