@@ -779,7 +779,10 @@ template <int WAVES, bool ELASTIC>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(6, 6))) void r123_mfma_kernel(
     const ElasticCtl ctl, const TripletItem *__restrict__ items, int nitems, int *__restrict__ counter,
     const TripletTerm *__restrict__ terms, int lds_words, int *ctlw) {
-    if (__builtin_amdgcn_readfirstlane(ctlw[kCtlForm]) != kFormBits) return;
+    const int form = __builtin_amdgcn_readfirstlane(ctlw[kCtlForm]);
+    if (form != kFormBits && form != kFormSplit) return;
+    const int split = form == kFormSplit ? __builtin_amdgcn_readfirstlane(ctlw[kCtlSplit]) : 32;
+    if (form == kFormSplit) counter = ctlw + kCtlItems2;
     unsigned long long *ops = reinterpret_cast<unsigned long long *>(ctlw + kCtlOps);
     extern __shared__ uint4 mfma_lds[];
     __shared__ int s_item;
@@ -827,7 +830,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(6, 6
         since_flush += add;
         const int64_t nblk = T.nblk;
         const MetaV mAB = load_meta(T.meta, tile_index(A, B, nblk));
-        if (!mAB.bins) continue;
+        if (!mAB.bins || __builtin_popcount(mAB.bins) > split) continue;
         const global_u8_ptr gAB = (global_u8_ptr)(T.tiles + tile_index(A, B, nblk) * kTileBytes);
         uint32_t xrest = mAB.bins;
         while (xrest) {
@@ -994,7 +997,9 @@ __global__ __launch_bounds__(WAVES * 64) void r123_runs_kernel(const ElasticCtl 
                                                                   const TripletTerm *__restrict__ terms,
                                                                   int lds_words, const int *ctlw) {
     // Launched beside the bit-mask kernel; runs only when chosen.
-    if (__builtin_amdgcn_readfirstlane(ctlw[kCtlForm]) != kFormRuns) return;
+    const int form = __builtin_amdgcn_readfirstlane(ctlw[kCtlForm]);
+    if (form != kFormRuns && form != kFormSplit) return;
+    const int split = form == kFormSplit ? __builtin_amdgcn_readfirstlane(ctlw[kCtlSplit]) : -1;
     extern __shared__ uint32_t hist[];
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
@@ -1042,6 +1047,9 @@ __global__ __launch_bounds__(WAVES * 64) void r123_runs_kernel(const ElasticCtl 
         // with a bin.
         const uint8_t *tany = T.tile_any;
         if (tany && !tany[tile_index(A, B, nblk)]) continue;
+        // split form: the bit-mask kernel counts the items with few x bins
+        if (split >= 0 && T.meta && __builtin_popcount(load_meta(T.meta, tile_index(A, B, nblk)).bins) <= split)
+            continue;
         const uint32_t sy = (uint32_t)runs_sy(n), sz = (uint32_t)runs_sz(n);
         const uint32_t copy_base = (uint32_t)((lane % COPIES) * runs_copy_stride(n, COPIES)) * 4u;
         const uint32_t dump = copy_base + ((uint32_t)n * sz + (uint32_t)lane) * 4u;
@@ -1288,11 +1296,16 @@ hipError_t launch_pair_tiles(const PairJob &J, bool f32, hipStream_t st) {
 // mean number of bins m and the fraction f of their pairs with one.  The
 // bit-mask kernel's work grows with the bin pairs of a block triple (about
 // m^2), the run-merging kernel's with its triplets in range (about f), so the
-// bit-mask kernel is chosen when m <= max_bins * sqrt(f).  force >= 0 chooses
-// that form.  One 64-thread workgroup; writes ctlw[kCtlForm], and m and f
-// (float bits) to ctlw[kCtlBinsMean], ctlw[kCtlFill].
+// bit-mask kernel is chosen when m <= max_bins * sqrt(f).  Otherwise, when
+// m <= split_mean and split_bins >= 0, the split form: the items whose (A, B)
+// tile holds at most split_bins bins go to the bit-mask kernel, the others
+// to the run-merging kernel; else the run-merging kernel alone.  force >= 0
+// chooses that form.  One 64-thread workgroup; writes ctlw[kCtlForm],
+// ctlw[kCtlSplit], and m and f (float bits) to ctlw[kCtlBinsMean],
+// ctlw[kCtlFill].
 __global__ __launch_bounds__(64) void r123_choose_kernel(const TripletTerm *__restrict__ terms, int nterms,
-                                                         int *ctlw, float max_bins, int force) {
+                                                         int *ctlw, float max_bins, int force, int split_bins,
+                                                         float split_mean) {
     const int t = threadIdx.x;
     unsigned long long nb = 0, nt = 0, np = 0;
     for (int q = 0; q < nterms; ++q) {
@@ -1310,7 +1323,13 @@ __global__ __launch_bounds__(64) void r123_choose_kernel(const TripletTerm *__re
     if (t == 0) {
         const float mean = nt ? (float)((double)nb / (double)nt) : 0.f;
         const float fill = nt ? (float)((double)np / ((double)nt * kTileBytes)) : 0.f;
-        ctlw[kCtlForm] = force >= 0 ? force : (mean <= max_bins * sqrtf(fill) ? kFormBits : kFormRuns);
+        int form = force;
+        if (form < 0)
+            form = mean <= max_bins * sqrtf(fill)              ? kFormBits
+                   : (split_bins >= 0 && mean <= split_mean) ? kFormSplit
+                                                             : kFormRuns;
+        ctlw[kCtlForm] = form;
+        ctlw[kCtlSplit] = split_bins;
         ctlw[kCtlBinsMean] = __float_as_int(mean);
         ctlw[kCtlFill] = __float_as_int(fill);
     }
@@ -1362,8 +1381,14 @@ int r123_bits_lds_bytes(int n) {
 }
 
 hipError_t launch_r123_choose(const TripletTerm *terms, int nterms, int *counter, float max_bins, int force,
-                              hipStream_t st) {
-    hipLaunchKernelGGL(r123_choose_kernel, dim3(1), dim3(64), 0, st, terms, nterms, counter, max_bins, force);
+                              int split_bins, float split_mean, hipStream_t st) {
+    // the split form needs the matrix-core bit-mask kernel's item counter
+    if (!use_mfma()) {
+        split_bins = -1;
+        if (force == kFormSplit) force = kFormRuns;
+    }
+    hipLaunchKernelGGL(r123_choose_kernel, dim3(1), dim3(64), 0, st, terms, nterms, counter, max_bins, force,
+                       split_bins, split_mean);
     return hipGetLastError();
 }
 

@@ -662,11 +662,13 @@ struct TripletShape {
     int bwaves, bgrid;        // bit-mask kernel
     int form;
     float bits_max;
+    int split_bins;
+    float split_mean;
 };
 
 static TripletShape triplet_shape(tpcf_ctx *ctx, int max_n) {
     constexpr int kLdsBudget = 155 * 1024;
-    TripletShape s{8, 1, 0, 8, 0, ctx->shape.form, ctx->shape.bits_max};
+    TripletShape s{8, 1, 0, 8, 0, ctx->shape.form, ctx->shape.bits_max, ctx->shape.split_bins, ctx->shape.split_mean};
     for (int k : {8, 4, 2, 1})
         if (r123_lds_bytes(max_n, k) <= kLdsBudget) {
             s.copies = k;
@@ -771,7 +773,8 @@ static int launch_queue(tpcf_ctx *ctx, const QueueDev &D, const TripletShape &sh
     }
     // Both LDS kernels are launched; the one not chosen by the choose kernel
     // returns at once (either one gives the same histograms).
-    HIP_TRY(launch_r123_choose(D.terms, D.nterms, D.counter, sh.bits_max, sh.form, st));
+    HIP_TRY(launch_r123_choose(D.terms, D.nterms, D.counter, sh.bits_max, sh.form, sh.split_bins, sh.split_mean,
+                               st));
     ElasticCtl ctl{};
     int grid = sh.grid, bgrid = sh.bgrid;
     if (hook) {
@@ -1448,8 +1451,16 @@ int tpcf_ctx_create(int device, tpcf_ctx **out) {
     if (const char *env = std::getenv("TPCF_R123_FORM")) {
         if (!std::strcmp(env, "runs")) c->shape.form = kFormRuns;
         if (!std::strcmp(env, "bits")) c->shape.form = kFormBits;
+        if (!std::strcmp(env, "split")) c->shape.form = kFormSplit;
     }
     if (const char *env = std::getenv("TPCF_R123_BITS_MAX")) c->shape.bits_max = (float)std::atof(env);
+    if (const char *env = std::getenv("TPCF_R123_SPLIT")) {
+        int bins = 0;
+        float mean = 0.f;
+        const int got = std::sscanf(env, "%d,%f", &bins, &mean);
+        if (got >= 1) c->shape.split_bins = bins;
+        if (got >= 2) c->shape.split_mean = mean;
+    }
     *out = c;
     return TPCF_OK;
 }
@@ -1632,7 +1643,7 @@ struct tpcf_batch {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     QueueDev qd;
     int shift = 2;
-    TripletShape shape{8, 1, 0, 8, 0, -1, 7.0f};
+    TripletShape shape{8, 1, 0, 8, 0, -1, 7.0f, 5, 8.0f};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string tuning;  // autotuner log
@@ -1682,7 +1693,8 @@ static int batch_autotune(tpcf_batch *b) {
     const int n = b->qd.max_n;
     if (n > kMaxLdsR123Bins || !b->qd.nitems) return TPCF_OK;
     // The run-merging kernel's shapes, then the bit-mask kernel's, each
-    // timed with its form forced.
+    // timed with its form forced, then the split form with the best shape of
+    // each.
     const TripletShape base = b->shape;
     std::vector<TripletShape> cands;
     int kmax = 1;
@@ -1721,21 +1733,45 @@ static int batch_autotune(tpcf_batch *b) {
         c.form = kFormBits;
         cands.push_back(c);
     }
-    float best = 1e30f;
-    char line[160];
-    for (const auto &c : cands) {
+    float best = 1e30f, best_runs = 1e30f, best_bits = 1e30f;
+    TripletShape runs = base, bits = base;
+    char line[200];
+    auto consider = [&](const TripletShape &c) -> int {
         float ms = 0.f;
         if (int e = batch_time(b, c, ms)) return e;
         if (c.form == kFormBits)
             std::snprintf(line, sizeof line, "bit-mask waves=%d wg=%d: %.3f ms\n", c.bwaves, c.bgrid, ms);
+        else if (c.form == kFormSplit)
+            std::snprintf(line, sizeof line,
+                          "split (<= %d x bins: bit-mask) waves=%d copies=%d wg=%d, bit-mask waves=%d wg=%d: %.3f ms\n",
+                          c.split_bins, c.waves, c.copies, c.grid, c.bwaves, c.bgrid, ms);
         else
             std::snprintf(line, sizeof line, "run-merging waves=%d copies=%d wg=%d: %.3f ms\n", c.waves, c.copies,
                           c.grid, ms);
         b->tuning += line;
+        if (c.form == kFormRuns && ms < best_runs) {
+            best_runs = ms;
+            runs = c;
+        }
+        if (c.form == kFormBits && ms < best_bits) {
+            best_bits = ms;
+            bits = c;
+        }
         if (ms < best) {
             best = ms;
             b->shape = c;
         }
+        return TPCF_OK;
+    };
+    for (const auto &c : cands)
+        if (int e = consider(c)) return e;
+    // The split form with the best shape of each kernel.
+    if (best_runs < 1e30f && best_bits < 1e30f && base.split_bins >= 0) {
+        TripletShape c = runs;
+        c.bwaves = bits.bwaves;
+        c.bgrid = bits.bgrid;
+        c.form = kFormSplit;
+        if (int e = consider(c)) return e;
     }
     return TPCF_OK;
 }
@@ -1865,7 +1901,10 @@ const char *tpcf_batch_info(const tpcf_batch *b) {
     char head[160];
     std::snprintf(head, sizeof head,
                   "shape form=%s waves=%d copies=%d wg=%d bits_waves=%d bits_wg=%d graph=%d items=%d sqt_items=%d\n",
-                  b->shape.form == kFormBits ? "bit-mask" : (b->shape.form == kFormRuns ? "run-merging" : "auto"),
+                  b->shape.form == kFormBits    ? "bit-mask"
+                  : b->shape.form == kFormRuns  ? "run-merging"
+                  : b->shape.form == kFormSplit ? "split"
+                                                : "auto",
                   b->shape.waves, b->shape.copies, b->shape.grid, b->shape.bwaves, b->shape.bgrid, b->exec ? 1 : 0,
                   b->qd.nitems, (int)b->sqt.items.size());
     info = head + b->tuning;

@@ -86,10 +86,15 @@ struct LaunchShape {
     int copies = 1;    // privatized LDS histogram copies
     int per_cu = 0;    // workgroups per CU (0 = occupancy limit)
     // (r1,r2,r3) triplet kernel: -1 chosen per launch from the tiles' bins,
-    // 0 run-merging, 1 bit-mask; and the scale of the choice (the bit-mask
-    // kernel when the tiles' mean bins <= bits_max * sqrt(their fill)).
+    // 0 run-merging, 1 bit-mask, 2 split; and the scales of the choice (the
+    // bit-mask kernel when the tiles' mean bins <= bits_max * sqrt(their
+    // fill), else the split form when the mean <= split_mean: items whose
+    // (A, B) tile holds at most split_bins bins to the bit-mask kernel, the
+    // rest to the run-merging kernel; split_bins < 0: no split form).
     int form = -1;
     float bits_max = 7.0f;
+    int split_bins = 5;
+    float split_mean = 8.0f;
 };
 
 }  // namespace tpcf

@@ -79,7 +79,12 @@ __device__ inline bool elastic_leave(const ElasticCtl &c) {
 // chosen form, and the mean bins per tile and the fill of the tiles the
 // choice was made on.
 constexpr int kCtlItems = 0, kCtlOps = 2, kCtlForm = 4, kCtlBinsMean = 5, kCtlFill = 6;
-constexpr int kFormRuns = 0, kFormBits = 1;
+// Split form: items whose (A, B) tile holds at most ctlw[kCtlSplit] bins go
+// to the bit-mask kernel, which then takes its items from a counter of its
+// own (ctlw[kCtlItems2], cleared with the item counter); the others to the
+// run-merging kernel.
+constexpr int kCtlItems2 = 1, kCtlSplit = 7;
+constexpr int kFormRuns = 0, kFormBits = 1, kFormSplit = 2;
 
 // A unit of triplet work: blocks A <= B <= C, C in [c0, c1).
 struct TripletItem {
@@ -282,7 +287,7 @@ const void *r123_triplet_kernel_ptr(int waves, int copies, bool elastic = false)
 // The bit-mask triplet kernel, launched beside the run-merging one; a choose
 // kernel run first picks which of the two counts the launch.
 hipError_t launch_r123_choose(const TripletTerm *terms, int nterms, int *counter, float max_bins, int force,
-                              hipStream_t st);
+                              int split_bins, float split_mean, hipStream_t st);
 hipError_t launch_r123_bits(const ElasticCtl &ctl, const TripletItem *items, int nitems, int *counter,
                             const TripletTerm *terms, int max_n, int waves, int grid, hipStream_t st);
 int r123_bits_lds_bytes(int n);

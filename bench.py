@@ -50,7 +50,7 @@ VALU_PEAK = 256 * 64 * 2.4e9
 
 
 # stats.triplet_form: which (r1,r2,r3) LDS triplet kernel counted
-FORMS = {0: "run-merging", 1: "bit-mask"}
+FORMS = {0: "run-merging", 1: "bit-mask", 2: "split"}
 
 
 def valu_roofline(ops: float, seconds: float) -> dict:

@@ -123,11 +123,14 @@ typedef struct {
     double triplet_ms;         /* triplet kernel alone */
     double queue_ms;           /* serving: submit to admission */
     uint64_t mask_ops;         /* (r1,r2,r3) bit-mask triplet kernel: VALU lane-ops of its
-                                  cost model (whole launch; 0 for other kernels) */
+                                  cost model (whole launch, its items under the split form;
+                                  0 for other kernels) */
     double tile_bins_mean;     /* (r1,r2,r3): mean bins per pair tile with a bin */
     double tile_fill;          /* (r1,r2,r3): fraction of those tiles' pairs with a bin */
     int32_t triplet_form;      /* (r1,r2,r3) triplet kernel that counted: 0 run-merging,
-                                  1 bit-mask, -1 none of these */
+                                  1 bit-mask, 2 split (items whose (A,B) tile holds few bins
+                                  by the bit-mask kernel, the rest by the run-merging kernel),
+                                  -1 none of these */
     int32_t reserved;
 } tpcf_stats;
 

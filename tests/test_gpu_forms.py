@@ -1,9 +1,11 @@
-"""GPU: the two (r1,r2,r3) LDS triplet forms -- bit-mask (work per bin pair
-of a block triple) and run-merging (work per triplet) -- each forced through
-TPCF_R123_FORM, on geometries where either would be chosen: both equal the
-CPU oracle bit for bit, the stats name the form that counted, and the
-automatic choice follows the tiles' mean bins m and fill f (bit-mask when
-m <= 7.0 sqrt(f))."""
+"""GPU: the (r1,r2,r3) LDS triplet forms -- bit-mask (work per bin pair of a
+block triple), run-merging (work per triplet) and split (items whose (A,B)
+tile holds at most 5 bins by the bit-mask kernel, the rest by the
+run-merging kernel) -- each forced through TPCF_R123_FORM, on geometries
+where each would be chosen: all equal the CPU oracle bit for bit, the stats
+name the form that counted, and the automatic choice follows the tiles'
+mean bins m and fill f (bit-mask when m <= 7.0 sqrt(f), else split when
+m <= 8, else run-merging)."""
 import importlib
 import math
 import os
@@ -63,16 +65,17 @@ def test_both_forms_equal_the_oracle(orc, name, nd, nr, size, seed, spec, mode):
     make, cats = cube_job(nd, nr, size, seed)
     want = orc.clean_count(cats, tp.FOUR_TERMS, orc.R123, [orc.axis(spec.ax[0].n, spec.ax[0].lo, spec.ax[0].hi,
                                                                          "log" if spec.ax[0].scale else "lin")])
-    for form, code in (("runs", 0), ("bits", 1)):
+    for form, code in (("runs", 0), ("bits", 1), ("split", 2)):
         got, st = count_with_form(form, make, spec, mode)
         assert st.triplet_form == code
         assert np.array_equal(got, want), (name, form)
         if code == 1:
             assert st.mask_ops > 0
     got, st = count_with_form(None, make, spec, mode)
-    assert st.triplet_form in (0, 1) and st.tile_bins_mean > 0 and 0 < st.tile_fill <= 1
+    assert st.triplet_form in (0, 1, 2) and st.tile_bins_mean > 0 and 0 < st.tile_fill <= 1
     print(f"  auto: form {st.triplet_form}, mean bins {st.tile_bins_mean:.3f}, fill {st.tile_fill:.3f}", flush=True)
-    assert st.triplet_form == (1 if st.tile_bins_mean <= 7.0 * math.sqrt(st.tile_fill) else 0)
+    m, f = st.tile_bins_mean, st.tile_fill
+    assert st.triplet_form == (1 if m <= 7.0 * math.sqrt(f) else (2 if m <= 8.0 else 0))
     assert np.array_equal(got, want)
 
 
