@@ -772,9 +772,14 @@ static int launch_queue(tpcf_ctx *ctx, const QueueDev &D, const TripletShape &sh
         return TPCF_OK;
     }
     // Both LDS kernels are launched; the one not chosen by the choose kernel
-    // returns at once (either one gives the same histograms).
-    HIP_TRY(launch_r123_choose(D.terms, D.nterms, D.counter, sh.bits_max, sh.form, sh.split_bins, sh.split_mean,
-                               st));
+    // returns at once (either one gives the same histograms), and under the
+    // split form each counts its own items.  Elastic (serving) launches use
+    // one form only: their workgroups leave by one active count per launch,
+    // and the last active one stays to drain the queue, which holds for one
+    // queue but not for the split form's two.
+    const int split_bins = hook ? -1 : sh.split_bins;
+    const int form = hook && sh.form == kFormSplit ? kFormRuns : sh.form;
+    HIP_TRY(launch_r123_choose(D.terms, D.nterms, D.counter, sh.bits_max, form, split_bins, sh.split_mean, st));
     ElasticCtl ctl{};
     int grid = sh.grid, bgrid = sh.bgrid;
     if (hook) {
@@ -1061,7 +1066,7 @@ static int read_ctl(const int *ctl, tpcf_stats *s) {
     float mean, fill;
     std::memcpy(&mean, w + kCtlBinsMean, sizeof mean);
     std::memcpy(&fill, w + kCtlFill, sizeof fill);
-    s->mask_ops = w[kCtlForm] == kFormBits ? ops : 0;
+    s->mask_ops = w[kCtlForm] == kFormBits || w[kCtlForm] == kFormSplit ? ops : 0;
     s->triplet_form = w[kCtlForm];
     s->tile_bins_mean = mean;
     s->tile_fill = fill;

@@ -124,3 +124,27 @@ def test_short_jobs_overtake_a_running_long_job():
         for (_, out), (job, sp) in zip(tickets, work):
             alone, _ = ctx.count(job, sp)
             assert np.array_equal(out, alone)
+
+
+def test_split_form_asked_for_while_serving(orc, monkeypatch):
+    """TPCF_R123_FORM=split: count() runs the split form (items by the
+    (A,B) tile's bins over both kernels); the elastic serving launches of the
+    same context run one form instead (their workgroups leave by one active
+    count, which suits one queue); every result equals the oracle."""
+    print("case serving: split form asked for", flush=True)
+    monkeypatch.setenv("TPCF_R123_FORM", "split")
+    d = synth.neyman_scott(900, nclust=20, sigma=2.0, size=120.0, seed=21)
+    r = synth.uniform_cube(800, 120.0, seed=22)
+    spec = tp.r123_spec(20, 1.0, 60.0, "log")
+    want = orc.clean_count([d, r], tp.FOUR_TERMS, orc.R123, [orc.axis(20, 1.0, 60.0, "log")])
+    with tp.Context(0) as ctx:
+        cd, cr = ctx.catalog(d), ctx.catalog(r)
+        job = tp.four_term_job(cd, cr)
+        got, st = ctx.count(job, spec)
+        assert st.triplet_form == 2
+        assert np.array_equal(got, want)
+        tickets = [ctx.submit(job, spec) for _ in range(3)]
+        for t, out in tickets:
+            st = ctx.wait(t)
+            assert st.triplet_form == 0
+            assert np.array_equal(out, want)
