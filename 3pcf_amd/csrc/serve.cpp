@@ -21,11 +21,37 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "tpcf_host.h"
 
 namespace tpcf {
+
+// TPCF_SERVE_LOG=1: one stderr line per scheduling event of the serving
+// workers (batch taken, share claimed, launch grid, share changes, waits
+// for extra launches, batch done), with seconds since the first line.
+static bool serve_log_on() {
+    static const bool on = [] {
+        const char *e = std::getenv("TPCF_SERVE_LOG");
+        return e && *e && std::strcmp(e, "0") != 0;
+    }();
+    return on;
+}
+__attribute__((format(printf, 1, 2))) static void serve_log(const char *fmt, ...) {
+    if (!serve_log_on()) return;
+    static const auto t0 = std::chrono::steady_clock::now();
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    std::fprintf(stderr, "[serve %9.3f] %s\n",
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), buf);
+    std::fflush(stderr);
+}
 
 struct Ticket {
     tpcf_job job{};
@@ -180,13 +206,18 @@ struct ElasticScheduler {
             if (w == skip || !s.launched) continue;
             const int t = target(w, cus, 1 << 30);
             if (t < s.share) {
+                serve_log("slot %d shrinks %d -> %d workgroups", w, s.share, t);
                 set_desired(w, t);
                 s.share = t;
                 continue;
             }
             if (t == s.share) continue;
             hipStream_t h = idle_helper();
-            if (!h) continue;
+            if (!h) {
+                serve_log("slot %d cannot grow %d -> %d: no idle helper stream (%zu)", w, s.share, t,
+                          helpers.size());
+                continue;
+            }
             if (s.next_ext == s.ext.size()) {
                 hipEvent_t e = nullptr;
                 if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
@@ -205,6 +236,7 @@ struct ElasticScheduler {
                 (void)hipStreamSynchronize(h);
                 continue;
             }
+            serve_log("slot %d grows %d -> %d workgroups (extra launch %zu)", w, s.share, t, s.next_ext);
             ++s.next_ext;
             s.share = t;
         }
@@ -220,6 +252,7 @@ struct WorkerShare final : ElasticHook {
         std::lock_guard<std::mutex> g(S.mu);
         S.slots[w].claimed = true;
         S.slots[w].work = std::max(work, 1e6);
+        serve_log("slot %d claims (%.3g triplets)", w, work);
         S.rebalance(-1);
     }
     int control(int per_cu, int nitems, Relaunch relaunch, hipStream_t st, ElasticCtl *ctl, int *grid) override {
@@ -241,6 +274,7 @@ struct WorkerShare final : ElasticHook {
         s.share = t;
         s.relaunch = std::move(relaunch);
         *ctl = S.ctl(w);
+        serve_log("slot %d launches %d of %d workgroups over %d items (%d per CU)", w, t, *grid, nitems, s.per_cu);
         *grid = t;
         S.rebalance(w);  // the others, now that this launch's demand is known
         return TPCF_OK;
@@ -262,12 +296,14 @@ struct WorkerShare final : ElasticHook {
                     s.share = 0;
                     s.next_ext = 0;
                     s.relaunch = nullptr;
+                    serve_log("slot %d done", w);
                     S.rebalance(-1);
                     return;
                 }
                 wait.assign(s.ext.begin() + waited, s.ext.begin() + s.next_ext);
                 waited = s.next_ext;
             }
+            serve_log("slot %d waits for %zu extra launches", w, wait.size());
             for (hipEvent_t e : wait) (void)hipEventSynchronize(e);
         }
     }
@@ -293,7 +329,13 @@ static void worker_main(tpcf_ctx *ctx, int w) {
     hipStream_t st = nullptr;
     (void)queue_stream(ctx->num_cu, &st);
     std::unique_ptr<Workspace, WorkspaceDeleter> ws(new_workspace());
-    workspace_order(ws.get(), st);
+    // The worker's own pool (no pool: plain allocations, which wait for
+    // the device at a free but never for another worker's stream alone).
+    hipMemPool_t pool = nullptr;
+    if (worker_pool_create(ctx->device, &pool) == TPCF_OK)
+        workspace_order(ws.get(), st, pool);
+    else
+        clear_error();
     WorkerShare share(*ctx->elastic, w);
     for (;;) {
         std::vector<std::shared_ptr<Ticket>> batch;
@@ -364,8 +406,13 @@ static void worker_main(tpcf_ctx *ctx, int w) {
                     break;
                 }
                 if (!ctx->pending.empty() && cap > 0) {
-                    // Idle while queued jobs do not fit: give the memory back.
+                    // Idle while queued jobs do not fit: give the memory back
+                    // (the pool hands it to the device at the synchronization;
+                    // the stream is idle here).
+                    serve_log("worker %d gives its workspace back: %zu queued jobs do not fit", w,
+                              ctx->pending.size());
                     workspace_release(ws.get());
+                    if (st) (void)hipStreamSynchronize(st);
                     ctx->held[w] = 0;
                     account(ctx);
                     ctx->cv.notify_all();
@@ -374,6 +421,13 @@ static void worker_main(tpcf_ctx *ctx, int w) {
                 ctx->cv.wait(lk);
             }
             if (batch.empty()) break;
+        }
+        {
+            uint64_t work = 0;
+            for (auto &t : batch) work += t->work;
+            serve_log("worker %d takes %zu jobs (%s, %.3g triplets, first ticket %lld)%s", w, batch.size(),
+                      batch[0]->r123 ? "r123" : "sqt/jbj", (double)work, (long long)ids[0],
+                      release_first ? ", workspace released first" : "");
         }
         if (release_first) workspace_release(ws.get());
         std::vector<int> rcs(batch.size(), TPCF_OK);
@@ -412,6 +466,7 @@ static void worker_main(tpcf_ctx *ctx, int w) {
                 batch[k]->stats.queue_ms =
                     std::chrono::duration<double, std::milli>(batch[k]->t_admit - batch[k]->t_submit).count();
             }
+            serve_log("worker %d batch done (rc %d)", w, rcs[0]);
             // what the workspace really holds now (at most what was admitted)
             ctx->held[w] = workspace_held(ws.get());
             account(ctx);
@@ -428,6 +483,7 @@ static void worker_main(tpcf_ctx *ctx, int w) {
         (void)hipStreamSynchronize(st);  // the workspace's stream-ordered frees
         (void)hipStreamDestroy(st);
     }
+    if (pool) (void)hipMemPoolDestroy(pool);
 }
 
 }  // namespace tpcf

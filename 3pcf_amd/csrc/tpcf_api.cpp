@@ -66,7 +66,7 @@ int DevBuf::reserve(size_t need) {
         p = nullptr;
         bytes = 0;
     }
-    hipError_t e = ordered ? hipMallocAsync(&p, need, ordered) : hipMalloc(&p, need);
+    hipError_t e = ordered ? hipMallocFromPoolAsync(&p, need, pool, ordered) : hipMalloc(&p, need);
     if (e != hipSuccess) {
         p = nullptr;
         return set_error(TPCF_ENOMEM, "device allocation of %zu bytes failed: %s", need, hipGetErrorString(e));
@@ -644,8 +644,30 @@ struct Workspace {
     }
 };
 
-void workspace_order(Workspace *ws, hipStream_t st) {
-    ws->region.ordered = ws->queue.ordered = ws->sqt_queue.ordered = st;
+void workspace_order(Workspace *ws, hipStream_t st, hipMemPool_t pool) {
+    for (DevBuf *b : {&ws->region, &ws->queue, &ws->sqt_queue}) {
+        b->ordered = st;
+        b->pool = pool;
+    }
+}
+
+int worker_pool_create(int device, hipMemPool_t *pool) {
+    *pool = nullptr;
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device;
+    HIP_TRY(hipMemPoolCreate(pool, &props));
+    int off = 0;
+    uint64_t keep = 0;
+    if (hipError_t e = hipMemPoolSetAttribute(*pool, hipMemPoolReuseAllowInternalDependencies, &off);
+        e != hipSuccess || (e = hipMemPoolSetAttribute(*pool, hipMemPoolAttrReleaseThreshold, &keep)) != hipSuccess) {
+        (void)hipMemPoolDestroy(*pool);
+        *pool = nullptr;
+        return set_error(TPCF_EDEVICE, "cannot configure a worker's memory pool: %s", hipGetErrorString(e));
+    }
+    return TPCF_OK;
 }
 
 // Launch shapes of the LDS triplet kernels for bins up to max_n.  The
@@ -1053,14 +1075,20 @@ static int launch_sqt_fast_jobs(tpcf_ctx *ctx, Workspace &ws, const std::vector<
 // The (r1,r2,r3) LDS launch's control block (after the launch has
 // finished): the form that counted, the mean bins per tile it was chosen on
 // and the bit-mask kernel's modelled VALU lane-ops.  No launch: form -1.
-static int read_ctl(const int *ctl, tpcf_stats *s) {
+// Read on the launch's own stream: a null-stream hipMemcpy would also wait
+// for every kernel on the other blocking streams (the serving workers'
+// CU-masked streams are such streams), so a finished batch would wait for
+// the longest batch running beside it, and later launches on those streams
+// for that copy.
+static int read_ctl(const int *ctl, tpcf_stats *s, hipStream_t st) {
     s->mask_ops = 0;
     s->triplet_form = -1;
     s->tile_bins_mean = 0.0;
     s->tile_fill = 0.0;
     if (!ctl) return TPCF_OK;
     int w[8];
-    HIP_TRY(hipMemcpy(w, ctl, sizeof w, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(w, ctl, sizeof w, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     unsigned long long ops;
     std::memcpy(&ops, w + kCtlOps, sizeof ops);
     float mean, fill;
@@ -1135,7 +1163,7 @@ static int run_job(tpcf_ctx *ctx, Workspace &ws, const tpcf_job &job, const tpcf
         stats->device_ms = ms;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (int e = stage_times(ws, stats)) return e;
-        if (int e = read_ctl(ws.ctl, stats)) return e;
+        if (int e = read_ctl(ws.ctl, stats, st)) return e;
     }
     return TPCF_OK;
 }
@@ -1293,7 +1321,7 @@ int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_bin
     tot.device_ms = ms;
     tot.wall_ms = wall;
     if (int e = stage_times(ws, &tot)) return e;
-    if (int e = read_ctl(ws.ctl, &tot)) return e;
+    if (int e = read_ctl(ws.ctl, &tot, st)) return e;
     for (int j = 0; j < njobs && per; ++j) {
         per[j].pair_ms = tot.pair_ms;
         per[j].triplet_ms = tot.triplet_ms;
@@ -1893,7 +1921,7 @@ int tpcf_batch_run(tpcf_batch *b, uint64_t *const *hist, tpcf_stats *stats) {
         }
         s.device_ms = ms;
         s.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        if (int e = read_ctl(b->qd.nitems && b->qd.max_n <= kMaxLdsR123Bins ? b->qd.counter : nullptr, &s))
+        if (int e = read_ctl(b->qd.nitems && b->qd.max_n <= kMaxLdsR123Bins ? b->qd.counter : nullptr, &s, b->st))
             return e;
         *stats = s;
     }

@@ -48,10 +48,11 @@ struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
     bool fixed = false;
-    // Set: allocations are stream-ordered on this stream (hipMallocAsync /
-    // hipFreeAsync), so growing or releasing the buffer does not wait for
-    // every kernel on the device, as hipFree does.
+    // Set: allocations are stream-ordered on this stream (hipFreeAsync and
+    // hipMallocFromPoolAsync from `pool`), so growing or releasing the
+    // buffer does not wait for every kernel on the device, as hipFree does.
     hipStream_t ordered = nullptr;
+    hipMemPool_t pool = nullptr;
     ~DevBuf();
     int reserve(size_t need);  // grow-only
     void release();            // free the memory (owned buffers only)
@@ -172,8 +173,16 @@ struct ElasticHook {
 // of one family ((r1,r2,r3), or (s,q,theta) and JBJ).
 // Make the workspace's buffers stream-ordered on st (a serving worker's
 // stream: a worker growing its workspace then does not wait for the other
-// workers' batches).
-void workspace_order(Workspace *ws, hipStream_t st);
+// workers' batches), allocated from pool, a pool that only st allocates
+// from and frees into.  A pool shared between workers lets an allocation
+// reuse a block another worker freed by making the allocating stream wait
+// for that worker's stream (the runtime's internal dependencies): the new
+// batch then waits for a batch whose elastic launch the scheduler has just
+// shrunk to its floor for it.
+void workspace_order(Workspace *ws, hipStream_t st, hipMemPool_t pool);
+// A stream-ordered device pool for one serving worker: no reuse through
+// inserted dependencies, memory handed back at each synchronization.
+int worker_pool_create(int device, hipMemPool_t *pool);
 int run_batch(tpcf_ctx *ctx, Workspace &ws, const tpcf_job *jobs, const tpcf_binspec *specs, int njobs, int mode,
               uint64_t *const *hist, tpcf_stats *per, tpcf_stats *total, hipStream_t st,
               ElasticHook *hook = nullptr);
