@@ -481,9 +481,10 @@ static void worker_main(tpcf_ctx *ctx, int w) {
     }
     if (st) {
         (void)hipStreamSynchronize(st);  // the workspace's stream-ordered frees
-        (void)hipStreamDestroy(st);
     }
+    // the pool before the stream its frees were ordered on
     if (pool) (void)hipMemPoolDestroy(pool);
+    if (st) (void)hipStreamDestroy(st);
 }
 
 }  // namespace tpcf
