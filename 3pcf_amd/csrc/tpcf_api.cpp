@@ -528,7 +528,9 @@ static int plan_job(const tpcf_job &job, const tpcf_binspec &spec, int mode, Job
 // stream has been synchronized, builds the reach rows from them.
 static int stage_job(const tpcf_job &job, const JobPlan &P, char *base, hipStream_t st) {
     double *u64 = reinterpret_cast<double *>(base + P.o_u64);
-    HIP_TRY(hipMemsetAsync(u64, 0, sizeof(double) * 3 * P.L.npad, st));
+    // the padding between segments reads as zeros (a fill kernel, like every
+    // other clear of a job's region)
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(u64), 6 * P.L.npad, 0u, st));
     for (int s = 0; s < job.nseg; ++s) {
         const tpcf_catalog *c = job.seg[s];
         const int64_t m = P.L.seg_n[s];
