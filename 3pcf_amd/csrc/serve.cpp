@@ -407,15 +407,20 @@ static void worker_main(tpcf_ctx *ctx, int w) {
                 }
                 if (!ctx->pending.empty() && cap > 0) {
                     // Idle while queued jobs do not fit: give the memory back
-                    // (the pool hands it to the device at the synchronization;
-                    // the stream is idle here).
+                    // (the pool hands it to the device at the synchronization,
+                    // made without the lock: the stream is idle, but a
+                    // caller's null-stream work would hold it).
                     serve_log("worker %d gives its workspace back: %zu queued jobs do not fit", w,
                               ctx->pending.size());
                     workspace_release(ws.get());
-                    if (st) (void)hipStreamSynchronize(st);
                     ctx->held[w] = 0;
                     account(ctx);
                     ctx->cv.notify_all();
+                    if (st) {
+                        lk.unlock();
+                        (void)hipStreamSynchronize(st);
+                        lk.lock();
+                    }
                     continue;
                 }
                 ctx->cv.wait(lk);
